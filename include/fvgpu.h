@@ -1,0 +1,278 @@
+/*
+ * fvgpu.h -- C-ABI of the MI355X-native FVCOM mode-split time step.
+ *
+ * Two shared libraries sit behind this header:
+ *
+ *   libfvgrid.so  host C++: seeded synthetic cases, the grid topology and
+ *                 metrics of the reference's GRID_METRICS
+ *                 (FVCOM_source/grid_metrics.F:55-134) and the cold-start
+ *                 state (FVCOM_source/mod_startup.F:1062-1181, 1332-1408).
+ *   libfvgpu.so   HIP/CDNA4: the device-resident state and the time step
+ *                 that replaces INTERNAL_STEP (FVCOM_source/internal_step.F:29)
+ *                 and EXTERNAL_STEP (FVCOM_source/external_step.F:30).
+ *
+ * Array conventions follow the reference's ALL_VARS module
+ * (FVCOM_source/mod_main.F:1093-2165): every entity dimension keeps the
+ * dummy entry 0 ("outside", NBE=0 / IEC(:,2)=0 point there), indices stored
+ * in the arrays are 1-based, 3-D arrays are A(0:MT,1:KB) column-major
+ * (entity index fastest, then layer), multi-component arrays such as
+ * NV(0:NT,3) or A1U(0:NT,4) are component-major.  Ragged rings NBSN/NBVE
+ * (mod_main.F:1783,1791) are stored as CSR in the reference's ring order.
+ *
+ * Every entry point returns 0 on success and a non-zero status otherwise;
+ * fv_last_error() returns the message of the last failure on this thread.
+ */
+#ifndef FVGPU_H
+#define FVGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Run parameters (namelist values of NML_INTEGRATION / NML_PHYSICS /  */
+/* NML_RIVER_TYPE / NML_OPEN_BOUNDARY_CONTROL, FVCOM_source/mod_main.F */
+/* :121-942).                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct fv_params {
+  double  dte;                 /* EXTSTEP_SECONDS                         */
+  int32_t isplit;              /* ISPLIT, DTI = ISPLIT*DTE                */
+  int32_t iramp;               /* IRAMP (0: RAMP = 1)                     */
+  double  umol;                /* VERTICAL_MIXING_COEFFICIENT             */
+  double  vprnu;               /* VERTICAL_PRANDTL_NUMBER                 */
+  double  hprnu;               /* HORIZONTAL_PRANDTL_NUMBER               */
+  double  cbcmin;              /* BOTTOM_ROUGHNESS_MINIMUM                */
+  double  min_depth;           /* MIN_DEPTH (DEPTH_CHECK)                 */
+  int32_t horizontal_closure;  /* HORIZONTAL_MIXING_TYPE 'closure'=1      */
+  int32_t vertical_closure;    /* VERTICAL_MIXING_TYPE 'closure'=1        */
+  int32_t scalar_positivity;   /* SCALAR_POSITIVITY_CONTROL (FCT on)      */
+  int32_t noflux_bot;          /* NOFLUX_BOT_CONDITION                    */
+  int32_t barotropic;          /* BAROTROPIC                              */
+  int32_t temperature_active;  /* TEMPERATURE_ACTIVE                      */
+  int32_t salinity_active;     /* SALINITY_ACTIVE                         */
+  int32_t wind_on;             /* WIND_ON, WIND_TYPE='speed'              */
+  int32_t river_ts_specified;  /* RIVER_TS_SETTING == 'specified'         */
+  int32_t type_tsobc;          /* OBC_TS_TYPE 1..3                        */
+  int32_t obc_on;              /* OBC_ON                                  */
+  int32_t obc_elevation_on;    /* OBC_ELEVATION_FORCING_ON (spectral)     */
+  int64_t iint0;               /* IINT before the first step              */
+  double  time0;               /* IntTime at IINT0, seconds after SpecTime*/
+} fv_params;
+
+/* Extents of the model arrays (see fv_extent_len in libfvgrid). */
+enum fv_extent {
+  FVX_M1 = 0,   /* 0:M                       */
+  FVX_M2,       /* CSR offsets over nodes     */
+  FVX_N1,       /* 0:N                        */
+  FVX_N1x3,     /* (0:N,3)                    */
+  FVX_N1x4,     /* (0:N,4)                    */
+  FVX_NE1,      /* 0:NE                       */
+  FVX_NE1x2,    /* (0:NE,2)                   */
+  FVX_CV1,      /* 0:NCV                      */
+  FVX_CV1x2,    /* (0:NCV,2)                  */
+  FVX_M1K,      /* (0:M,KB)                   */
+  FVX_N1K,      /* (0:N,KB)                   */
+  FVX_RSN,      /* NBSN ring entries          */
+  FVX_RVE,      /* NBVE ring entries          */
+  FVX_OBC,      /* IOBCN                      */
+  FVX_OBCx2,    /* (IOBCN,2)                  */
+  FVX_OBCxT,    /* (IOBCN,NTIDE)              */
+  FVX_TIDE,     /* NTIDE                      */
+  FVX_RIV,      /* NUMQBC                     */
+  FVX_RIVxK,    /* (NUMQBC,KBM1)              */
+  FVX_COUNT
+};
+
+/*
+ * X(ctype, NAME, extent): the grid, metric, forcing and state arrays of a
+ * model, named as in the reference's ALL_VARS.  State arrays hold the
+ * initial (or snapshot) values on the host.
+ */
+#define FV_GRID_FIELDS(X)                                                   \
+  /* nodes */                                                               \
+  X(double,  vx,        FVX_M1)   X(double,  vy,        FVX_M1)             \
+  X(double,  h,         FVX_M1)   X(double,  art1,      FVX_M1)             \
+  X(double,  art2,      FVX_M1)   X(double,  grav_n,    FVX_M1)             \
+  X(double,  nn_hvc,    FVX_M1)   X(double,  sita_gd,   FVX_M1)             \
+  X(double,  phpn,      FVX_M1)   X(int32_t, isonb,     FVX_M1)             \
+  X(int32_t, ntsn,      FVX_M1)   X(int32_t, ntve,      FVX_M1)             \
+  X(int32_t, nbsn_off,  FVX_M2)   X(int32_t, nbve_off,  FVX_M2)             \
+  /* node rings */                                                          \
+  X(int32_t, nbsn,      FVX_RSN)  X(double,  dltxtrie,  FVX_RSN)            \
+  X(double,  dltytrie,  FVX_RSN)                                            \
+  X(int32_t, nbve,      FVX_RVE)  X(int32_t, nbvt,      FVX_RVE)            \
+  X(double,  dltxecec,  FVX_RVE)  X(double,  dltyecec,  FVX_RVE)            \
+  X(double,  dltxnec,   FVX_RVE)  X(double,  dltynec,   FVX_RVE)            \
+  /* elements */                                                            \
+  X(double,  xc,        FVX_N1)   X(double,  yc,        FVX_N1)             \
+  X(double,  art,       FVX_N1)   X(double,  h1,        FVX_N1)             \
+  X(double,  cor,       FVX_N1)   X(double,  cc_hvc,    FVX_N1)             \
+  X(double,  cc_sponge, FVX_N1)   X(double,  cc_z0b,    FVX_N1)             \
+  X(double,  grav_e,    FVX_N1)   X(double,  epor,      FVX_N1)             \
+  X(double,  alpha,     FVX_N1)   X(int32_t, isbce,     FVX_N1)             \
+  X(int32_t, iucp,      FVX_N1)   X(int32_t, nv,        FVX_N1x3)           \
+  X(int32_t, nbe,       FVX_N1x3) X(double,  a1u,       FVX_N1x4)           \
+  X(double,  a2u,       FVX_N1x4) X(double,  awx,       FVX_N1x3)           \
+  X(double,  awy,       FVX_N1x3)                                           \
+  /* edges */                                                               \
+  X(int32_t, iec,       FVX_NE1x2) X(int32_t, ienode,   FVX_NE1x2)          \
+  X(int32_t, isbc,      FVX_NE1)  X(double,  xijc,      FVX_NE1)            \
+  X(double,  yijc,      FVX_NE1)  X(double,  dltxc,     FVX_NE1)            \
+  X(double,  dltyc,     FVX_NE1)                                            \
+  /* control-volume sub-edges */                                            \
+  X(int32_t, niec,      FVX_CV1x2) X(int32_t, ntrg,     FVX_CV1)            \
+  X(double,  dltxe,     FVX_CV1)  X(double,  dltye,     FVX_CV1)            \
+  X(double,  dltxncve,  FVX_CV1x2) X(double, dltyncve,  FVX_CV1x2)          \
+  /* sigma coordinates */                                                   \
+  X(double,  z,         FVX_M1K)  X(double,  zz,        FVX_M1K)            \
+  X(double,  dz,        FVX_M1K)  X(double,  dzz,       FVX_M1K)            \
+  X(double,  z1,        FVX_N1K)  X(double,  zz1,       FVX_N1K)            \
+  X(double,  dz1,       FVX_N1K)  X(double,  dzz1,      FVX_N1K)            \
+  /* open boundary (all nodes of type 1: elevation specified) */            \
+  X(int32_t, i_obc_n,   FVX_OBC)  X(int32_t, next_obc,  FVX_OBC)            \
+  X(int32_t, nadjc_obc, FVX_OBC)  X(int32_t, adjc_obc,  FVX_OBCx2)          \
+  X(double,  fluxf_obc, FVX_OBCx2) X(double, dltn_obc,  FVX_OBC)            \
+  X(double,  emean,     FVX_OBC)  X(double,  apt,       FVX_OBCxT)          \
+  X(double,  phai,      FVX_OBCxT) X(double, period,    FVX_TIDE)           \
+  /* rivers (RIVER_INFLOW_LOCATION = 'node') */                             \
+  X(int32_t, inodeq,    FVX_RIV)  X(double,  qarea,     FVX_RIV)            \
+  X(double,  angleq,    FVX_RIV)  X(double,  qdis0,     FVX_RIV)            \
+  X(double,  tdis,      FVX_RIV)  X(double,  sdis,      FVX_RIV)            \
+  X(double,  vqdist,    FVX_RIVxK)                                          \
+  /* constant wind speed at elements (WIND_TYPE = 'speed') */               \
+  X(double,  uuwind,    FVX_N1)   X(double,  vvwind,    FVX_N1)
+
+/* Prognostic / persistent state carried between internal steps. */
+#define FV_STATE_FIELDS(X)                                                  \
+  X(double, el,   FVX_M1)  X(double, et,   FVX_M1)                          \
+  X(double, d,    FVX_M1)  X(double, dt,   FVX_M1)                          \
+  X(double, ua,   FVX_N1)  X(double, va,   FVX_N1)                          \
+  X(double, el1,  FVX_N1)  X(double, et1,  FVX_N1)                          \
+  X(double, d1,   FVX_N1)  X(double, dt1,  FVX_N1)                          \
+  X(double, u,    FVX_N1K) X(double, v,    FVX_N1K)                         \
+  X(double, w,    FVX_N1K) X(double, ww,   FVX_N1K)                         \
+  X(double, t,    FVX_N1K) X(double, s,    FVX_N1K)                         \
+  X(double, rho,  FVX_N1K) X(double, rmean, FVX_N1K)                        \
+  X(double, km1,  FVX_N1K)                                                  \
+  X(double, t1,   FVX_M1K) X(double, s1,   FVX_M1K)                         \
+  X(double, rho1, FVX_M1K) X(double, rmean1, FVX_M1K)                       \
+  X(double, tmean1, FVX_M1K) X(double, smean1, FVX_M1K)                     \
+  X(double, q2,   FVX_M1K) X(double, q2l,  FVX_M1K)                         \
+  X(double, l,    FVX_M1K) X(double, km,   FVX_M1K)                         \
+  X(double, kh,   FVX_M1K) X(double, kq,   FVX_M1K)                         \
+  X(double, wts,  FVX_M1K)
+
+#define FV_ALL_FIELDS(X) FV_GRID_FIELDS(X) FV_STATE_FIELDS(X)
+
+/* Host model: grid + metrics + forcing + state, owned by libfvgrid. */
+typedef struct fv_model {
+  int32_t m, n, ne, ncv, ncv_i, kb, kbm1, kbm2;
+  int32_t iobcn, ntide, numqbc, nring_sn, nring_ve;
+  fv_params p;
+#define FV_DECL_FIELD(T, NAME, EXT) T* NAME;
+  FV_ALL_FIELDS(FV_DECL_FIELD)
+#undef FV_DECL_FIELD
+} fv_model;
+
+/* Synthetic case description (the seeded configurations). */
+typedef struct fv_case {
+  int32_t nx, ny;            /* structured node counts along x and y      */
+  double  dx;                /* node spacing (m)                          */
+  double  jitter;            /* uniform node displacement, fraction of dx */
+  int32_t kb;                /* sigma levels (KB)                         */
+  int32_t depth_kind;        /* 0: linear 5 m (west) .. 20 m (east);
+                                1: 5+195*(x/Lx)^1.5 plus seamounts        */
+  int32_t n_seamounts;
+  int32_t ntide;             /* 1: M2; 3: M2,S2,K1 (seeded phases)        */
+  double  tide_amp[3];
+  int32_t river;             /* 1: one river at the west-end node         */
+  double  river_q, river_t, river_s;
+  int32_t wind;              /* 1: seeded Gaussian random field           */
+  double  wind_mean, wind_std, wind_corr;
+  int32_t ts_kind;           /* 0: T=15, S=30*x/Lx ;
+                                1: T=20-10*(z/h)... S=33+2*(z/h)          */
+  double  el_pert_std, el_pert_corr;  /* initial elevation perturbation   */
+  double  coriolis;          /* f-plane Coriolis parameter (s^-1)         */
+  uint64_t seed;
+  fv_params p;
+} fv_case;
+
+/* ------------------------------- libfvgrid ------------------------------ */
+const char* fv_last_error(void);
+
+/* Fill *c with the defaults of a named configuration:
+ *   0 estuary channel 81x21 @500 m, KB=11 (3200 elements)
+ *   1 jittered 1001x501 @200 m, KB=41, seamounts, M2/S2/K1, wind
+ *   3 ensemble member 501x251 @200 m, KB=21                              */
+int  fv_case_defaults(int32_t config, fv_case* c);
+/* Build grid, metrics, forcing and cold-start state of a synthetic case. */
+int  fv_model_build(const fv_case* c, fv_model** out);
+/* Build from an explicit mesh: NV(1:N,1:3) clockwise, 1-based, row-major
+ * triples; OBC node list 1-based.  Depth h(1:M).  Forcing arrays zero.   */
+int  fv_model_from_mesh(int32_t m, int32_t n, const double* x, const double* y,
+                        const int32_t* nv_rows, const double* h, int32_t kb,
+                        int32_t iobcn, const int32_t* obc_nodes,
+                        const fv_params* p, fv_model** out);
+void fv_model_free(fv_model* mdl);
+int64_t fv_extent_len(const fv_model* mdl, int32_t extent);
+/* Look up one array by its ALL_VARS name (lower case).  *dtype: 0 double,
+ * 1 int32.  Returns non-zero if the name is unknown.                     */
+int  fv_model_field(const fv_model* mdl, const char* name, void** ptr,
+                    int64_t* len, int32_t* dtype);
+/* Number of named fields, and the i-th name (for iteration).             */
+int32_t     fv_model_nfields(void);
+const char* fv_model_field_name(int32_t i);
+/* Cold start: EL=0, rest, DENS2 density, RMEAN/TMEAN/SMEAN, default
+ * turbulence (mod_startup.F:1062-1181, 1332-1408).                       */
+int  fv_model_cold_start(fv_model* mdl);
+
+/* ------------------------------- libfvgpu ------------------------------- */
+typedef struct fvgpu_handle fvgpu_handle;
+
+/* Diagnostics gathered on the device (SHUTDOWN_CHECK, DEPTH_CHECK and the
+ * FP64 budgets; FVCOM_source/mod_utils.F:320-432, depth_check.F).        */
+typedef struct fv_diag {
+  int32_t nonfinite;      /* 1 if any of EL, UA, VA, U, V, T1, S1 is not finite */
+  int32_t depth_fail;     /* 1 if MIN(D) < MIN_DEPTH was seen                   */
+  double  min_depth;      /* minimum total depth D over nodes                    */
+  double  volume;         /* sum ART1*D over nodes (m^3), compensated sum        */
+  double  salt;           /* sum ART1*D*DZ*S1 over node layers (psu m^3)         */
+  double  heat;           /* sum ART1*D*DZ*T1 over node layers (degC m^3)        */
+  int64_t iint;           /* internal steps completed (IINT)                     */
+} fv_diag;
+
+/* Precision modes. */
+enum fv_precision { FV_FP64 = 0, FV_MIXED = 1 };
+
+/* Upload grid, metrics, forcing and state; the device owns the copies.
+ * Replaces ALLOC_VARS + GRID_METRICS + STARTUP for the device state.     */
+int  fvgpu_create(const fv_model* mdl, int32_t precision, fvgpu_handle** out);
+/* One call of INTERNAL_STEP (internal_step.F:29), ISPLIT EXTERNAL_STEPs
+ * included.  Advances IINT by one.                                       */
+int  fvgpu_internal_step(fvgpu_handle* h);
+/* nsteps internal steps; with use_graph!=0 each step is one hipGraph
+ * launch captured on first use.                                          */
+int  fvgpu_run(fvgpu_handle* h, int64_t nsteps, int32_t use_graph);
+/* One call of EXTERNAL_STEP (external_step.F:30) on the current state,
+ * for parity tests of the 2-D mode alone.  iext is IEXT (1..ISPLIT).     */
+int  fvgpu_external_step(fvgpu_handle* h, int32_t iext);
+/* Copy one state or grid array (ALL_VARS name, lower case) to/from the host
+ * (double precision on the host side whatever the device precision).     */
+int  fvgpu_get_state(fvgpu_handle* h, const char* name, double* host, int64_t len);
+int  fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64_t len);
+int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
+/* Kernel-level timing of the last fvgpu_run: average device time of the
+ * named kernel family in microseconds, measured with HIP events on the
+ * handle's stream.  Returns non-zero if the name is unknown.             */
+int  fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t* count);
+int  fvgpu_set_profiling(fvgpu_handle* h, int32_t on);
+int  fvgpu_synchronize(fvgpu_handle* h);
+void fvgpu_destroy(fvgpu_handle* h);
+int  fvgpu_device_count(int32_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVGPU_H */

@@ -1,0 +1,308 @@
+"""MI355X-native FVCOM mode-split time step.
+
+Host-side mirror of the reference's call surface: ``Model`` holds the grid
+and state arrays under their ALL_VARS names (FVCOM_source/mod_main.F), and
+``DeviceModel`` is the device-resident state whose ``internal_step`` and
+``external_step`` replace INTERNAL_STEP (FVCOM_source/internal_step.F:29)
+and EXTERNAL_STEP (FVCOM_source/external_step.F:30).  Both talk to the
+C-ABI declared in ``include/fvgpu.h``; PyTorch is not needed here.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "lib")
+
+# FVX extents in include/fvgpu.h order
+_EXT = ["M1", "M2", "N1", "N1x3", "N1x4", "NE1", "NE1x2", "CV1", "CV1x2", "M1K", "N1K",
+        "RSN", "RVE", "OBC", "OBCx2", "OBCxT", "TIDE", "RIV", "RIVxK"]
+
+
+class FvParams(C.Structure):
+    _fields_ = [("dte", C.c_double), ("isplit", C.c_int32), ("iramp", C.c_int32),
+                ("umol", C.c_double), ("vprnu", C.c_double), ("hprnu", C.c_double),
+                ("cbcmin", C.c_double), ("min_depth", C.c_double),
+                ("horizontal_closure", C.c_int32), ("vertical_closure", C.c_int32),
+                ("scalar_positivity", C.c_int32), ("noflux_bot", C.c_int32),
+                ("barotropic", C.c_int32), ("temperature_active", C.c_int32),
+                ("salinity_active", C.c_int32), ("wind_on", C.c_int32),
+                ("river_ts_specified", C.c_int32), ("type_tsobc", C.c_int32),
+                ("obc_on", C.c_int32), ("obc_elevation_on", C.c_int32),
+                ("iint0", C.c_int64), ("time0", C.c_double)]
+
+
+class FvCase(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double), ("jitter", C.c_double),
+                ("kb", C.c_int32), ("depth_kind", C.c_int32), ("n_seamounts", C.c_int32),
+                ("ntide", C.c_int32), ("tide_amp", C.c_double * 3), ("river", C.c_int32),
+                ("river_q", C.c_double), ("river_t", C.c_double), ("river_s", C.c_double),
+                ("wind", C.c_int32), ("wind_mean", C.c_double), ("wind_std", C.c_double),
+                ("wind_corr", C.c_double), ("ts_kind", C.c_int32), ("el_pert_std", C.c_double),
+                ("el_pert_corr", C.c_double), ("coriolis", C.c_double), ("seed", C.c_uint64),
+                ("p", FvParams)]
+
+
+class FvModelHeader(C.Structure):
+    """Leading scalar part of ``fv_model`` (arrays are reached by name)."""
+    _fields_ = [(n, C.c_int32) for n in ("m", "n", "ne", "ncv", "ncv_i", "kb", "kbm1", "kbm2",
+                                         "iobcn", "ntide", "numqbc", "nring_sn", "nring_ve")] + \
+               [("p", FvParams)]
+
+
+class FvDiag(C.Structure):
+    _fields_ = [("nonfinite", C.c_int32), ("depth_fail", C.c_int32), ("min_depth", C.c_double),
+                ("volume", C.c_double), ("salt", C.c_double), ("heat", C.c_double),
+                ("iint", C.c_int64)]
+
+
+_grid = None
+_gpu = None
+
+
+def lib_path(name):
+    return os.path.join(_LIB, name)
+
+
+def grid_lib():
+    """Load libfvgrid.so (host metrics); raises if it has not been built."""
+    global _grid
+    if _grid is None:
+        path = lib_path("libfvgrid.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libfvgrid.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.fv_last_error.restype = C.c_char_p
+        L.fv_case_defaults.argtypes = [C.c_int32, C.POINTER(FvCase)]
+        L.fv_model_build.argtypes = [C.POINTER(FvCase), C.POINTER(C.c_void_p)]
+        L.fv_model_from_mesh.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                         C.POINTER(FvParams), C.POINTER(C.c_void_p)]
+        L.fv_model_free.argtypes = [C.c_void_p]
+        L.fv_model_field.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        L.fv_model_nfields.restype = C.c_int32
+        L.fv_model_field_name.restype = C.c_char_p
+        L.fv_model_field_name.argtypes = [C.c_int32]
+        L.fv_model_cold_start.argtypes = [C.c_void_p]
+        L.fv_extent_len.restype = C.c_int64
+        L.fv_extent_len.argtypes = [C.c_void_p, C.c_int32]
+        _grid = L
+    return _grid
+
+
+def gpu_lib():
+    """Load libfvgpu.so; fails loudly when the HIP library is missing."""
+    global _gpu
+    if _gpu is None:
+        path = lib_path("libfvgpu.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libfvgpu.so (HIP) is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.fv_last_error.restype = C.c_char_p
+        L.fvgpu_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+        L.fvgpu_internal_step.argtypes = [vp]
+        L.fvgpu_run.argtypes = [vp, C.c_int64, C.c_int32]
+        L.fvgpu_external_step.argtypes = [vp, C.c_int32]
+        L.fvgpu_get_state.argtypes = [vp, C.c_char_p, vp, C.c_int64]
+        L.fvgpu_put_state.argtypes = [vp, C.c_char_p, vp, C.c_int64]
+        L.fvgpu_diagnostics.argtypes = [vp, C.POINTER(FvDiag)]
+        L.fvgpu_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
+        L.fvgpu_synchronize.argtypes = [vp]
+        L.fvgpu_destroy.argtypes = [vp]
+        L.fvgpu_device_count.argtypes = [C.POINTER(C.c_int32)]
+        _gpu = L
+    return _gpu
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        msg = lib.fv_last_error()
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else "?"))
+
+
+class Model:
+    """Host grid + metrics + forcing + state of one instance (libfvgrid)."""
+
+    def __init__(self, ptr):
+        self._ptr = C.c_void_p(ptr)
+        self.hdr = FvModelHeader.from_address(ptr)
+
+    @staticmethod
+    def case_defaults(config):
+        c = FvCase()
+        L = grid_lib()
+        _check(L, L.fv_case_defaults(int(config), C.byref(c)), "fv_case_defaults")
+        return c
+
+    @classmethod
+    def build(cls, config=0, **overrides):
+        """Synthetic configuration ``config`` (0, 1, 3) with field overrides
+        (``nx=``, ``ny=``, ``kb=``, ``seed=`` ... or ``p_<param>=``)."""
+        c = cls.case_defaults(config)
+        for k, v in overrides.items():
+            if k.startswith("p_"):
+                setattr(c.p, k[2:], v)
+            else:
+                setattr(c, k, v)
+        L = grid_lib()
+        out = C.c_void_p()
+        _check(L, L.fv_model_build(C.byref(c), C.byref(out)), "fv_model_build")
+        return cls(out.value)
+
+    @classmethod
+    def from_mesh(cls, x, y, nv_rows, h, kb, obc_nodes=(), params=None):
+        """Explicit mesh: clockwise 1-based triangles ``nv_rows`` (N,3)."""
+        L = grid_lib()
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        h = np.ascontiguousarray(h, np.float64)
+        nv = np.ascontiguousarray(nv_rows, np.int32)
+        obc = np.ascontiguousarray(obc_nodes, np.int32)
+        if params is None:
+            params = cls.case_defaults(0).p
+        out = C.c_void_p()
+        rc = L.fv_model_from_mesh(len(x), nv.shape[0], x.ctypes.data, y.ctypes.data, nv.ctypes.data,
+                                  h.ctypes.data, int(kb), len(obc), obc.ctypes.data if len(obc) else None,
+                                  C.byref(params), C.byref(out))
+        _check(L, rc, "fv_model_from_mesh")
+        return cls(out.value)
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    @property
+    def m(self):
+        return self.hdr.m
+
+    @property
+    def n(self):
+        return self.hdr.n
+
+    @property
+    def kb(self):
+        return self.hdr.kb
+
+    def field(self, name):
+        """Live numpy view of an ALL_VARS array (lower-case name)."""
+        L = grid_lib()
+        p = C.c_void_p()
+        n = C.c_int64()
+        dt = C.c_int32()
+        _check(L, L.fv_model_field(self._ptr, name.encode(), C.byref(p), C.byref(n), C.byref(dt)),
+               "field " + name)
+        if n.value == 0 or not p.value:
+            return np.zeros(0, np.float64 if dt.value == 0 else np.int32)
+        ct = C.c_double if dt.value == 0 else C.c_int32
+        arr = (ct * n.value).from_address(p.value)
+        return np.ctypeslib.as_array(arr)
+
+    def field3(self, name):
+        """3-D array viewed as [KB, entity+1] (layer-major, like A(0:MT,KB))."""
+        a = self.field(name)
+        return a.reshape(self.kb, -1)
+
+    def field_names(self):
+        L = grid_lib()
+        return [L.fv_model_field_name(i).decode() for i in range(L.fv_model_nfields())]
+
+    def close(self):
+        if self._ptr is not None and self._ptr.value:
+            grid_lib().fv_model_free(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+class DeviceModel:
+    """Device-resident state of one model on the current GPU (libfvgpu)."""
+
+    def __init__(self, model, precision="fp64"):
+        L = gpu_lib()
+        self.model = model
+        h = C.c_void_p()
+        _check(L, L.fvgpu_create(model.ptr, PRECISION[precision], C.byref(h)), "fvgpu_create")
+        self._h = h
+
+    def internal_step(self):
+        L = gpu_lib()
+        _check(L, L.fvgpu_internal_step(self._h), "fvgpu_internal_step")
+
+    def run(self, nsteps, graph=True):
+        L = gpu_lib()
+        _check(L, L.fvgpu_run(self._h, int(nsteps), 1 if graph else 0), "fvgpu_run")
+
+    def external_step(self, iext=1):
+        L = gpu_lib()
+        _check(L, L.fvgpu_external_step(self._h, int(iext)), "fvgpu_external_step")
+
+    def get(self, name):
+        L = gpu_lib()
+        ref = self.model.field(name) if name in self.model.field_names() else None
+        n = len(ref) if ref is not None else self._len_of(name)
+        out = np.empty(n, np.float64)
+        _check(L, L.fvgpu_get_state(self._h, name.encode(), out.ctypes.data, n), "get " + name)
+        return out
+
+    def _len_of(self, name):
+        m1, n1, kb = self.model.m + 1, self.model.n + 1, self.model.kb
+        node2 = {"elf", "dtfa", "egf"}
+        elem2 = {"uard", "vard", "adx2d", "ady2d", "drx2d", "dry2d", "advua", "advva", "pstx", "psty",
+                 "cbc", "wubot", "wvbot", "wusurf", "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf"}
+        elem3 = {"uf", "vf", "advx", "advy", "drhox", "drhoy", "viscofm"}
+        if name in node2:
+            return m1
+        if name in elem2:
+            return n1
+        if name in elem3:
+            return n1 * kb
+        return m1 * kb
+
+    def put(self, name, values):
+        L = gpu_lib()
+        v = np.ascontiguousarray(values, np.float64)
+        _check(L, L.fvgpu_put_state(self._h, name.encode(), v.ctypes.data, len(v)), "put " + name)
+
+    def diagnostics(self):
+        L = gpu_lib()
+        d = FvDiag()
+        _check(L, L.fvgpu_diagnostics(self._h, C.byref(d)), "fvgpu_diagnostics")
+        return d
+
+    def kernel_time(self, name):
+        L = gpu_lib()
+        us = C.c_double()
+        cnt = C.c_int64()
+        _check(L, L.fvgpu_kernel_time(self._h, name.encode(), C.byref(us), C.byref(cnt)),
+               "fvgpu_kernel_time")
+        return us.value, cnt.value
+
+    def set_profiling(self, on):
+        L = gpu_lib()
+        _check(L, L.fvgpu_set_profiling(self._h, 1 if on else 0), "fvgpu_set_profiling")
+
+    def synchronize(self):
+        L = gpu_lib()
+        _check(L, L.fvgpu_synchronize(self._h), "fvgpu_synchronize")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            gpu_lib().fvgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

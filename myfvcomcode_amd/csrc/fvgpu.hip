@@ -1,0 +1,692 @@
+// fvgpu.hip -- MI355X (gfx950) implementation of FVCOM's mode-split time
+// step behind the C-ABI of include/fvgpu.h.
+//
+// fvgpu_internal_step replaces INTERNAL_STEP (FVCOM_source/internal_step.F:29)
+// and runs ISPLIT external steps (EXTERNAL_STEP, external_step.F:30) inside
+// it.  The state stays resident in HBM between calls; the host only
+// uploads at fvgpu_create and copies arrays on request.
+//
+// Design points:
+//  * every scatter loop of the reference becomes a gather through sorted
+//    CSR lists (node -> control-volume sub-edges, element -> edges), so no
+//    atomics are used and sums are formed in the reference's order;
+//  * the RK stages of the 2-D mode ping-pong between three buffer sets with
+//    the fourth stage written in place over the RK base (pointwise reads
+//    only), so the state returns to fixed buffers after every external step
+//    and one internal step can be captured as a replayable hipGraph;
+//  * time-dependent scalars (RAMP, ExtTime) live in a device clock updated
+//    by single-thread kernels, never by host writes between launches.
+#include "fvgpu_kernels_tracer.h"
+#include "../../include/fvgpu.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+thread_local std::string g_err;
+int fail(const std::string& m) { g_err = m; return 1; }
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct KernelStat { double total_ms = 0; int64_t count = 0; };
+
+}  // namespace
+
+struct fvgpu_handle {
+  const fv_model* md = nullptr;
+  int precision = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevGrid g{};
+  std::vector<void*> allocs;
+  // device state
+  DevClock* clk = nullptr;
+  Ext2DState ext[3];      // ext[0] = state / RK base, ext[1], ext[2] stage buffers
+  IntArgs ia{};
+  double *xflux_obcn = nullptr, *fobn = nullptr;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // profiling
+  bool profiling = false;
+  std::map<std::string, KernelStat> stats;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_next = 0;
+  void* diag_part = nullptr;   // 256 partial sums of the diagnostics reduction
+};
+
+namespace {
+
+template <typename T>
+int dalloc(fvgpu_handle* h, T** p, size_t n) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
+  HIPCHK(hipMemset(q, 0, (n ? n : 1) * sizeof(T)));
+  h->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+template <typename T>
+int dupload(fvgpu_handle* h, const T** p, const T* src, size_t n) {
+  T* q = nullptr;
+  if (dalloc(h, &q, n)) return 1;
+  if (n) HIPCHK(hipMemcpy(q, src, n * sizeof(T), hipMemcpyHostToDevice));
+  *p = q;
+  return 0;
+}
+
+int dupload_rw(fvgpu_handle* h, double** p, const double* src, size_t n) {
+  if (dalloc(h, p, n)) return 1;
+  if (n && src) HIPCHK(hipMemcpy(*p, src, n * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// profiling helpers: events around launches when enabled (eager mode only)
+hipEvent_t next_event(fvgpu_handle* h) {
+  if (h->event_next >= h->event_pool.size()) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    h->event_pool.push_back(e);
+  }
+  return h->event_pool[h->event_next++];
+}
+
+struct Launch {
+  fvgpu_handle* h;
+  const char* name;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  Launch(fvgpu_handle* h_, const char* n) : h(h_), name(n) {
+    if (h->profiling) { e0 = next_event(h); hipEventRecord(e0, h->stream); }
+  }
+  ~Launch() {
+    if (h->profiling) {
+      e1 = next_event(h);
+      hipEventRecord(e1, h->stream);
+      h->pending.push_back({name, {e0, e1}});
+    }
+  }
+};
+
+int flush_profile(fvgpu_handle* h) {
+  if (h->pending.empty()) return 0;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (auto& p : h->pending) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    KernelStat& s = h->stats[p.first];
+    s.total_ms += ms;
+    s.count += 1;
+  }
+  h->pending.clear();
+  h->event_next = 0;
+  return 0;
+}
+
+#define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...)                                   \
+  do {                                                                            \
+    Launch l_(h, NAME);                                                           \
+    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), 0, h->stream, __VA_ARGS__); \
+  } while (0)
+
+inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
+
+const double ALPHA_RK[5] = {0.0, 0.2500, 1.0 / 3.0, 0.5000, 1.0};
+
+// ------------------------------------------------------------- the steps
+ExtArgs make_ext_args(fvgpu_handle* h, int K, const Ext2DState& cur, const Ext2DState& nxt) {
+  ExtArgs a{};
+  a.cur = cur;
+  a.base = h->ext[0];
+  a.nxt = nxt;
+  a.K = K;
+  a.alpha = ALPHA_RK[K];
+  a.xflux_obcn = h->xflux_obcn;
+  a.fobn = h->fobn;
+  a.uard_obcn = h->ia.uard_obcn;
+  a.egf = h->ia.egf;
+  a.fluxobn = h->ia.fluxobn;
+  a.uard = h->ia.uard;
+  a.vard = h->ia.vard;
+  a.adx2d = h->ia.adx2d; a.ady2d = h->ia.ady2d; a.drx2d = h->ia.drx2d; a.dry2d = h->ia.dry2d;
+  a.wubot = h->ia.wubot; a.wvbot = h->ia.wvbot;
+  a.qdis = h->ia.qdis;
+  a.clk = h->clk;
+  return a;
+}
+
+void enqueue_external_step(fvgpu_handle* h, int iext) {
+  const DevGrid& g = h->g;
+  LAUNCH("ext_clock", k_ext_clock, 1, 64, h->clk, iext);
+  // stage buffers: 1: A->B, 2: B->C, 3: C->B, 4: B->A (in place on the RK base)
+  const Ext2DState* cur[5] = {nullptr, &h->ext[0], &h->ext[1], &h->ext[2], &h->ext[1]};
+  const Ext2DState* nxt[5] = {nullptr, &h->ext[1], &h->ext[2], &h->ext[1], &h->ext[0]};
+  for (int K = 1; K <= 4; ++K) {
+    ExtArgs a = make_ext_args(h, K, *cur[K], *nxt[K]);
+    LAUNCH("ext_elevation", k_ext_elevation, nb(g.M + 1), 256, g, a);
+    LAUNCH("ext_momentum", k_ext_momentum, nb(g.N + 1), 256, g, a);
+  }
+}
+
+void enqueue_internal_step(fvgpu_handle* h) {
+  const DevGrid& g = h->g;
+  IntArgs& a = h->ia;
+  const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
+  LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
+  LAUNCH("adjust1_brough", k_adjust1_brough, nb(g.N + 1), 256, g, a);
+  LAUNCH("adv3d_sum", k_adv3d_sum, nb(g.N + 1), 256, g, a);
+  LAUNCH("baropg", k_baropg, nb(g.N + 1), 256, g, a, (int)h->md->p.barotropic);
+  if (!h->md->p.barotropic) LAUNCH("rho1_roundtrip", k_rho1_roundtrip, nb(g.M + 1), 256, g, a);
+  LAUNCH("advave_int", k_advave_int, nb(g.N + 1), 256, g, a);
+  LAUNCH("zero", k_zero, nb(g.m1), 256, a.egf, g.m1);
+  if (g.IOBCN > 0) LAUNCH("zero", k_zero, nb(g.IOBCN), 256, a.uard_obcn, (long)g.IOBCN);
+  for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step(h, iext);
+  LAUNCH("adjust2", k_adjust2, nb(g.N + 1), 256, g, a);
+  LAUNCH("vertvl", k_vertvl, nb(g.M + 1), 256, g, a);
+  LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
+  LAUNCH("viscofh", k_viscofh, nb((long)g.M * g.KBM1), 256, g, a);
+  LAUNCH("advuv", k_advuv, nb(g.N + 1), 256, g, a);
+  LAUNCH("vdif_uv", k_vdif_uv, nb(g.N + 1), 256, g, a);
+  LAUNCH("wreal", k_wreal, nb(g.N + 1), 256, g, a);
+  const double qmean1 = (double)1.E-8f;
+  if (h->md->p.vertical_closure) {
+    LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * (g.KBM1 - 1)), 256, g, a, a.q2, nullptr, qmean1, 2, g.KBM1);
+    LAUNCH("adv_q", k_adv_q, nb(g.M + 1), 256, g, a, a.q2, a.q2f);
+    LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * (g.KBM1 - 1)), 256, g, a, a.q2l, nullptr, qmean1, 2, g.KBM1);
+    LAUNCH("adv_q", k_adv_q, nb(g.M + 1), 256, g, a, a.q2l, a.q2lf);
+    if (h->md->p.scalar_positivity) {
+      LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2, a.q2f);
+      LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2l, a.q2lf);
+    }
+    LAUNCH("vdif_q", k_vdif_q, nb(g.M + 1), 256, g, a);
+    LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
+    LAUNCH("copy", k_copy, nb(mk), 256, a.q2l, a.q2lf, mk);
+  }
+  LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.km, a.km1, g.KB);
+  for (int which = 1; which <= 2; ++which) {
+    if (which == 1 && !h->md->p.temperature_active) continue;
+    if (which == 2 && !h->md->p.salinity_active) continue;
+    double* f1 = which == 1 ? a.t1 : a.s1;
+    double* ff = which == 1 ? a.tf1 : a.sf1;
+    const double* fm = which == 1 ? a.tmean1 : a.smean1;
+    const double* fdis = which == 1 ? g.tdis : g.sdis;
+    LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * g.KBM1), 256, g, a, f1, fm, 0.0, 1, g.KBM1);
+    LAUNCH("adv_ts", k_adv_ts, nb(g.M + 1), 256, g, a, f1, ff, fdis, which);
+    if (h->md->p.scalar_positivity) LAUNCH("fct_ts", k_fct_ts, nb(g.M + 1), 256, g, f1, ff);
+    LAUNCH("vdif_ts", k_vdif_ts, nb(g.M + 1), 256, g, a, ff, which);
+    LAUNCH("bcond_ts", k_bcond_ts, nb(g.IOBCN + 1), 256, g, a, which);
+    LAUNCH("copy", k_copy, nb(mk), 256, f1, ff, mk);
+    LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, f1, which == 1 ? a.t : a.s, g.KB);
+  }
+  if (!h->md->p.barotropic) {
+    LAUNCH("dens2", k_dens2, nb(mk), 256, g, a);
+    LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.rho1, a.rho, g.KB);
+  }
+  LAUNCH("copy", k_copy, nb(nk), 256, a.u, a.uf, nk);
+  LAUNCH("copy", k_copy, nb(nk), 256, a.v, a.vf, nk);
+  LAUNCH("end_2d", k_end_2d, nb(std::max(g.M, g.N) + 1), 256, g, a);
+}
+
+// ------------------------------------------------------ derived structures
+int build_lists(fvgpu_handle* h) {
+  const fv_model* md = h->md;
+  const int M = md->m, N = md->n, NE = md->ne, NCV = md->ncv;
+  DevGrid& g = h->g;
+  // node -> CV sub-edges sorted by sub-edge index
+  std::vector<int> cnt(M + 2, 0);
+  for (int c = 1; c <= NCV; ++c) { cnt[md->niec[c]]++; cnt[md->niec[(size_t)(NCV + 1) + c]]++; }
+  std::vector<int> off(M + 2, 0);
+  for (int i = 0; i <= M; ++i) off[i + 1] = off[i] + cnt[i];
+  std::vector<int> ent(off[M + 1]), fill(M + 1, 0);
+  for (int c = 1; c <= NCV; ++c) {
+    const int ia = md->niec[c], ib = md->niec[(size_t)(NCV + 1) + c];
+    ent[off[ia] + fill[ia]++] = (c << 1) | 0;
+    ent[off[ib] + fill[ib]++] = (c << 1) | 1;
+  }
+  if (dupload(h, &g.cv_off, off.data(), off.size())) return 1;
+  if (dupload(h, &g.cv_ent, ent.data(), ent.size())) return 1;
+  // element -> its three edges sorted by edge index
+  std::vector<int> ed(3 * (size_t)(N + 1), 0), efill(N + 1, 0);
+  for (int e = 1; e <= NE; ++e) {
+    const int ia = md->iec[e], ib = md->iec[(size_t)(NE + 1) + e];
+    if (ia > 0) { if (efill[ia] >= 3) return fail("element with >3 edges"); ed[3 * ia + efill[ia]++] = (e << 1) | 0; }
+    if (ib > 0) { if (efill[ib] >= 3) return fail("element with >3 edges"); ed[3 * ib + efill[ib]++] = (e << 1) | 1; }
+  }
+  for (int i = 1; i <= N; ++i) if (efill[i] != 3) return fail("element without 3 edges");
+  if (dupload(h, &g.ed_ent, ed.data(), ed.size())) return 1;
+  // FLUXOBN contributions per element in OBC order
+  const int IOBCN = md->iobcn;
+  std::vector<std::vector<int>> fo(N + 1);
+  for (int i = 0; i < IOBCN; ++i) {
+    if (md->nadjc_obc[i] > 0) fo[md->adjc_obc[i]].push_back((i << 1) | 0);
+    if (md->nadjc_obc[i] > 1) fo[md->adjc_obc[(size_t)IOBCN + i]].push_back((i << 1) | 1);
+  }
+  std::vector<int> fo_off(N + 2, 0), fo_ent;
+  for (int i = 0; i <= N; ++i) { fo_off[i + 1] = fo_off[i] + (int)fo[i].size(); for (int x : fo[i]) fo_ent.push_back(x); }
+  if (dupload(h, &g.fo_off, fo_off.data(), fo_off.size())) return 1;
+  if (dupload(h, &g.fo_obc, fo_ent.data(), fo_ent.size())) return 1;
+  std::vector<int> nobc(M + 1, -1), nriv(M + 1, -1), skip(M + 1, 0);
+  for (int i = 0; i < IOBCN; ++i) { nobc[md->i_obc_n[i]] = i; skip[md->i_obc_n[i]] = 1; }
+  for (int j = md->numqbc - 1; j >= 0; --j) { nriv[md->inodeq[j]] = j; skip[md->inodeq[j]] = 1; }
+  if (dupload(h, &g.node_obc, nobc.data(), nobc.size())) return 1;
+  if (dupload(h, &g.node_riv, nriv.data(), nriv.size())) return 1;
+  if (dupload(h, &g.fct_skip, skip.data(), skip.size())) return 1;
+  // element -> rivers (NBVE(J,1) then NBVE(J,NTVE(J)) of the river node)
+  std::vector<std::vector<int>> re(N + 1);
+  for (int k = 0; k < md->numqbc; ++k) {
+    const int j = md->inodeq[k];
+    const int i1 = md->nbve[md->nbve_off[j]], i2 = md->nbve[md->nbve_off[j] + md->ntve[j] - 1];
+    re[i1].push_back(k);
+    re[i2].push_back(k);
+  }
+  std::vector<int> re_off(N + 2, 0), re_ent;
+  for (int i = 0; i <= N; ++i) { re_off[i + 1] = re_off[i] + (int)re[i].size(); for (int x : re[i]) re_ent.push_back(x); }
+  if (dupload(h, &g.re_off, re_off.data(), re_off.size())) return 1;
+  if (dupload(h, &g.re_riv, re_ent.data(), re_ent.size())) return 1;
+  return 0;
+}
+
+bool columns_uniform(const double* a, long l1, int kb, int first) {
+  for (int k = 0; k < kb; ++k) {
+    const double v = a[(long)k * l1 + first];
+    for (long i = first; i < l1; ++i)
+      if (std::memcmp(&a[(long)k * l1 + i], &v, sizeof(double)) != 0) return false;
+  }
+  return true;
+}
+
+int upload_grid(fvgpu_handle* h) {
+  const fv_model* md = h->md;
+  DevGrid& g = h->g;
+  const int M = md->m, N = md->n, KB = md->kb;
+  const size_t m1 = M + 1, n1 = N + 1;
+  g.M = M; g.N = N; g.NE = md->ne; g.NCV = md->ncv; g.KB = KB; g.KBM1 = md->kbm1; g.KBM2 = md->kbm2;
+  g.IOBCN = md->iobcn; g.NUMQBC = md->numqbc; g.NTIDE = md->ntide;
+  g.m1 = (long)m1; g.n1 = (long)n1;
+  if (KB > FV_MAXKB) return fail("KB exceeds FV_MAXKB");
+  const fv_params& p = md->p;
+  g.dte = p.dte; g.dti = p.dte * p.isplit; g.umol = p.umol; g.vprnu = p.vprnu; g.hprnu = p.hprnu;
+  g.cbcmin = p.cbcmin; g.min_depth = p.min_depth;
+  g.fact = p.horizontal_closure ? 1.0 : 0.0; g.fm1 = 1.0 - g.fact;
+  g.const1 = std::pow(16.6, .6666667) * 1.00;
+  g.cvhp1 = std::pow(15.8 * 100.0, 0.6666667);
+  g.isplit = p.isplit; g.iramp = p.iramp; g.noflux_bot = p.noflux_bot;
+  g.river_ts_specified = p.river_ts_specified; g.type_tsobc = p.type_tsobc; g.obc_on = p.obc_on;
+  g.obc_elev = p.obc_elevation_on; g.wind_on = p.wind_on;
+#define UP(field, n) if (dupload(h, &g.field, md->field, (n))) return 1
+  UP(vx, m1); UP(vy, m1); UP(h, m1); UP(art1, m1); UP(art2, m1); UP(grav_n, m1); UP(nn_hvc, m1); UP(phpn, m1);
+  UP(isonb, m1); UP(ntsn, m1); UP(ntve, m1); UP(nbsn_off, m1 + 1); UP(nbve_off, m1 + 1);
+  UP(nbsn, md->nring_sn); UP(dltxtrie, md->nring_sn); UP(dltytrie, md->nring_sn);
+  UP(nbve, md->nring_ve); UP(dltxecec, md->nring_ve); UP(dltyecec, md->nring_ve);
+  UP(dltxnec, md->nring_ve); UP(dltynec, md->nring_ve);
+  UP(xc, n1); UP(yc, n1); UP(art, n1); UP(h1, n1); UP(cor, n1); UP(cc_hvc, n1); UP(cc_sponge, n1);
+  UP(cc_z0b, n1); UP(grav_e, n1); UP(epor, n1); UP(isbce, n1); UP(iucp, n1);
+  UP(nv, 3 * n1); UP(nbe, 3 * n1); UP(a1u, 4 * n1); UP(a2u, 4 * n1); UP(awx, 3 * n1); UP(awy, 3 * n1);
+  const size_t ne1 = md->ne + 1, cv1 = md->ncv + 1;
+  UP(iec, 2 * ne1); UP(ienode, 2 * ne1); UP(isbc, ne1); UP(xijc, ne1); UP(yijc, ne1); UP(dltxc, ne1); UP(dltyc, ne1);
+  UP(niec, 2 * cv1); UP(ntrg, cv1); UP(dltxe, cv1); UP(dltye, cv1); UP(dltxncve, 2 * cv1); UP(dltyncve, 2 * cv1);
+  const size_t ob = md->iobcn, nt = md->ntide, nr = md->numqbc;
+  UP(i_obc_n, ob); UP(next_obc, ob); UP(nadjc_obc, ob); UP(adjc_obc, 2 * ob); UP(fluxf_obc, 2 * ob);
+  UP(dltn_obc, ob); UP(emean, ob); UP(apt, ob * nt); UP(phai, ob * nt); UP(period, nt);
+  UP(inodeq, nr); UP(qarea, nr); UP(qdis0, nr); UP(tdis, nr); UP(sdis, nr); UP(vqdist, nr * md->kbm1);
+#undef UP
+  // host-evaluated trigonometry of setup angles (the same libm the setup uses)
+  std::vector<double> cs(m1), sn(m1);
+  for (size_t i = 0; i < m1; ++i) { cs[i] = std::cos(md->sita_gd[i]); sn[i] = std::sin(md->sita_gd[i]); }
+  if (dupload(h, &g.cos_sita, cs.data(), m1)) return 1;
+  if (dupload(h, &g.sin_sita, sn.data(), m1)) return 1;
+  std::vector<double> ca(n1), sa(n1);
+  for (size_t i = 0; i < n1; ++i) { ca[i] = std::cos(md->alpha[i]); sa[i] = std::sin(md->alpha[i]); }
+  if (dupload(h, &g.cos_alpha, ca.data(), n1)) return 1;
+  if (dupload(h, &g.sin_alpha, sa.data(), n1)) return 1;
+  std::vector<double> cq(nr), sq_(nr);
+  for (size_t i = 0; i < nr; ++i) { cq[i] = std::cos(md->angleq[i]); sq_[i] = std::sin(md->angleq[i]); }
+  if (dupload(h, &g.cos_angleq, cq.data(), nr)) return 1;
+  if (dupload(h, &g.sin_angleq, sq_.data(), nr)) return 1;
+  // wind stress of the constant wind speed (PSIMPLE_DRAG, mod_bulk.F:58-90)
+  std::vector<double> wx(n1, 0.0), wy(n1, 0.0);
+  if (p.wind_on)
+    for (size_t i = 1; i < n1; ++i) {
+      const double tx = md->uuwind[i], ty = md->vvwind[i];
+      const double wds = std::sqrt(tx * tx + ty * ty);
+      double cd = (double)1.2E-3f;
+      if (wds >= 11.0) cd = (0.49 + 0.065 * wds) * 1.E-3;
+      if (wds >= 25.0) cd = (0.49 + 0.065 * 25.0) * 1.E-3;
+      wx[i] = 1.2 * cd * tx * wds;
+      wy[i] = 1.2 * cd * ty * wds;
+    }
+  if (dupload(h, &g.wsx, wx.data(), n1)) return 1;
+  if (dupload(h, &g.wsy, wy.data(), n1)) return 1;
+  // sigma: one column when every column is identical
+  const bool un = columns_uniform(md->z, m1, KB, 0) && columns_uniform(md->dz, m1, KB, 0) &&
+                  columns_uniform(md->dzz, m1, KB, 0) && columns_uniform(md->zz, m1, KB, 0);
+  const bool ue = columns_uniform(md->z1, n1, KB, 1) && columns_uniform(md->dz1, n1, KB, 1) &&
+                  columns_uniform(md->dzz1, n1, KB, 1) && columns_uniform(md->zz1, n1, KB, 1);
+  g.sig_uniform = (un && ue) ? 1 : 0;
+  auto col = [&](const double* a, size_t l1, size_t first) {
+    std::vector<double> c(KB);
+    for (int k = 0; k < KB; ++k) c[k] = a[(size_t)k * l1 + first];
+    return c;
+  };
+  if (g.sig_uniform) {
+    std::vector<double> c;
+    c = col(md->z, m1, 0); if (dupload(h, &g.z, c.data(), KB)) return 1;
+    c = col(md->zz, m1, 0); if (dupload(h, &g.zz, c.data(), KB)) return 1;
+    c = col(md->dz, m1, 0); if (dupload(h, &g.dz, c.data(), KB)) return 1;
+    c = col(md->dzz, m1, 0); if (dupload(h, &g.dzz, c.data(), KB)) return 1;
+    c = col(md->z1, n1, 1); if (dupload(h, &g.z1, c.data(), KB)) return 1;
+    c = col(md->zz1, n1, 1); if (dupload(h, &g.zz1, c.data(), KB)) return 1;
+    c = col(md->dz1, n1, 1); if (dupload(h, &g.dz1, c.data(), KB)) return 1;
+    c = col(md->dzz1, n1, 1); if (dupload(h, &g.dzz1, c.data(), KB)) return 1;
+  } else {
+    if (dupload(h, &g.z, md->z, m1 * KB)) return 1;
+    if (dupload(h, &g.zz, md->zz, m1 * KB)) return 1;
+    if (dupload(h, &g.dz, md->dz, m1 * KB)) return 1;
+    if (dupload(h, &g.dzz, md->dzz, m1 * KB)) return 1;
+    if (dupload(h, &g.z1, md->z1, n1 * KB)) return 1;
+    if (dupload(h, &g.zz1, md->zz1, n1 * KB)) return 1;
+    if (dupload(h, &g.dz1, md->dz1, n1 * KB)) return 1;
+    if (dupload(h, &g.dzz1, md->dzz1, n1 * KB)) return 1;
+  }
+  return build_lists(h);
+}
+
+int upload_state(fvgpu_handle* h) {
+  const fv_model* md = h->md;
+  const DevGrid& g = h->g;
+  const size_t m1 = g.m1, n1 = g.n1, mk = m1 * g.KB, nk = n1 * g.KB;
+  IntArgs& a = h->ia;
+  for (int b = 0; b < 3; ++b) {
+    Ext2DState& s = h->ext[b];
+    if (dupload_rw(h, &s.el, b == 0 ? md->el : nullptr, m1)) return 1;
+    if (dupload_rw(h, &s.d, b == 0 ? md->d : nullptr, m1)) return 1;
+    if (dupload_rw(h, &s.ua, b == 0 ? md->ua : nullptr, n1)) return 1;
+    if (dupload_rw(h, &s.va, b == 0 ? md->va : nullptr, n1)) return 1;
+    if (dupload_rw(h, &s.el1, b == 0 ? md->el1 : nullptr, n1)) return 1;
+    if (dupload_rw(h, &s.d1, b == 0 ? md->d1 : nullptr, n1)) return 1;
+  }
+  a.el = h->ext[0].el; a.d = h->ext[0].d; a.ua = h->ext[0].ua; a.va = h->ext[0].va;
+  a.el1 = h->ext[0].el1; a.d1 = h->ext[0].d1;
+  a.elf = a.el;   // after the external loop ELF = EL
+#define RW(field, src, n) if (dupload_rw(h, &a.field, src, (n))) return 1
+  RW(et, md->et, m1); RW(dt, md->dt, m1); RW(et1, md->et1, n1); RW(dt1, md->dt1, n1);
+  RW(egf, nullptr, m1); RW(dtfa, md->d, m1);
+  RW(u, md->u, nk); RW(v, md->v, nk); RW(w, md->w, nk); RW(ww, md->ww, nk); RW(uf, nullptr, nk); RW(vf, nullptr, nk);
+  RW(drhox, nullptr, nk); RW(drhoy, nullptr, nk); RW(rho, md->rho, nk); RW(rmean, md->rmean, nk);
+  RW(t, md->t, nk); RW(s, md->s, nk); RW(km1, md->km1, nk); RW(viscofm, nullptr, nk);
+  RW(t1, md->t1, mk); RW(s1, md->s1, mk); RW(tf1, nullptr, mk); RW(sf1, nullptr, mk); RW(rho1, md->rho1, mk);
+  RW(rmean1, md->rmean1, mk); RW(tmean1, md->tmean1, mk); RW(smean1, md->smean1, mk);
+  RW(q2, md->q2, mk); RW(q2l, md->q2l, mk); RW(q2f, nullptr, mk); RW(q2lf, nullptr, mk);
+  RW(l, md->l, mk); RW(km, md->km, mk); RW(kh, md->kh, mk); RW(kq, md->kq, mk); RW(wts, md->wts, mk);
+  RW(viscofh, nullptr, mk);
+  RW(gx, nullptr, mk); RW(gy, nullptr, mk); RW(gxd, nullptr, mk); RW(gyd, nullptr, mk);
+  RW(fmin_, nullptr, mk); RW(fmax_, nullptr, mk);
+  RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
+  RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
+  RW(uard, nullptr, n1); RW(vard, nullptr, n1); RW(fluxobn, nullptr, n1);
+  RW(pfpxb, nullptr, m1); RW(pfpyb, nullptr, m1); RW(ah_bottom, nullptr, m1);
+  RW(qdis, nullptr, (size_t)g.NUMQBC); RW(xflux_obc, nullptr, (size_t)g.IOBCN * g.KB);
+  RW(uard_obcn, nullptr, (size_t)g.IOBCN);
+#undef RW
+  if (dalloc(h, &h->xflux_obcn, (size_t)g.IOBCN)) return 1;
+  if (dalloc(h, &h->fobn, (size_t)g.IOBCN)) return 1;
+  if (dalloc(h, &h->clk, 1)) return 1;
+  {
+    double* dp = nullptr;
+    if (dalloc(h, &dp, 256 * 8)) return 1;  // 256 x DiagAcc (8 words each)
+    h->diag_part = dp;
+  }
+  DevClock c{};
+  c.iint = md->p.iint0; c.iext = 0; c.ramp_int = 0; c.ramp_ext = 0;
+  c.ext_time = md->p.time0; c.time0 = md->p.time0; c.dti = g.dti; c.dte = g.dte;
+  c.isplit = g.isplit; c.iramp = g.iramp; c.depth_fail = 0;
+  HIPCHK(hipMemcpy(h->clk, &c, sizeof(c), hipMemcpyHostToDevice));
+  a.clk = h->clk;
+  return 0;
+}
+
+double* lookup(fvgpu_handle* h, const std::string& n, size_t* len) {
+  IntArgs& a = h->ia;
+  const size_t m1 = h->g.m1, n1 = h->g.n1, mk = m1 * h->g.KB, nk = n1 * h->g.KB;
+  struct E { const char* n; double* p; size_t l; };
+  const E t[] = {
+      {"el", a.el, m1}, {"d", a.d, m1}, {"et", a.et, m1}, {"dt", a.dt, m1}, {"elf", a.el, m1},
+      {"egf", a.egf, m1}, {"dtfa", a.dtfa, m1},
+      {"ua", a.ua, n1}, {"va", a.va, n1}, {"el1", a.el1, n1}, {"d1", a.d1, n1}, {"et1", a.et1, n1},
+      {"dt1", a.dt1, n1}, {"adx2d", a.adx2d, n1}, {"ady2d", a.ady2d, n1}, {"drx2d", a.drx2d, n1},
+      {"dry2d", a.dry2d, n1}, {"cbc", a.cbc, n1}, {"wubot", a.wubot, n1}, {"wvbot", a.wvbot, n1},
+      {"uard", a.uard, n1}, {"vard", a.vard, n1},
+      {"u", a.u, nk}, {"v", a.v, nk}, {"w", a.w, nk}, {"ww", a.ww, nk}, {"uf", a.uf, nk}, {"vf", a.vf, nk},
+      {"drhox", a.drhox, nk}, {"drhoy", a.drhoy, nk}, {"rho", a.rho, nk}, {"rmean", a.rmean, nk},
+      {"t", a.t, nk}, {"s", a.s, nk}, {"km1", a.km1, nk}, {"viscofm", a.viscofm, nk},
+      {"t1", a.t1, mk}, {"s1", a.s1, mk}, {"tf1", a.tf1, mk}, {"sf1", a.sf1, mk}, {"rho1", a.rho1, mk},
+      {"rmean1", a.rmean1, mk}, {"tmean1", a.tmean1, mk}, {"smean1", a.smean1, mk}, {"q2", a.q2, mk},
+      {"q2l", a.q2l, mk}, {"q2f", a.q2f, mk}, {"q2lf", a.q2lf, mk}, {"l", a.l, mk}, {"km", a.km, mk},
+      {"kh", a.kh, mk}, {"kq", a.kq, mk}, {"wts", a.wts, mk}, {"viscofh", a.viscofh, mk},
+  };
+  for (const E& e : t)
+    if (n == e.n) { *len = e.l; return e.p; }
+  return nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C-ABI
+extern "C" {
+
+const char* fv_last_error(void) { return g_err.c_str(); }
+
+int fvgpu_device_count(int32_t* n) {
+  int c = 0;
+  HIPCHK(hipGetDeviceCount(&c));
+  *n = c;
+  return 0;
+}
+
+int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
+  if (!md || !out) return fail("null argument");
+  if (precision != FV_FP64) return fail("only the FP64 form is available in this build");
+  if (md->kb > FV_MAXKB) return fail("KB too large");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail("no HIP device");
+  fvgpu_handle* h = new fvgpu_handle();
+  h->md = md;
+  h->precision = precision;
+  HIPCHK(hipGetDevice(&h->device));
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
+  if (upload_grid(h) || upload_state(h)) { fvgpu_destroy(h); return 1; }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *out = h;
+  return 0;
+}
+
+int fvgpu_internal_step(fvgpu_handle* h) {
+  enqueue_internal_step(h);
+  HIPCHK(hipGetLastError());
+  if (h->profiling) return flush_profile(h);
+  return 0;
+}
+
+int fvgpu_external_step(fvgpu_handle* h, int32_t iext) {
+  // the clock as INTERNAL_STEP would set it for IINT+1, then one EXTERNAL_STEP
+  LAUNCH("int_clock", k_int_clock, 1, 64, h->g, h->clk, h->ia.qdis);
+  enqueue_external_step(h, iext);
+  // restore IINT (this call does not complete an internal step)
+  DevClock c;
+  HIPCHK(hipMemcpyAsync(&c, h->clk, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  c.iint -= 1;
+  HIPCHK(hipMemcpyAsync(h->clk, &c, sizeof(c), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fvgpu_run(fvgpu_handle* h, int64_t nsteps, int32_t use_graph) {
+  if (!use_graph || h->profiling) {
+    for (int64_t s = 0; s < nsteps; ++s) {
+      enqueue_internal_step(h);
+      if (h->profiling && flush_profile(h)) return 1;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  if (!h->gexec) {
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_internal_step(h);
+    HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
+    HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  }
+  for (int64_t s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int fvgpu_get_state(fvgpu_handle* h, const char* name, double* host, int64_t len) {
+  size_t l = 0;
+  double* p = lookup(h, name, &l);
+  if (!p) return fail(std::string("unknown state array ") + name);
+  if ((size_t)len != l) return fail(std::string("length mismatch for ") + name);
+  HIPCHK(hipMemcpyAsync(host, p, l * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64_t len) {
+  size_t l = 0;
+  double* p = lookup(h, name, &l);
+  if (!p) return fail(std::string("unknown state array ") + name);
+  if ((size_t)len != l) return fail(std::string("length mismatch for ") + name);
+  HIPCHK(hipMemcpyAsync(p, host, l * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);  // fvgpu_diag.hip
+
+int fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t* count) {
+  auto it = h->stats.find(name);
+  if (it == h->stats.end()) return fail(std::string("no timing for ") + name);
+  *count = it->second.count;
+  *avg_us = it->second.count ? 1000.0 * it->second.total_ms / it->second.count : 0.0;
+  return 0;
+}
+
+int fvgpu_set_profiling(fvgpu_handle* h, int32_t on) {
+  h->profiling = on != 0;
+  h->stats.clear();
+  return 0;
+}
+
+int fvgpu_synchronize(fvgpu_handle* h) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+void fvgpu_destroy(fvgpu_handle* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  for (hipEvent_t e : h->event_pool) hipEventDestroy(e);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------- diagnostics
+// Deterministic two-level reduction with Neumaier compensation for the FP64
+// budgets; flags for non-finite values and the minimum depth.
+namespace {
+struct DiagAcc { double vol, vc, salt, sc, heat, hc, dmin; int bad; };
+
+__device__ void neumaier(double& s, double& c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+__global__ void k_diag_partial(DevGrid g, IntArgs a, DiagAcc* part) {
+  __shared__ DiagAcc sh[256];
+  const int tid = threadIdx.x;
+  DiagAcc acc{0, 0, 0, 0, 0, 0, 1e300, 0};
+  for (long i = (long)blockIdx.x * blockDim.x + tid + 1; i <= g.M; i += (long)gridDim.x * blockDim.x) {
+    const double d = a.d[i];
+    neumaier(acc.vol, acc.vc, g.art1[i] * d);
+    acc.dmin = fmin(acc.dmin, d);
+    if (!isfinite(a.el[i])) acc.bad = 1;
+    for (int k = 1; k <= g.KBM1; ++k) {
+      const double w = g.art1[i] * d * DZ((int)i, k);
+      const double s = a.s1[NI(g, (int)i, k)], t = a.t1[NI(g, (int)i, k)];
+      neumaier(acc.salt, acc.sc, w * s);
+      neumaier(acc.heat, acc.hc, w * t);
+      if (!isfinite(s) || !isfinite(t)) acc.bad = 1;
+    }
+  }
+  for (long i = (long)blockIdx.x * blockDim.x + tid + 1; i <= g.N; i += (long)gridDim.x * blockDim.x) {
+    if (!isfinite(a.ua[i]) || !isfinite(a.va[i])) acc.bad = 1;
+    for (int k = 1; k <= g.KBM1; ++k)
+      if (!isfinite(a.u[EI(g, (int)i, k)]) || !isfinite(a.v[EI(g, (int)i, k)])) acc.bad = 1;
+  }
+  sh[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    DiagAcc r = sh[0];
+    for (int j = 1; j < blockDim.x; ++j) {
+      neumaier(r.vol, r.vc, sh[j].vol); r.vc += sh[j].vc;
+      neumaier(r.salt, r.sc, sh[j].salt); r.sc += sh[j].sc;
+      neumaier(r.heat, r.hc, sh[j].heat); r.hc += sh[j].hc;
+      r.dmin = fmin(r.dmin, sh[j].dmin);
+      r.bad |= sh[j].bad;
+    }
+    part[blockIdx.x] = r;
+  }
+}
+}  // namespace
+
+extern "C" int fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out) {
+  const int nblk = 256;
+  DiagAcc* part = (DiagAcc*)h->diag_part;
+  hipLaunchKernelGGL(k_diag_partial, dim3(nblk), dim3(256), 0, h->stream, h->g, h->ia, part);
+  std::vector<DiagAcc> hp(nblk);
+  HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(DiagAcc) * nblk, hipMemcpyDeviceToHost, h->stream));
+  DevClock c;
+  HIPCHK(hipMemcpyAsync(&c, h->clk, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double vol = 0, vc = 0, salt = 0, sc = 0, heat = 0, hc = 0, dmin = 1e300;
+  int bad = 0;
+  auto nm = [](double& s, double& cc, double x) {
+    const double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) cc += (s - t) + x; else cc += (x - t) + s;
+    s = t;
+  };
+  for (const DiagAcc& r : hp) {
+    nm(vol, vc, r.vol); vc += r.vc;
+    nm(salt, sc, r.salt); sc += r.sc;
+    nm(heat, hc, r.heat); hc += r.hc;
+    dmin = std::min(dmin, r.dmin);
+    bad |= r.bad;
+  }
+  out->nonfinite = bad;
+  out->depth_fail = c.depth_fail;
+  out->min_depth = dmin;
+  out->volume = vol + vc;
+  out->salt = salt + sc;
+  out->heat = heat + hc;
+  out->iint = c.iint;
+  return 0;
+}

@@ -1,0 +1,93 @@
+// fvgpu_common.h -- device-side layout shared by the HIP kernels.
+//
+// All arrays keep the reference's dummy entry 0 and its column-major 3-D
+// layout A(0:MT,1:KB): entity index fastest, so a wavefront of consecutive
+// nodes/elements at one layer reads consecutive addresses.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FV_MAXKB 64
+
+struct DevGrid {
+  int M, N, NE, NCV, KB, KBM1, KBM2, IOBCN, NUMQBC, NTIDE;
+  long m1, n1;
+  int sig_uniform;          // all node (and all element) sigma columns identical
+  double dte, dti, umol, vprnu, hprnu, cbcmin, min_depth;
+  double fact, fm1;         // horizontal mixing closure / constant switches
+  double const1, cvhp1;     // 16.6**.6666667*SEF and (15.8*CBCNST)**0.6666667
+  int isplit, iramp, noflux_bot, river_ts_specified, type_tsobc, obc_on, obc_elev, wind_on;
+  // nodes
+  const double *vx, *vy, *h, *art1, *art2, *grav_n, *nn_hvc, *cos_sita, *sin_sita, *phpn;
+  const int *isonb, *ntsn, *nbsn_off, *nbsn, *ntve, *nbve_off, *nbve;
+  const double *dltxtrie, *dltytrie, *dltxecec, *dltyecec, *dltxnec, *dltynec;
+  // elements
+  const double *xc, *yc, *art, *h1, *cor, *cc_hvc, *cc_sponge, *cc_z0b, *grav_e, *epor;
+  const double *sin_alpha, *cos_alpha, *wsx, *wsy;
+  const int *isbce, *iucp, *nv, *nbe;
+  const double *a1u, *a2u, *awx, *awy;
+  // edges and CV sub-edges
+  const int *iec, *ienode, *isbc, *niec, *ntrg;
+  const double *xijc, *yijc, *dltxc, *dltyc, *dltxe, *dltye, *dltxncve, *dltyncve;
+  // sigma (per column, or one column when sig_uniform)
+  const double *z, *zz, *dz, *dzz, *z1, *zz1, *dz1, *dzz1;
+  // gather lists
+  const int *cv_off, *cv_ent;      // node -> (cv<<1 | side), sorted by cv
+  const int *ed_ent;               // element -> 3 x (edge<<1 | side), sorted by edge
+  const int *fo_off, *fo_obc;      // element -> FLUXOBN contributions (obc<<1 | slot)
+  const int *node_obc;             // node -> OBC index or -1
+  const int *node_riv;             // node -> river index or -1
+  const int *re_off, *re_riv;      // element -> rivers whose NBVE(1) / NBVE(NTVE) it is
+  const int *fct_skip;             // node skipped by FCT (OBC or river node)
+  // OBC / tide / rivers
+  const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc;
+  const double *fluxf_obc, *dltn_obc, *emean, *apt, *phai, *period;
+  const int *inodeq;
+  const double *qarea, *cos_angleq, *sin_angleq, *qdis0, *tdis, *sdis, *vqdist;
+};
+
+// 1-based layer accessors
+__device__ __forceinline__ long NI(const DevGrid& g, int i, int k) { return (long)(k - 1) * g.m1 + i; }
+__device__ __forceinline__ long EI(const DevGrid& g, int i, int k) { return (long)(k - 1) * g.n1 + i; }
+__device__ __forceinline__ double SIGN_(const DevGrid& g, const double* a, int i, int k) {
+  return g.sig_uniform ? a[k - 1] : a[NI(g, i, k)];
+}
+__device__ __forceinline__ double SIGE_(const DevGrid& g, const double* a, int i, int k) {
+  return g.sig_uniform ? a[k - 1] : a[EI(g, i, k)];
+}
+#define DZ(i, k) SIGN_(g, g.dz, i, k)
+#define DZZ(i, k) SIGN_(g, g.dzz, i, k)
+#define ZN(i, k) SIGN_(g, g.z, i, k)
+#define ZZN(i, k) SIGN_(g, g.zz, i, k)
+#define DZ1(i, k) SIGE_(g, g.dz1, i, k)
+#define DZZ1(i, k) SIGE_(g, g.dzz1, i, k)
+#define Z1(i, k) SIGE_(g, g.z1, i, k)
+#define ZZ1(i, k) SIGE_(g, g.zz1, i, k)
+#define NV_(i, j) g.nv[(long)((j)-1) * g.n1 + (i)]
+#define NBE_(i, j) g.nbe[(long)((j)-1) * g.n1 + (i)]
+#define A1U_(i, j) g.a1u[(long)((j)-1) * g.n1 + (i)]
+#define A2U_(i, j) g.a2u[(long)((j)-1) * g.n1 + (i)]
+#define AWX_(i, j) g.awx[(long)((j)-1) * g.n1 + (i)]
+#define AWY_(i, j) g.awy[(long)((j)-1) * g.n1 + (i)]
+#define IEC_(e, j) g.iec[(long)((j)-1) * (g.NE + 1) + (e)]
+#define IENODE_(e, j) g.ienode[(long)((j)-1) * (g.NE + 1) + (e)]
+#define NIEC_(c, j) g.niec[(long)((j)-1) * (g.NCV + 1) + (c)]
+#define DLTXNCVE_(c, j) g.dltxncve[(long)((j)-1) * (g.NCV + 1) + (c)]
+#define DLTYNCVE_(c, j) g.dltyncve[(long)((j)-1) * (g.NCV + 1) + (c)]
+
+__device__ __forceinline__ double fsign1(double b) { return b >= 0.0 ? 1.0 : -1.0; }
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// Device clock: the time-dependent scalars of the current step, written by
+// tiny kernels so that a captured graph replays correctly.
+struct DevClock {
+  long long iint;
+  int iext;
+  double ramp_int;      // RAMP of INTERNAL_STEP
+  double ramp_ext;      // RAMP of the current EXTERNAL_STEP
+  double ext_time;      // ExtTime (s since SpecTime)
+  double time0, dti, dte;
+  int isplit, iramp;
+  int depth_fail;
+  int pad;
+};

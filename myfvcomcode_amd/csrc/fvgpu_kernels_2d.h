@@ -1,0 +1,251 @@
+// fvgpu_kernels_2d.h -- the barotropic (external) mode on the device.
+//
+// One Runge-Kutta stage of EXTERNAL_STEP (FVCOM_source/external_step.F:159-438)
+// is two kernels:
+//   k_ext_elevation  node-parallel: EXTEL_EDGE (extel_edge.F:32-229) as a
+//                    gather over each node's control-volume sub-edges, the
+//                    OBC tide and RK blend (BCOND_GCN(1,K), bcond_gcn.F:116;
+//                    BCOND_ASL, mod_obcs.F:509-592), the OBC flux of FLUX_OBN
+//                    (mod_obcs.F:1181-1215), D = H + EL, DEPTH_CHECK, and the
+//                    stage-3 EGF / stage-4 UARD_OBCN accumulations.
+//   k_ext_momentum   element-parallel: N2E2D of the new elevation,
+//                    ADVAVE_EDGE_GCN (advave_edge_gcn.F:546-992) gathered over
+//                    the element's three edges, EXTUV_EDGE (extuv_edge.F:33-196),
+//                    the wall projection BCOND_GCN(2,K) (bcond_gcn.F:325-420),
+//                    D1 = H1 + EL1 and the stage-3 UARD/VARD accumulation.
+// The per-node and per-element gather lists are sorted by sub-edge / edge
+// index, so every sum is formed in the order of the reference's scatter loop.
+#pragma once
+#include "fvgpu_common.h"
+#include "../../include/fv_detmath.h"
+
+struct Ext2DState {      // one buffer set of the 2-D state
+  double *el, *d;        // nodes
+  double *ua, *va, *el1, *d1;  // elements
+};
+
+struct ExtArgs {
+  Ext2DState cur;        // stage input
+  Ext2DState base;       // RK base (ELRK, UARK, VARK, ELRK1)
+  Ext2DState nxt;        // stage output (may alias base: pointwise use only)
+  int K;                 // RK stage 1..4
+  double alpha;          // ALPHA_RK(K)
+  double *xflux_obcn, *fobn, *uard_obcn, *egf, *fluxobn, *uard, *vard;
+  const double *adx2d, *ady2d, *drx2d, *dry2d, *wubot, *wvbot;
+  const double *qdis;    // QDIS (internal-mode river discharge)
+  DevClock* clk;
+};
+
+// Wall projection of a velocity pair (BCOND_GCN cases 2/3/5).
+__device__ __forceinline__ void wall_project(const DevGrid& g, int i, double& u, double& v) {
+  const int b = g.isbce[i];
+  if (b == 3) { u = 0.0; v = 0.0; }
+  if (b != 1) return;
+  for (int j = 0; j < g.NUMQBC; ++j) {
+    const int i1 = g.inodeq[j];
+    const int j1 = g.nbve[g.nbve_off[i1]];
+    const int j2 = g.nbve[g.nbve_off[i1] + g.ntve[i1] - 1];
+    if (i == j1 || i == j2) {
+      const double ca = g.cos_angleq[j], sa = g.sin_angleq[j];
+      double untmp = u * ca + v * sa;
+      const double vntmp = -u * sa + v * ca;
+      untmp = fmax(untmp, 0.0);
+      u = untmp * ca - vntmp * sa;
+      v = untmp * sa + vntmp * ca;
+      return;
+    }
+  }
+  const double s = g.sin_alpha[i], c = g.cos_alpha[i];
+  const double ui = u * (s * s) - v * s * c;
+  const double vi = -u * s * c + v * (c * c);
+  u = ui;
+  v = vi;
+}
+
+// Spectral tide at OBC node index `obc` (BCOND_ASL, mod_obcs.F:552-582).
+__device__ __forceinline__ double tide_force(const DevGrid& g, int obc, double rk_time, double ramp) {
+  const double PI2 = 6.283185307179586;
+  double force = 0.0;
+  for (int j = 0; j < g.NTIDE; ++j) {
+    const double phai_ij = g.phai[(long)j * g.IOBCN + obc] * PI2 / 360.0;
+    const double time_flt = rk_time * PI2 / g.period[j];
+    force = g.apt[(long)j * g.IOBCN + obc] * fv_cos(time_flt - phai_ij) + force;
+  }
+  force = force + g.emean[obc];
+  return force * ramp;
+}
+
+__global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.M) return;
+  double xflux = 0.0;
+  for (int p = g.cv_off[i]; p < g.cv_off[i + 1]; ++p) {
+    const int ent = g.cv_ent[p], c = ent >> 1;
+    const int i1 = g.ntrg[c];
+    const double ex = a.cur.d1[i1] * (-a.cur.ua[i1] * g.dltye[c] + a.cur.va[i1] * g.dltxe[c]);
+    if (ent & 1) xflux = xflux + ex;
+    else xflux = xflux - ex;
+  }
+  const int obc = g.node_obc[i];
+  double xobc = 0.0;
+  if (obc >= 0) { xobc = xflux; xflux = 0.0; a.xflux_obcn[obc] = xobc; }
+  const int rv = g.node_riv[i];
+  if (rv >= 0) {
+    for (int j = rv; j < g.NUMQBC; ++j)
+      if (g.inodeq[j] == i) xflux = xflux - a.qdis[j];
+  }
+  const double dtk = a.alpha * g.dte;
+  const double elrk = a.base.el[i];
+  double elf = elrk - dtk * xflux / g.art1[i];
+  if (obc >= 0) {
+    if (g.obc_elev) {
+      const double rk_time = a.clk->ext_time + g.dte * (a.alpha - 1.0);
+      elf = tide_force(g, obc, rk_time, a.clk->ramp_ext);
+    }
+    elf = elrk + a.alpha * (elf - elrk);
+    a.fobn[obc] = -(elf - elrk) * g.art1[i] / (a.alpha * g.dte) - xobc;
+    if (a.K == 4) {
+      const double tmp = -(elf - elrk) * g.art1[i] / g.dte - xobc;
+      a.uard_obcn[obc] = a.uard_obcn[obc] + tmp / (double)g.isplit;
+    }
+  }
+  const double dnew = g.h[i] + elf;
+  a.nxt.el[i] = elf;
+  a.nxt.d[i] = dnew;
+  if (a.K == 3) a.egf[i] = a.egf[i] + elf / (double)g.isplit;
+  if (i >= 1 && dnew < g.min_depth) a.clk->depth_fail = 1;
+}
+
+// Edge flux of ADVAVE_EDGE_GCN for edge `e`, evaluated exactly as the
+// reference's edge loop body (advave_edge_gcn.F:546-884).
+struct AdvaveFlux { double xterm, yterm, pterm_x, pterm_y; };
+
+__device__ __forceinline__ void advave_edge(const DevGrid& g, const double* ua, const double* va,
+                                            const double* d, const double* el, const double* d1,
+                                            int e, int self, int side, double& xf, double& yf,
+                                            double& px, double& py) {
+  const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
+  const double dij = 0.5 * (d[j1] + d[j2]);
+  const double elij = 0.5 * (el[j1] + el[j2]);
+  int k1 = NBE_(ia, 1), k2 = NBE_(ia, 2), k3 = NBE_(ia, 3);
+  const double cofa1 = A1U_(ia, 1) * ua[ia] + A1U_(ia, 2) * ua[k1] + A1U_(ia, 3) * ua[k2] + A1U_(ia, 4) * ua[k3];
+  const double cofa2 = A2U_(ia, 1) * ua[ia] + A2U_(ia, 2) * ua[k1] + A2U_(ia, 3) * ua[k2] + A2U_(ia, 4) * ua[k3];
+  const double cofa5 = A1U_(ia, 1) * va[ia] + A1U_(ia, 2) * va[k1] + A1U_(ia, 3) * va[k2] + A1U_(ia, 4) * va[k3];
+  const double cofa6 = A2U_(ia, 1) * va[ia] + A2U_(ia, 2) * va[k1] + A2U_(ia, 3) * va[k2] + A2U_(ia, 4) * va[k3];
+  double xij = g.xijc[e] - g.xc[ia], yij = g.yijc[e] - g.yc[ia];
+  const double uij1 = ua[ia] + cofa1 * xij + cofa2 * yij;
+  const double vij1 = va[ia] + cofa5 * xij + cofa6 * yij;
+  k1 = NBE_(ib, 1); k2 = NBE_(ib, 2); k3 = NBE_(ib, 3);
+  const double cofa3 = A1U_(ib, 1) * ua[ib] + A1U_(ib, 2) * ua[k1] + A1U_(ib, 3) * ua[k2] + A1U_(ib, 4) * ua[k3];
+  const double cofa4 = A2U_(ib, 1) * ua[ib] + A2U_(ib, 2) * ua[k1] + A2U_(ib, 3) * ua[k2] + A2U_(ib, 4) * ua[k3];
+  const double cofa7 = A1U_(ib, 1) * va[ib] + A1U_(ib, 2) * va[k1] + A1U_(ib, 3) * va[k2] + A1U_(ib, 4) * va[k3];
+  const double cofa8 = A2U_(ib, 1) * va[ib] + A2U_(ib, 2) * va[k1] + A2U_(ib, 3) * va[k2] + A2U_(ib, 4) * va[k3];
+  xij = g.xijc[e] - g.xc[ib]; yij = g.yijc[e] - g.yc[ib];
+  const double uij2 = ua[ib] + cofa3 * xij + cofa4 * yij;
+  const double vij2 = va[ib] + cofa7 * xij + cofa8 * yij;
+  const double uij = 0.5 * (uij1 + uij2), vij = 0.5 * (vij1 + vij2);
+  const double dlx = g.dltxc[e], dly = g.dltyc[e];
+  const double un_tmp = -uij * dly + vij * dlx;
+  const double viscof1 = g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + 0.5 * sq(cofa2 + cofa5));
+  const double viscof2 = g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + 0.5 * sq(cofa4 + cofa7));
+  const double viscof = (g.fact * 0.5 * (viscof1 * g.cc_hvc[ia] + viscof2 * g.cc_hvc[ib]) +
+                         g.fm1 * 0.5 * (g.cc_hvc[ia] + g.cc_hvc[ib])) / g.hprnu;
+  const double txxij = (cofa1 + cofa3) * viscof;
+  const double tyyij = (cofa6 + cofa8) * viscof;
+  const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
+  const double fxx = dij * (txxij * dly - txyij * dlx);
+  const double fyy = dij * (txyij * dly - tyyij * dlx);
+  const double sgn = fsign1(un_tmp);
+  const double xadv = dij * un_tmp * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
+  const double yadv = dij * un_tmp * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
+  const double isbc_tmp = (double)g.isbc[e];
+  const double ep = g.epor[self], iu = (double)g.iucp[self];
+  const double gd = g.grav_e[self] * d1[self] * elij;
+  if (side == 0) {
+    xf = xf + (xadv + fxx * ep) * (1.0 - isbc_tmp) * iu;
+    yf = yf + (yadv + fyy * ep) * (1.0 - isbc_tmp) * iu;
+    px = px - gd * dly;
+    py = py + gd * dlx;
+  } else {
+    xf = xf - (xadv + fxx * ep) * (1.0 - isbc_tmp) * iu;
+    yf = yf - (yadv + fyy * ep) * (1.0 - isbc_tmp) * iu;
+    px = px + gd * dly;
+    py = py - gd * dlx;
+  }
+}
+
+// ADVAVE_EDGE_GCN for element i: edge gather + OBC flux + rivers.
+__device__ __forceinline__ void advave_element(const DevGrid& g, int i, const double* ua, const double* va,
+                                               const double* d, const double* el, const double* d1,
+                                               const double* fluxobn, const double* qdis,
+                                               double& xf, double& yf, double& px, double& py) {
+  xf = 0.0; yf = 0.0; px = 0.0; py = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int ent = g.ed_ent[3 * i + q];
+    advave_edge(g, ua, va, d, el, d1, ent >> 1, i, ent & 1, xf, yf, px, py);
+  }
+  if (g.isbce[i] == 2) {
+    xf = (xf + fluxobn[i] * ua[i]) * (double)g.iucp[i];
+    yf = (yf + fluxobn[i] * va[i]) * (double)g.iucp[i];
+  }
+  for (int p = g.re_off[i]; p < g.re_off[i + 1]; ++p) {
+    const int k = g.re_riv[p];
+    const double vlctyq = qdis[k] / g.qarea[k];
+    xf = xf - 0.5 * qdis[k] * vlctyq * g.cos_angleq[k];
+    yf = yf - 0.5 * qdis[k] * vlctyq * g.sin_angleq[k];
+  }
+}
+
+__global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > g.N) return;
+  const double one_third = 1.0 / 3.0;
+  const double elf1 = one_third * (a.nxt.el[NV_(i, 1)] + a.nxt.el[NV_(i, 2)] + a.nxt.el[NV_(i, 3)]);
+  // FLUX_OBN: contributions to this open-boundary element, in OBC order
+  if (g.isbce[i] == 2) {
+    double f = 0.0;
+    for (int p = g.fo_off[i]; p < g.fo_off[i + 1]; ++p) {
+      const int ent = g.fo_obc[p], ob = ent >> 1, slot = ent & 1;
+      f = f + g.fluxf_obc[(long)slot * g.IOBCN + ob] * a.fobn[ob];
+    }
+    a.fluxobn[i] = f;
+  }
+  double xf, yf, px, py;
+  advave_element(g, i, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.cur.d1, a.fluxobn, a.qdis, xf, yf, px, py);
+  // EXTUV_EDGE
+  const double ua = a.cur.ua[i], va = a.cur.va[i], d1 = a.cur.d1[i], art = g.art[i], cor = g.cor[i];
+  const double ramp = a.clk->ramp_ext;
+  const double wus2 = g.wind_on ? g.wsx[i] * ramp * 0.001 : 0.0;
+  const double wvs2 = g.wind_on ? g.wsy[i] * ramp * 0.001 : 0.0;
+  const double resx = a.adx2d[i] + xf + a.drx2d[i] + px - cor * va * d1 * art - (wus2 + a.wubot[i]) * art;
+  const double resy = a.ady2d[i] + yf + a.dry2d[i] + py + cor * ua * d1 * art - (wvs2 + a.wvbot[i]) * art;
+  const double h1 = g.h1[i], elrk1 = a.base.el1[i];
+  double uaf = (a.base.ua[i] * (h1 + elrk1) - a.alpha * g.dte * resx / art) / (h1 + elf1);
+  double vaf = (a.base.va[i] * (h1 + elrk1) - a.alpha * g.dte * resy / art) / (h1 + elf1);
+  uaf = uaf / (1.0 + g.cc_sponge[i] * uaf * uaf);
+  vaf = vaf / (1.0 + g.cc_sponge[i] * vaf * vaf);
+  wall_project(g, i, uaf, vaf);
+  const double d1n = h1 + elf1;
+  a.nxt.ua[i] = uaf;
+  a.nxt.va[i] = vaf;
+  a.nxt.el1[i] = elf1;
+  a.nxt.d1[i] = d1n;
+  if (a.K == 3) {
+    a.uard[i] = a.uard[i] + uaf * d1n;
+    a.vard[i] = a.vard[i] + vaf * d1n;
+  }
+}
+
+// Start of EXTERNAL_STEP: clock and RAMP (external_step.F:83-88).
+__global__ void k_ext_clock(DevClock* c, int iext) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  c->iext = iext;
+  c->ext_time = c->time0 + (double)(c->iint - 1) * c->dti + (double)iext * c->dte;
+  if (c->iramp != 0) {
+    const double tmp = (double)(c->iint - 1) + (double)iext / (double)c->isplit;
+    c->ramp_ext = fv_tanh(tmp / (double)c->iramp);
+  } else {
+    c->ramp_ext = 1.0;
+  }
+}

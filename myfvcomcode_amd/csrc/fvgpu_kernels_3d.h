@@ -1,0 +1,554 @@
+// fvgpu_kernels_3d.h -- the baroclinic (internal) mode on the device.
+//
+// Element kernels own one element column each (layers in a loop, so the
+// column sums ADX2D, DRX2D and the prefix sums of BAROPG are formed in the
+// reference's k order); node kernels own one node column.  Horizontal fluxes
+// are gathered through the same sorted edge / sub-edge lists as the 2-D mode.
+#pragma once
+#include "fvgpu_kernels_2d.h"
+
+struct IntArgs {
+  // elements, 3-D
+  double *u, *v, *w, *ww, *uf, *vf, *drhox, *drhoy, *rho, *rmean, *t, *s, *km1, *viscofm;
+  // nodes, 3-D
+  double *t1, *s1, *tf1, *sf1, *rho1, *rmean1, *tmean1, *smean1, *q2, *q2l, *q2f, *q2lf;
+  double *l, *km, *kh, *kq, *wts, *viscofh;
+  // node-layer scratch: ring gradients and bounds
+  double *gx, *gy, *gxd, *gyd, *fmin_, *fmax_;
+  // 2-D
+  double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;
+  double *el, *d, *et, *dt, *egf, *dtfa, *elf;          // nodes (elf = EL after the ext. loop)
+  double *ua, *va, *el1, *d1, *et1, *dt1;               // elements
+  double *pfpxb, *pfpyb, *ah_bottom;
+  double *qdis, *xflux_obc, *uard_obcn;
+  DevClock* clk;
+};
+
+#define GRID1(n) (((n) + 255) / 256), 256
+
+// Start of INTERNAL_STEP (internal_step.F:115-120) and BCOND_GCN(8,0) rivers.
+__global__ void k_int_clock(DevGrid g, DevClock* c, double* qdis) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  c->iint += 1;
+  if (c->iramp != 0) c->ramp_int = fv_tanh((double)c->iint / (double)c->iramp);
+  else c->ramp_int = 1.0;
+  for (int j = 0; j < g.NUMQBC; ++j) qdis[j] = g.qdis0[j] * c->ramp_int;
+}
+
+__global__ void k_zero(double* p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_copy(double* dst, const double* src, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ADJUST2D3D(1) (adjust2d3d.F:52-63), BCOND_GCN(5,0) (bcond_gcn.F:537-616)
+// and BOTTOM_ROUGHNESS (brough.F:88-151) for one element column.
+__global__ void k_adjust1_brough(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > g.N) return;
+  double uac = 0.0, vac = 0.0;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    uac = uac + a.u[EI(g, i, k)] * DZ1(i, k);
+    vac = vac + a.v[EI(g, i, k)] * DZ1(i, k);
+  }
+  const double du = a.ua[i] - uac, dv = a.va[i] - vac;
+  double ub = 0.0, vb = 0.0;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    double uu = a.u[EI(g, i, k)] + 1.0 * du;
+    double vv = a.v[EI(g, i, k)] + 1.0 * dv;
+    wall_project(g, i, uu, vv);
+    a.u[EI(g, i, k)] = uu;
+    a.v[EI(g, i, k)] = vv;
+    ub = uu; vb = vv;
+  }
+  const double vk2 = .160;
+  const double zb = ZZ1(i, g.KBM1) - Z1(i, g.KB);
+  double cbc;
+  if (a.dt1[i] > 3.0) cbc = vk2 / sq(fv_log(zb * a.dt1[i] / g.cc_z0b[i]));
+  else cbc = vk2 / sq(fv_log(zb * 3.0 / g.cc_z0b[i]));
+  if (cbc < g.cbcmin) cbc = g.cbcmin;
+  a.cbc[i] = cbc;
+  const double btps = cbc * sqrt(ub * ub + vb * vb);
+  a.wubot[i] = -btps * ub;
+  a.wvbot[i] = -btps * vb;
+}
+
+// ADVECTION_EDGE_GCN (advection_edge_gcn.F:397-734) summed over the column
+// into ADX2D/ADY2D (internal_step.F:374-384).
+__device__ __forceinline__ void adv3d_edge(const DevGrid& g, const double* U, const double* V,
+                                           const double* dtn, int e, int self, int side, int k,
+                                           double& xf, double& yf) {
+  const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
+  const int k1 = NBE_(ia, 1), k2 = NBE_(ia, 2), k3 = NBE_(ia, 3), k4 = NBE_(ib, 1), k5 = NBE_(ib, 2), k6 = NBE_(ib, 3);
+  const double xija = g.xijc[e] - g.xc[ia], yija = g.yijc[e] - g.yc[ia];
+  const double xijb = g.xijc[e] - g.xc[ib], yijb = g.yijc[e] - g.yc[ib];
+  const double dij = 0.5 * (dtn[j1] * DZ(j1, k) + dtn[j2] * DZ(j2, k));
+  const long ko = (long)(k - 1) * g.n1;
+  const double uia = U[ko + ia], vIA = V[ko + ia], uib = U[ko + ib], vIB = V[ko + ib];
+  const double cofa1 = A1U_(ia, 1) * uia + A1U_(ia, 2) * U[ko + k1] + A1U_(ia, 3) * U[ko + k2] + A1U_(ia, 4) * U[ko + k3];
+  const double cofa2 = A2U_(ia, 1) * uia + A2U_(ia, 2) * U[ko + k1] + A2U_(ia, 3) * U[ko + k2] + A2U_(ia, 4) * U[ko + k3];
+  const double cofa5 = A1U_(ia, 1) * vIA + A1U_(ia, 2) * V[ko + k1] + A1U_(ia, 3) * V[ko + k2] + A1U_(ia, 4) * V[ko + k3];
+  const double cofa6 = A2U_(ia, 1) * vIA + A2U_(ia, 2) * V[ko + k1] + A2U_(ia, 3) * V[ko + k2] + A2U_(ia, 4) * V[ko + k3];
+  const double uij1 = uia + cofa1 * xija + cofa2 * yija;
+  const double vij1 = vIA + cofa5 * xija + cofa6 * yija;
+  const double cofa3 = A1U_(ib, 1) * uib + A1U_(ib, 2) * U[ko + k4] + A1U_(ib, 3) * U[ko + k5] + A1U_(ib, 4) * U[ko + k6];
+  const double cofa4 = A2U_(ib, 1) * uib + A2U_(ib, 2) * U[ko + k4] + A2U_(ib, 3) * U[ko + k5] + A2U_(ib, 4) * U[ko + k6];
+  const double cofa7 = A1U_(ib, 1) * vIB + A1U_(ib, 2) * V[ko + k4] + A1U_(ib, 3) * V[ko + k5] + A1U_(ib, 4) * V[ko + k6];
+  const double cofa8 = A2U_(ib, 1) * vIB + A2U_(ib, 2) * V[ko + k4] + A2U_(ib, 3) * V[ko + k5] + A2U_(ib, 4) * V[ko + k6];
+  const double uij2 = uib + cofa3 * xijb + cofa4 * yijb;
+  const double vij2 = vIB + cofa7 * xijb + cofa8 * yijb;
+  const double uij = 0.5 * (uij1 + uij2), vij = 0.5 * (vij1 + vij2);
+  const double dlx = g.dltxc[e], dly = g.dltyc[e];
+  const double un_tmp = vij * dlx - uij * dly;
+  const double viscof1 = g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + 0.5 * sq(cofa2 + cofa5));
+  const double viscof2 = g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + 0.5 * sq(cofa4 + cofa7));
+  const double viscof = (g.fact * 0.5 * (viscof1 * g.cc_hvc[ia] + viscof2 * g.cc_hvc[ib]) +
+                         g.fm1 * 0.5 * (g.cc_hvc[ia] + g.cc_hvc[ib])) / g.hprnu;
+  const double txxij = (cofa1 + cofa3) * viscof, tyyij = (cofa6 + cofa8) * viscof;
+  const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
+  const double fxx = dij * (txxij * dly - txyij * dlx);
+  const double fyy = dij * (txyij * dly - tyyij * dlx);
+  const double sgn = fsign1(un_tmp);
+  const double xadv = dij * un_tmp * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
+  const double yadv = dij * un_tmp * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
+  const double isbc_tmp = (double)g.isbc[e];
+  const double tp = (double)(1 - g.isbc[e]) * g.epor[self];
+  const double ep = g.epor[self];
+  if (side == 0) {
+    xf = xf + xadv * tp + (fxx + 3.0 * fxx * isbc_tmp) * ep;
+    yf = yf + yadv * tp + (fyy + 3.0 * fyy * isbc_tmp) * ep;
+  } else {
+    xf = xf - xadv * tp - (fxx + 3.0 * fxx * isbc_tmp) * ep;
+    yf = yf - yadv * tp - (fyy + 3.0 * fyy * isbc_tmp) * ep;
+  }
+}
+
+__global__ void k_adv3d_sum(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > g.N) return;
+  int ent[3];
+  for (int q = 0; q < 3; ++q) ent[q] = g.ed_ent[3 * i + q];
+  double sx = 0.0, sy = 0.0;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    double xf = 0.0, yf = 0.0;
+    for (int q = 0; q < 3; ++q) adv3d_edge(g, a.u, a.v, a.dt, ent[q] >> 1, i, ent[q] & 1, k, xf, yf);
+    if (g.isbce[i] == 2) { xf = 0.0; yf = 0.0; }
+    for (int p = g.re_off[i]; p < g.re_off[i + 1]; ++p) {
+      const int r = g.re_riv[p];
+      const int j = g.inodeq[r];
+      const double vlctyq = a.qdis[r] / g.qarea[r];
+      const double vq = g.vqdist[(long)(k - 1) * g.NUMQBC + r];
+      const double temp = 0.5 * a.qdis[r] * vq * vq * vlctyq / DZ(j, k);
+      xf = xf - temp * g.cos_angleq[r];
+      yf = yf - temp * g.sin_angleq[r];
+    }
+    sx = sx + xf;
+    sy = sy + yf;
+  }
+  a.adx2d[i] = sx;
+  a.ady2d[i] = sy;
+}
+
+// BAROPG (baropg.F:33-166) with the RMEAN round trip of RHO applied to the
+// element's own column; the node round trip is k_rho1_roundtrip.
+__global__ void k_baropg(DevGrid g, IntArgs a, int barotropic) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.N) return;
+  if (i == 0) {
+    for (int k = 1; k <= g.KB; ++k) { a.rmean[EI(g, 0, k)] = 0.0; a.rho[EI(g, 0, k)] = 0.0; }
+    return;
+  }
+  if (barotropic) { a.drx2d[i] = 0.0; a.dry2d[i] = 0.0; return; }
+  int nn[3][2];
+  double dyv[3], dxv[3], dij[3];
+  for (int j = 1; j <= 3; ++j) {
+    const int jj1 = j + 1 - ((j + 1) / 4) * 3, jj2 = j + 2 - ((j + 2) / 4) * 3;
+    nn[j - 1][0] = NV_(i, jj1); nn[j - 1][1] = NV_(i, jj2);
+    dyv[j - 1] = g.vy[nn[j - 1][0]] - g.vy[nn[j - 1][1]];
+    dxv[j - 1] = g.vx[nn[j - 1][1]] - g.vx[nn[j - 1][0]];
+    dij[j - 1] = 0.5 * (a.dt[nn[j - 1][0]] + a.dt[nn[j - 1][1]]);
+  }
+  const double dt1 = a.dt1[i], ge = g.grav_e[i], ramp = a.clk->ramp_int;
+  double rprev[3] = {0, 0, 0}, d1c[3] = {0, 0, 0}, d2c = 0.0, rhoprev = 0.0, zzprev = 0.0;
+  double sx = 0.0, sy = 0.0;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const double zzk = ZZ1(i, k);
+    const double rhok = a.rho[EI(g, i, k)] - a.rmean[EI(g, i, k)];
+    double rij[3];
+    for (int j = 0; j < 3; ++j) {
+      const double r0 = a.rho1[NI(g, nn[j][0], k)] - a.rmean1[NI(g, nn[j][0], k)];
+      const double r1 = a.rho1[NI(g, nn[j][1], k)] - a.rmean1[NI(g, nn[j][1], k)];
+      rij[j] = 0.5 * (r0 + r1);
+      if (k == 1) d1c[j] = rij[j] * (-zzk);
+      else {
+        const double t = 0.5 * (rprev[j] + rij[j]) * (zzprev - zzk);
+        d1c[j] = t + d1c[j];
+      }
+    }
+    if (k == 1) d2c = 0.0;
+    else {
+      const double t = 0.5 * (zzprev + zzk) * (rhok - rhoprev);
+      d2c = d2c + t;
+    }
+    double dx = 0.0, dy = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      double drho1 = dyv[j] * d1c[j] * dt1;
+      double drho2 = dyv[j] * dij[j] * d2c;
+      dx = dx + drho1 + drho2;
+      drho1 = dxv[j] * d1c[j] * dt1;
+      drho2 = dxv[j] * dij[j] * d2c;
+      dy = dy + drho1 + drho2;
+    }
+    dx = dx * dt1 * DZ1(i, k) * ge * ramp;
+    dy = dy * dt1 * DZ1(i, k) * ge * ramp;
+    a.drhox[EI(g, i, k)] = dx;
+    a.drhoy[EI(g, i, k)] = dy;
+    sx = sx + dx;
+    sy = sy + dy;
+    for (int j = 0; j < 3; ++j) rprev[j] = rij[j];
+    rhoprev = rhok;
+    zzprev = zzk;
+  }
+  a.drx2d[i] = sx;
+  a.dry2d[i] = sy;
+  for (int k = 1; k <= g.KB; ++k) {
+    const long q = EI(g, i, k);
+    a.rho[q] = (a.rho[q] - a.rmean[q]) + a.rmean[q];
+  }
+}
+
+__global__ void k_rho1_roundtrip(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.M) return;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const long q = NI(g, i, k);
+    a.rho1[q] = (a.rho1[q] - a.rmean1[q]) + a.rmean1[q];
+  }
+}
+
+// Ext-mode ADVAVE of INTERNAL_STEP and GX/GY (internal_step.F:387-392),
+// plus the zeroing of UARD/VARD (internal_step.F:395-397).
+__global__ void k_advave_int(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.N) return;
+  a.uard[i] = 0.0;
+  a.vard[i] = 0.0;
+  if (i < 1) return;
+  double xf, yf, px, py;
+  advave_element(g, i, a.ua, a.va, a.d, a.el, a.d1, a.fluxobn, a.qdis, xf, yf, px, py);
+  a.adx2d[i] = a.adx2d[i] - xf;
+  a.ady2d[i] = a.ady2d[i] - yf;
+}
+
+// ADJUST2D3D(2) (adjust2d3d.F:65-110).
+__global__ void k_adjust2(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.N) return;
+  const double uard = a.uard[i] / (double)g.isplit;
+  const double vard = a.vard[i] / (double)g.isplit;
+  a.uard[i] = uard;
+  a.vard[i] = vard;
+  if (i < 1) return;
+  double utmp = 0.0, vtmp = 0.0;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    utmp = utmp + a.u[EI(g, i, k)] * DZ1(i, k);
+    vtmp = vtmp + a.v[EI(g, i, k)] * DZ1(i, k);
+  }
+  const double dt1 = a.dt1[i];
+  utmp = utmp * dt1;
+  vtmp = vtmp * dt1;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    a.u[EI(g, i, k)] = a.u[EI(g, i, k)] - (utmp - uard) / dt1;
+    a.v[EI(g, i, k)] = a.v[EI(g, i, k)] - (vtmp - vard) / dt1;
+  }
+}
+
+// VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer, column
+// integration of omega, linear mass-error correction, DTFA.
+__global__ void k_vertvl(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.M) return;
+  if (i == 0) {
+    for (int k = 1; k <= g.KB; ++k) a.wts[NI(g, 0, k)] = 0.0;
+    a.dtfa[0] = a.d[0];
+    return;
+  }
+  const int c0 = g.cv_off[i], c1 = g.cv_off[i + 1];
+  const bool obc = g.isonb[i] == 2;
+  const int rv = g.node_riv[i];
+  const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
+  double w = 0.0;
+  a.wts[NI(g, i, 1)] = 0.0;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    double xflux = 0.0;
+    const long ko = (long)(k - 1) * g.n1;
+    for (int p = c0; p < c1; ++p) {
+      const int ent = g.cv_ent[p], c = ent >> 1;
+      const int i1 = g.ntrg[c];
+      const double dij = a.dt1[i1] * DZ1(i1, k);
+      const double ex = dij * (-a.u[ko + i1] * g.dltye[c] + a.v[ko + i1] * g.dltxe[c]);
+      if (ent & 1) xflux = xflux + ex;
+      else xflux = xflux - ex;
+    }
+    if (obc) xflux = 0.0;
+    if (rv >= 0)
+      for (int j = rv; j < g.NUMQBC; ++j)
+        if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+    w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
+    a.wts[NI(g, i, k + 1)] = w;
+  }
+  const double wbottom = 0.0;
+  double dtfa = a.d[i];
+  if (fabs(w - wbottom) > 1.0E-8 && !obc) {
+    double tmp1 = a.elf[i] * (double)g.KBM1 - (w - wbottom) * g.dti / DZ(i, 1);
+    tmp1 = tmp1 / (double)g.KBM1;
+    dtfa = tmp1 + g.h[i];
+    for (int k = 2; k <= g.KB; ++k) {
+      const double ratio = (double)((float)(k - 1) / (float)g.KBM1);
+      a.wts[NI(g, i, k)] = a.wts[NI(g, i, k)] - ratio * (w - wbottom);
+    }
+  }
+  a.dtfa[i] = dtfa;
+}
+
+// N2E3D (mod_main.F:1633-1645) over layers 1..kmax.
+__global__ void k_n2e3d(DevGrid g, const double* nvar, double* evar, int kmax) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long tot = (long)g.N * kmax;
+  if (t >= tot) return;
+  const int i = (int)(t % g.N) + 1, k = (int)(t / g.N) + 1;
+  const double one_third = 1.0 / 3.0;
+  evar[EI(g, i, k)] = one_third * (nvar[NI(g, NV_(i, 1), k)] + nvar[NI(g, NV_(i, 2), k)] + nvar[NI(g, NV_(i, 3), k)]);
+}
+
+// VISCOF_H (viscofh.F:33-127), one node-layer per thread.
+__global__ void k_viscofh(DevGrid g, IntArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.KBM1) return;
+  const int i = (int)(t % g.M) + 1, k = (int)(t / g.M) + 1;
+  const long ko = (long)(k - 1) * g.n1;
+  const int o = g.nbve_off[i], nt = g.ntve[i];
+  const bool bnd = g.isonb[i] != 0;
+  double pupx = 0.0, pupy = 0.0, pvpx = 0.0, pvpy = 0.0;
+  // first element, interior elements, last element: for NTVE==1 the single
+  // element enters twice, as in the reference's three-part loop
+  {
+    const int i1 = g.nbve[o];
+    const double uu = a.u[ko + i1], vv = a.v[ko + i1];
+    pupx = pupx + uu * g.dltyecec[o]; pupy = pupy + uu * g.dltxecec[o];
+    pvpx = pvpx + vv * g.dltyecec[o]; pvpy = pvpy + vv * g.dltxecec[o];
+    if (bnd) {
+      pupx = pupx + uu * g.dltynec[o]; pupy = pupy + uu * g.dltxnec[o];
+      pvpx = pvpx + vv * g.dltynec[o]; pvpy = pvpy + vv * g.dltxnec[o];
+    }
+  }
+  for (int j = 1; j <= nt - 2; ++j) {
+    const int i1 = g.nbve[o + j];
+    const double uu = a.u[ko + i1], vv = a.v[ko + i1];
+    pupx = pupx + uu * g.dltyecec[o + j]; pupy = pupy + uu * g.dltxecec[o + j];
+    pvpx = pvpx + vv * g.dltyecec[o + j]; pvpy = pvpy + vv * g.dltxecec[o + j];
+  }
+  {
+    const int j = nt - 1, i1 = g.nbve[o + j];
+    const double uu = a.u[ko + i1], vv = a.v[ko + i1];
+    pupx = pupx + uu * g.dltyecec[o + j]; pupy = pupy + uu * g.dltxecec[o + j];
+    pvpx = pvpx + vv * g.dltyecec[o + j]; pvpy = pvpy + vv * g.dltxecec[o + j];
+    if (bnd) {
+      pupx = pupx + uu * (-g.dltynec[o + j]); pupy = pupy + uu * (-g.dltxnec[o + j]);
+      pvpx = pvpx + vv * (-g.dltynec[o + j]); pvpy = pvpy + vv * (-g.dltxnec[o + j]);
+    }
+  }
+  const double art1 = g.art1[i];
+  pupx = pupx / art1; pupy = pupy / art1; pvpx = pvpx / art1; pvpy = pvpy / art1;
+  const double tmp1 = sq(pupx) + sq(pvpy);
+  const double tmp2 = 0.5 * sq(pupy + pvpx);
+  a.viscofh[NI(g, i, k)] = sqrt(tmp1 + tmp2) * art1;
+}
+
+// ADV_UV_EDGE_GCN (adv_uv_edge_gcn.F:31-1198) for one element column.
+__global__ void k_advuv(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.N) return;
+  if (i == 0) {
+    for (int k = 1; k <= g.KB; ++k) a.viscofm[EI(g, 0, k)] = 0.0;
+    return;
+  }
+  int ent[3];
+  for (int q = 0; q < 3; ++q) ent[q] = g.ed_ent[3 * i + q];
+  const double art = g.art[i], dt1 = a.dt1[i], d1 = a.d1[i], cor = g.cor[i];
+  const double* U = a.u; const double* V = a.v; const double* W = a.w;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.n1;
+    double xf = 0.0, yf = 0.0, px = 0.0, py = 0.0, vm = 0.0;
+    for (int q = 0; q < 3; ++q) {
+      const int e = ent[q] >> 1, side = ent[q] & 1;
+      const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
+      const double elij = 0.5 * (a.egf[j1] + a.egf[j2]);
+      const int k1 = NBE_(ia, 1), k2 = NBE_(ia, 2), k3 = NBE_(ia, 3), k4 = NBE_(ib, 1), k5 = NBE_(ib, 2), k6 = NBE_(ib, 3);
+      const double dij = 0.5 * (a.dt[j1] * DZ(j1, k) + a.dt[j2] * DZ(j2, k));
+      const double xija = g.xijc[e] - g.xc[ia], yija = g.yijc[e] - g.yc[ia];
+      const double xijb = g.xijc[e] - g.xc[ib], yijb = g.yijc[e] - g.yc[ib];
+      const double uia = U[ko + ia], vIA = V[ko + ia], uib = U[ko + ib], vIB = V[ko + ib];
+      const double cofa1 = A1U_(ia, 1) * uia + A1U_(ia, 2) * U[ko + k1] + A1U_(ia, 3) * U[ko + k2] + A1U_(ia, 4) * U[ko + k3];
+      const double cofa2 = A2U_(ia, 1) * uia + A2U_(ia, 2) * U[ko + k1] + A2U_(ia, 3) * U[ko + k2] + A2U_(ia, 4) * U[ko + k3];
+      const double cofa5 = A1U_(ia, 1) * vIA + A1U_(ia, 2) * V[ko + k1] + A1U_(ia, 3) * V[ko + k2] + A1U_(ia, 4) * V[ko + k3];
+      const double cofa6 = A2U_(ia, 1) * vIA + A2U_(ia, 2) * V[ko + k1] + A2U_(ia, 3) * V[ko + k2] + A2U_(ia, 4) * V[ko + k3];
+      const double uij1 = uia + cofa1 * xija + cofa2 * yija;
+      const double vij1 = vIA + cofa5 * xija + cofa6 * yija;
+      const double cofa3 = A1U_(ib, 1) * uib + A1U_(ib, 2) * U[ko + k4] + A1U_(ib, 3) * U[ko + k5] + A1U_(ib, 4) * U[ko + k6];
+      const double cofa4 = A2U_(ib, 1) * uib + A2U_(ib, 2) * U[ko + k4] + A2U_(ib, 3) * U[ko + k5] + A2U_(ib, 4) * U[ko + k6];
+      const double cofa7 = A1U_(ib, 1) * vIB + A1U_(ib, 2) * V[ko + k4] + A1U_(ib, 3) * V[ko + k5] + A1U_(ib, 4) * V[ko + k6];
+      const double cofa8 = A2U_(ib, 1) * vIB + A2U_(ib, 2) * V[ko + k4] + A2U_(ib, 3) * V[ko + k5] + A2U_(ib, 4) * V[ko + k6];
+      const double uij2 = uib + cofa3 * xijb + cofa4 * yijb;
+      const double vij2 = vIB + cofa7 * xijb + cofa8 * yijb;
+      const double uij = 0.5 * (uij1 + uij2), vij = 0.5 * (vij1 + vij2);
+      const double dlx = g.dltxc[e], dly = g.dltyc[e];
+      const double exflux = dij * (-uij * dly + vij * dlx);
+      const double viscof1 = g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + 0.5 * sq(cofa2 + cofa5));
+      const double viscof2 = g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + 0.5 * sq(cofa4 + cofa7));
+      const double viscof = (g.fact * 0.5 * (viscof1 * g.cc_hvc[ia] + viscof2 * g.cc_hvc[ib]) +
+                             g.fm1 * 0.5 * (g.cc_hvc[ia] + g.cc_hvc[ib])) / g.hprnu;
+      vm = vm + viscof;
+      const double txxij = (cofa1 + cofa3) * viscof, tyyij = (cofa6 + cofa8) * viscof;
+      const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
+      const double fxx = dij * (txxij * dly - txyij * dlx);
+      const double fyy = dij * (txyij * dly - tyyij * dlx);
+      const double sgn = fsign1(exflux);
+      const double xadv = exflux * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
+      const double yadv = exflux * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
+      const double isbc_tmp = (double)g.isbc[e];
+      const double tp = (double)(1 - g.isbc[e]) * g.epor[i];
+      const double ep = g.epor[i];
+      const double gd = g.grav_e[i] * dt1 * DZ1(i, k) * elij;
+      if (side == 0) {
+        xf = xf + xadv * tp + (fxx + 3.0 * fxx * isbc_tmp) * ep;
+        yf = yf + yadv * tp + (fyy + 3.0 * fyy * isbc_tmp) * ep;
+        px = px - gd * dly;
+        py = py + gd * dlx;
+      } else {
+        xf = xf - xadv * tp - (fxx + 3.0 * fxx * isbc_tmp) * ep;
+        yf = yf - yadv * tp - (fyy + 3.0 * fyy * isbc_tmp) * ep;
+        px = px + gd * dly;
+        py = py - gd * dlx;
+      }
+    }
+    xf = xf + px;
+    yf = yf + py;
+    // vertical advection (adv_uv_edge_gcn.F:927-946)
+    const double uk = U[ko + i], vk = V[ko + i], dzk = DZ1(i, k);
+    double xfluxv, yfluxv;
+    if (k == 1) {
+      const long kp = ko + g.n1; const double dzp = DZ1(i, k + 1);
+      xfluxv = -W[kp + i] * (uk * dzp + U[kp + i] * dzk) / (dzk + dzp);
+      yfluxv = -W[kp + i] * (vk * dzp + V[kp + i] * dzk) / (dzk + dzp);
+    } else if (k == g.KBM1) {
+      const long km = ko - g.n1; const double dzm = DZ1(i, k - 1);
+      xfluxv = W[ko + i] * (uk * dzm + U[km + i] * dzk) / (dzk + dzm);
+      yfluxv = W[ko + i] * (vk * dzm + V[km + i] * dzk) / (dzk + dzm);
+    } else {
+      const long kp = ko + g.n1, km = ko - g.n1;
+      const double dzp = DZ1(i, k + 1), dzm = DZ1(i, k - 1);
+      xfluxv = W[ko + i] * (uk * dzm + U[km + i] * dzk) / (dzk + dzm) - W[kp + i] * (uk * dzp + U[kp + i] * dzk) / (dzk + dzp);
+      yfluxv = W[ko + i] * (vk * dzm + V[km + i] * dzk) / (dzk + dzm) - W[kp + i] * (vk * dzp + V[kp + i] * dzk) / (dzk + dzp);
+    }
+    xf = xf + xfluxv * art + a.drhox[ko + i] - cor * vk * dt1 * dzk * art;
+    yf = yf + yfluxv * art + a.drhoy[ko + i] + cor * uk * dt1 * dzk * art;
+    if (g.isbce[i] == 2) { xf = 0.0; yf = 0.0; }
+    for (int p = g.re_off[i]; p < g.re_off[i + 1]; ++p) {
+      const int r = g.re_riv[p];
+      const int j = g.inodeq[r];
+      const double vlctyq = a.qdis[r] / g.qarea[r];
+      const double vq = g.vqdist[(long)(k - 1) * g.NUMQBC + r];
+      const double temp = 0.5 * a.qdis[r] * vq * vq * vlctyq / DZ(j, k);
+      xf = xf - temp * g.cos_angleq[r];
+      yf = yf - temp * g.sin_angleq[r];
+    }
+    a.uf[ko + i] = uk * dt1 / d1 - g.dti * xf / art / (d1 * dzk);
+    a.vf[ko + i] = vk * dt1 / d1 - g.dti * yf / art / (d1 * dzk);
+    a.viscofm[ko + i] = vm / art;
+  }
+  a.viscofm[EI(g, i, g.KB)] = 0.0 / art;
+}
+
+// VDIF_UV (vdif_uv.F:34-189) then BCOND_GCN(3,0) (bcond_gcn.F:424-530).
+__global__ void k_vdif_uv(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > g.N) return;
+  const int KB = g.KB, KBM1 = g.KBM1, KBM2 = g.KBM2;
+  const double dti = g.dti, umol = g.umol, d1 = a.d1[i];
+  double A[FV_MAXKB + 1], C[FV_MAXKB + 1], VHU[FV_MAXKB + 1], VHV[FV_MAXKB + 1], VHPU[FV_MAXKB + 1], VHPV[FV_MAXKB + 1];
+  double* UF = a.uf; double* VF = a.vf;
+  if (d1 > 0.0) {
+    for (int k = 1; k <= KB; ++k) C[k] = a.km1[EI(g, i, k)];
+    for (int k = 2; k <= KBM1; ++k) {
+      A[k - 1] = -dti * (C[k] + umol) / (DZ1(i, k - 1) * DZZ1(i, k - 1) * d1 * d1);
+      C[k] = -dti * (C[k] + umol) / (DZ1(i, k) * DZZ1(i, k - 1) * d1 * d1);
+    }
+    const int it1 = g.isonb[NV_(i, 1)], it2 = g.isonb[NV_(i, 2)], it3 = g.isonb[NV_(i, 3)];
+    const double tmp = (it1 == 2 || it2 == 2 || it3 == 2) ? 0.0 : 1.0;
+    const double ramp = a.clk->ramp_int;
+    const double wus = g.wind_on ? -g.wsx[i] * ramp * 0.001 : 0.0;
+    const double wvs = g.wind_on ? -g.wsy[i] * ramp * 0.001 : 0.0;
+    VHU[1] = A[1] / (A[1] - 1.);
+    VHV[1] = A[1] / (A[1] - 1.);
+    VHPU[1] = (-dti * wus * tmp / (-DZ1(i, 1) * d1) - UF[EI(g, i, 1)]) / (A[1] - 1.);
+    VHPV[1] = (-dti * wvs * tmp / (-DZ1(i, 1) * d1) - VF[EI(g, i, 1)]) / (A[1] - 1.);
+    for (int k = 2; k <= KBM2; ++k) {
+      VHPU[k] = 1.0 / (A[k] + C[k] * (1. - VHU[k - 1]) - 1.);
+      VHPV[k] = 1.0 / (A[k] + C[k] * (1. - VHV[k - 1]) - 1.);
+      VHU[k] = A[k] * VHPU[k];
+      VHV[k] = A[k] * VHPV[k];
+      VHPU[k] = (C[k] * VHPU[k - 1] - UF[EI(g, i, k)]) * VHPU[k];
+      VHPV[k] = (C[k] * VHPV[k - 1] - VF[EI(g, i, k)]) * VHPV[k];
+    }
+    const double ub = a.u[EI(g, i, KBM1)], vb = a.v[EI(g, i, KBM1)];
+    const double tps = a.cbc[i] * sqrt(ub * ub + vb * vb);
+    a.tps[i] = tps;
+    UF[EI(g, i, KBM1)] = (C[KBM1] * VHPU[KBM2] - UF[EI(g, i, KBM1)]) /
+                         (tps * dti / (-DZ1(i, KBM1) * d1) - 1. - (VHU[KBM2] - 1.) * C[KBM1]);
+    VF[EI(g, i, KBM1)] = (C[KBM1] * VHPV[KBM2] - VF[EI(g, i, KBM1)]) /
+                         (tps * dti / (-DZ1(i, KBM1) * d1) - 1. - (VHV[KBM2] - 1.) * C[KBM1]);
+    for (int k = 2; k <= KBM1; ++k) {
+      const int ki = KB - k;
+      UF[EI(g, i, ki)] = (VHU[ki] * UF[EI(g, i, ki + 1)] + VHPU[ki]);
+      VF[EI(g, i, ki)] = (VHV[ki] * VF[EI(g, i, ki + 1)] + VHPV[ki]);
+    }
+  }
+  const double sp = g.cc_sponge[i];
+  for (int k = 1; k <= KBM1; ++k) {
+    double uu = UF[EI(g, i, k)], vv = VF[EI(g, i, k)];
+    uu = uu / (1.0 + sp * uu * uu);
+    vv = vv / (1.0 + sp * vv * vv);
+    wall_project(g, i, uu, vv);
+    UF[EI(g, i, k)] = uu;
+    VF[EI(g, i, k)] = vv;
+  }
+  if (d1 > 0.0) {
+    a.wubot[i] = -a.tps[i] * a.u[EI(g, i, KBM1)];
+    a.wvbot[i] = -a.tps[i] * a.v[EI(g, i, KBM1)];
+  }
+}
+
+// WREAL (wreal.F:32-140).
+__global__ void k_wreal(DevGrid g, IntArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > g.N) return;
+  const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
+  const double dddx = AWX_(i, 1) * a.d[j1] + AWX_(i, 2) * a.d[j2] + AWX_(i, 3) * a.d[j3];
+  const double dddy = AWY_(i, 1) * a.d[j1] + AWY_(i, 2) * a.d[j2] + AWY_(i, 3) * a.d[j3];
+  const double dedx = AWX_(i, 1) * a.elf[j1] + AWX_(i, 2) * a.elf[j2] + AWX_(i, 3) * a.elf[j3];
+  const double dedy = AWY_(i, 1) * a.elf[j1] + AWY_(i, 2) * a.elf[j2] + AWY_(i, 3) * a.elf[j3];
+  const double etf1aa = (1.0 / 3.0) * (a.el[j1] + a.el[j2] + a.el[j3]);
+  const double ww2c = (etf1aa - a.et1[i]) / g.dti;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const long q = EI(g, i, k);
+    const double ww1 = 0.5 * (a.w[q] + a.w[q + g.n1]) + a.u[q] * (ZZ1(i, k) * dddx + dedx) + a.v[q] * (ZZ1(i, k) * dddy + dedy);
+    const double ww2 = (ZZ1(i, k) + 1.) * (etf1aa - a.et1[i]) / g.dti;
+    a.ww[q] = ww1 + ww2;
+    (void)ww2c;
+  }
+}

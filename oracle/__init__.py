@@ -1,0 +1,114 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the serial FP64 oracle.
+
+The oracle restates FVCOM's INTERNAL_STEP / EXTERNAL_STEP
+(FVCOM_source/internal_step.F:29, external_step.F:30) in the reference's
+scatter order (oracle/fvcom_oracle.c).  Only tests/, the smoke check and
+the benchmark's CPU leg may load it; the product never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libfvoracle.so")
+_lib = None
+
+
+def build(verbose=False):
+    subprocess.run(["make", "-s", "-C", _HERE], check=True,
+                   stdout=None if verbose else subprocess.DEVNULL)
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        vp = C.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [vp]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_internal_step.argtypes = [vp]
+        L.orc_run.argtypes = [vp, C.c_longlong]
+        L.orc_external_only.argtypes = [vp, C.c_int]
+        L.orc_get.argtypes = [vp, C.c_char_p, vp, C.c_longlong]
+        L.orc_put.argtypes = [vp, C.c_char_p, vp, C.c_longlong]
+        L.orc_iint.restype = C.c_longlong
+        L.orc_iint.argtypes = [vp]
+        L.orc_depth_fail.argtypes = [vp]
+        L.orc_budgets.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """FP64 CPU state of one model, stepped by the oracle."""
+
+    def __init__(self, model):
+        self.model = model  # keep the grid alive
+        self._o = C.c_void_p(lib().orc_create(model.ptr))
+
+    def internal_step(self):
+        lib().orc_internal_step(self._o)
+
+    def run(self, nsteps):
+        lib().orc_run(self._o, int(nsteps))
+
+    def external_step(self, iext=1):
+        lib().orc_external_only(self._o, int(iext))
+
+    def _len(self, name):
+        m1, n1, kb = self.model.m + 1, self.model.n + 1, self.model.kb
+        node2 = {"el", "et", "d", "dt", "elf", "dtfa", "egf"}
+        elem2 = {"ua", "va", "el1", "et1", "d1", "dt1", "uard", "vard", "adx2d", "ady2d", "drx2d",
+                 "dry2d", "advua", "advva", "pstx", "psty", "cbc", "wubot", "wvbot", "wusurf",
+                 "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf"}
+        elem3 = {"u", "v", "w", "ww", "uf", "vf", "advx", "advy", "drhox", "drhoy", "rho", "rmean",
+                 "t", "s", "km1", "viscofm"}
+        if name in node2:
+            return m1
+        if name in elem2:
+            return n1
+        if name in elem3:
+            return n1 * kb
+        return m1 * kb
+
+    def get(self, name):
+        n = self._len(name)
+        out = np.empty(n, np.float64)
+        if lib().orc_get(self._o, name.encode(), out.ctypes.data, n) != 0:
+            raise KeyError(name)
+        return out
+
+    def put(self, name, values):
+        v = np.ascontiguousarray(values, np.float64)
+        if lib().orc_put(self._o, name.encode(), v.ctypes.data, len(v)) != 0:
+            raise KeyError(name)
+
+    @property
+    def iint(self):
+        return lib().orc_iint(self._o)
+
+    @property
+    def depth_fail(self):
+        return bool(lib().orc_depth_fail(self._o))
+
+    def budgets(self):
+        v, s, h = C.c_double(), C.c_double(), C.c_double()
+        lib().orc_budgets(self._o, C.byref(v), C.byref(s), C.byref(h))
+        return v.value, s.value, h.value
+
+    def close(self):
+        if self._o is not None and self._o.value:
+            lib().orc_destroy(self._o)
+            self._o = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
