@@ -1,0 +1,138 @@
+"""Benchmark of the MI355X mode-split time step (one GPU).
+
+Workload: the 1001x501-node jittered mesh (1,000,000 elements x 40 sigma
+layers, KB=41), depth 5+195*(x/Lx)^1.5 with 20 seeded seamounts, M2/S2/K1
+tide on the east boundary, constant seeded wind field, MY2.5 closure,
+DTE = 2 s, ISPLIT = 10 (DTI = 20 s), synthetic seeded data.
+
+A step is one INTERNAL_STEP (ISPLIT external steps included).  The timed
+region runs K steps with the state resident in HBM; HIP events around every
+launch of the dominant kernel family give its average device time for the
+roofline figure.  The CPU baseline is the serial FP64 oracle on a bounded
+sample of the same workload (the same generator at 251x126 nodes, 62,500
+elements x 40 layers), scaled to the metric's unit.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sim-days/wall-hour @1M elem x 40 layers; elem-layer-steps/s vs plain FP64"
+PEAK_HBM = 8.0e12
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--kernel", default="ext_momentum", help="kernel family timed with HIP events")
+    ap.add_argument("--nx", type=int, default=1001)
+    ap.add_argument("--ny", type=int, default=501)
+    ap.add_argument("--kb", type=int, default=41)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time graph replays instead of eager launches")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the path would shard by domain "
+              "decomposition (see DESIGN.md) but no multi-GPU run is provided", file=sys.stderr)
+        sys.exit(2)
+
+    import myfvcomcode_amd as F
+    from myfvcomcode_amd import roofline
+
+    t0 = time.time()
+    model = F.Model.build(1, nx=args.nx, ny=args.ny, kb=args.kb)
+    n, kbm1 = model.n, model.kb - 1
+    dti = model.hdr.p.dte * model.hdr.p.isplit
+    print("bench: built %d nodes / %d elements x %d layers in %.1f s" % (model.m, n, kbm1, time.time() - t0),
+          file=sys.stderr, flush=True)
+    dev = F.DeviceModel(model)
+    dev.run(args.warmup, graph=args.graph)
+    dev.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+
+    dev.set_profiling(True, kernel=args.kernel)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.run(args.steps, graph=False)
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    dev.set_profiling(False)
+    avg_us, count = dev.kernel_time(args.kernel)
+    diag = dev.diagnostics()
+    if diag.nonfinite:
+        print("bench: non-finite state", file=sys.stderr)
+        sys.exit(1)
+
+    rate = n * kbm1 * args.steps / wall                      # element-layer-steps/s
+    sdph = rate * dti / (24.0 * n * kbm1)                     # simulated days per wall-hour
+    bytes_launch = roofline.bytes_per_launch(args.kernel, model)
+    achieved = bytes_launch / (avg_us * 1e-6) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "%s_traffic.json" % args.kernel)
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_steps, args.kb)
+
+    out = {
+        "metric": METRIC,
+        "value": sdph,
+        "unit": "sim-days/wall-hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered mesh, seamounts, M2/S2/K1 tide, Gaussian random wind)",
+        "config": {"workload": "FVCOM INTERNAL_STEP, %dx%d-node jittered mesh, %d elements x %d layers, "
+                               "DTE=2 s ISPLIT=10, MY2.5, plain FP64" % (args.nx, args.ny, n, kbm1),
+                   "elements": n, "layers": kbm1, "dti_s": dti, "mode": "graph" if args.graph else "eager"},
+        "elem_layer_steps_per_s": rate,
+        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved * 1e9 / PEAK_HBM, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_us": avg_us,
+                     "launches": count},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(steps, kb):
+    """Serial FP64 oracle on the 251x126-node sample of the same generator."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    m = F.Model.build(1, nx=251, ny=126, kb=kb)
+    o = Oracle(m)
+    o.internal_step()                 # first step outside the clock (page-in)
+    t0 = time.perf_counter()
+    o.run(steps)
+    wall = time.perf_counter() - t0
+    n, kbm1 = m.n, m.kb - 1
+    rate = n * kbm1 * steps / wall
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    sdph_1m = rate * dti / (24.0 * 1000000 * 40)
+    return {"value": sdph_1m, "unit": "sim-days/wall-hour (scaled to 1M x 40)", "cores": 1, "kind": "port",
+            "elem_layer_steps_per_s": rate,
+            "sample": "oracle (serial FP64 C restatement, -O3) on the config-1 generator at 251x126 nodes "
+                      "(%d elements x %d layers), %d internal steps, %.1f s" % (n, kbm1, steps, wall)}
+
+
+if __name__ == "__main__":
+    main()

@@ -224,6 +224,14 @@ int  fv_model_field(const fv_model* mdl, const char* name, void** ptr,
 /* Number of named fields, and the i-th name (for iteration).             */
 int32_t     fv_model_nfields(void);
 const char* fv_model_field_name(int32_t i);
+/* Spectral tide of ntide constituents, the same amplitude and phase at
+ * every OBC node (BCOND_ASL, mod_obcs.F:552-582).                        */
+int  fv_model_set_tide(fv_model* mdl, int32_t ntide, const double* period,
+                       const double* amp, const double* phase_deg);
+/* Add a river at a solid-boundary node (RIVER_INFLOW_LOCATION='node',
+ * mod_force.F:3470-3511): discharge q (m^3/s), temperature t, salinity s,
+ * vertical distribution proportional to DZ.                             */
+int  fv_model_set_river(fv_model* mdl, int32_t node, double q, double t, double s);
 /* Cold start: EL=0, rest, DENS2 density, RMEAN/TMEAN/SMEAN, default
  * turbulence (mod_startup.F:1062-1181, 1332-1408).                       */
 int  fv_model_cold_start(fv_model* mdl);
@@ -268,6 +276,8 @@ int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
  * handle's stream.  Returns non-zero if the name is unknown.             */
 int  fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t* count);
 int  fvgpu_set_profiling(fvgpu_handle* h, int32_t on);
+/* Restrict the timing events to one kernel family (NULL or "": all).     */
+int  fvgpu_profile_only(fvgpu_handle* h, const char* name);
 int  fvgpu_synchronize(fvgpu_handle* h);
 void fvgpu_destroy(fvgpu_handle* h);
 int  fvgpu_device_count(int32_t* n);

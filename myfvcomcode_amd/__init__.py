@@ -86,6 +86,8 @@ def grid_lib():
         L.fv_model_field_name.restype = C.c_char_p
         L.fv_model_field_name.argtypes = [C.c_int32]
         L.fv_model_cold_start.argtypes = [C.c_void_p]
+        L.fv_model_set_tide.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fv_model_set_river.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.fv_extent_len.restype = C.c_int64
         L.fv_extent_len.argtypes = [C.c_void_p, C.c_int32]
         _grid = L
@@ -111,6 +113,7 @@ def gpu_lib():
         L.fvgpu_diagnostics.argtypes = [vp, C.POINTER(FvDiag)]
         L.fvgpu_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
+        L.fvgpu_profile_only.argtypes = [vp, C.c_char_p]
         L.fvgpu_synchronize.argtypes = [vp]
         L.fvgpu_destroy.argtypes = [vp]
         L.fvgpu_device_count.argtypes = [C.POINTER(C.c_int32)]
@@ -206,6 +209,26 @@ class Model:
         a = self.field(name)
         return a.reshape(self.kb, -1)
 
+    def set_tide(self, periods, amps, phases_deg):
+        """Spectral tide, uniform along the open boundary."""
+        L = grid_lib()
+        p = np.ascontiguousarray(periods, np.float64)
+        a = np.ascontiguousarray(amps, np.float64)
+        f = np.ascontiguousarray(phases_deg, np.float64)
+        _check(L, L.fv_model_set_tide(self._ptr, len(p), p.ctypes.data, a.ctypes.data, f.ctypes.data),
+               "fv_model_set_tide")
+
+    def set_river(self, node, q, t=15.0, s=0.0):
+        """Node river at a solid-boundary node (1-based index)."""
+        L = grid_lib()
+        _check(L, L.fv_model_set_river(self._ptr, int(node), float(q), float(t), float(s)),
+               "fv_model_set_river")
+
+    def cold_start(self):
+        """Recompute the cold-start state from the current T1/S1/EL fields."""
+        L = grid_lib()
+        _check(L, L.fv_model_cold_start(self._ptr), "fv_model_cold_start")
+
     def field_names(self):
         L = grid_lib()
         return [L.fv_model_field_name(i).decode() for i in range(L.fv_model_nfields())]
@@ -288,8 +311,10 @@ class DeviceModel:
                "fvgpu_kernel_time")
         return us.value, cnt.value
 
-    def set_profiling(self, on):
+    def set_profiling(self, on, kernel=None):
+        """HIP events around launches (all families, or only ``kernel``)."""
         L = gpu_lib()
+        _check(L, L.fvgpu_profile_only(self._h, (kernel or "").encode()), "fvgpu_profile_only")
         _check(L, L.fvgpu_set_profiling(self._h, 1 if on else 0), "fvgpu_set_profiling")
 
     def synchronize(self):

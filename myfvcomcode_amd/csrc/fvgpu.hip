@@ -59,6 +59,7 @@ struct fvgpu_handle {
   hipGraphExec_t gexec = nullptr;
   // profiling
   bool profiling = false;
+  std::string profile_only;    // empty: every kernel family
   std::map<std::string, KernelStat> stats;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
@@ -107,11 +108,13 @@ struct Launch {
   fvgpu_handle* h;
   const char* name;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool on;
   Launch(fvgpu_handle* h_, const char* n) : h(h_), name(n) {
-    if (h->profiling) { e0 = next_event(h); hipEventRecord(e0, h->stream); }
+    on = h->profiling && (h->profile_only.empty() || h->profile_only == n);
+    if (on) { e0 = next_event(h); hipEventRecord(e0, h->stream); }
   }
   ~Launch() {
-    if (h->profiling) {
+    if (on) {
       e1 = next_event(h);
       hipEventRecord(e1, h->stream);
       h->pending.push_back({name, {e0, e1}});
@@ -537,9 +540,11 @@ int fvgpu_run(fvgpu_handle* h, int64_t nsteps, int32_t use_graph) {
   if (!use_graph || h->profiling) {
     for (int64_t s = 0; s < nsteps; ++s) {
       enqueue_internal_step(h);
-      if (h->profiling && flush_profile(h)) return 1;
+      // bound the number of outstanding timing events without a per-step sync
+      if (h->profiling && h->pending.size() > 20000 && flush_profile(h)) return 1;
     }
     HIPCHK(hipGetLastError());
+    if (h->profiling) return flush_profile(h);
     return 0;
   }
   if (!h->gexec) {
@@ -584,8 +589,14 @@ int fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t
 }
 
 int fvgpu_set_profiling(fvgpu_handle* h, int32_t on) {
+  if (h->profiling && flush_profile(h)) return 1;
   h->profiling = on != 0;
   h->stats.clear();
+  return 0;
+}
+
+int fvgpu_profile_only(fvgpu_handle* h, const char* name) {
+  h->profile_only = name ? name : "";
   return 0;
 }
 

@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -614,7 +615,56 @@ extern "C" int fv_model_from_mesh(int32_t m, int32_t n, const double* x, const d
   md->p = *p;
   int rc = build_metrics(md, x, y, nv_rows, h, iobcn, obc_nodes);
   if (rc) { fv_model_free(md); return rc; }
+  // constant physics coefficients (mod_setup.F:314-371, 402-427)
+  for (int i = 0; i <= n; ++i) {
+    md->cc_hvc[i] = i ? 0.2 : 0.0;
+    md->cc_z0b[i] = i ? 0.001 : 0.0;
+    md->cor[i] = i ? 1.0e-4 : 0.0;
+    md->cc_sponge[i] = 0.0;
+  }
+  for (int i = 0; i <= m; ++i) md->nn_hvc[i] = i ? 0.2 : 0.0;
+  rc = fv_model_cold_start(md);
+  if (rc) { fv_model_free(md); return rc; }
   *out = md;
+  return 0;
+}
+
+extern "C" int fv_model_set_river(fv_model* md, int32_t node, double q, double t, double s) {
+  const int KBM1 = md->kbm1;
+  const size_t m1 = md->m + 1;
+  if (node < 1 || node > md->m) return fail("river node out of range");
+  if (md->isonb[node] != 1) return fail("river node is not on the solid boundary");
+  const int k = md->numqbc;
+  auto grow = [&](auto*& a, size_t n) {  // n = k + 1 entries
+    using T = std::remove_reference_t<decltype(*a)>;
+    T* b = (T*)xalloc(sizeof(T) * n);
+    std::memcpy(b, a, sizeof(T) * (size_t)k);  // keep the k existing rivers
+    std::free(a);
+    a = b;
+  };
+  grow(md->inodeq, k + 1); grow(md->qarea, k + 1); grow(md->angleq, k + 1);
+  grow(md->qdis0, k + 1); grow(md->tdis, k + 1); grow(md->sdis, k + 1);
+  {
+    double* v = (double*)xalloc(sizeof(double) * (size_t)(k + 1) * KBM1);
+    for (int l = 0; l < KBM1; ++l)
+      for (int j = 0; j < k; ++j) v[(size_t)l * (k + 1) + j] = md->vqdist[(size_t)l * k + j];
+    std::free(md->vqdist);
+    md->vqdist = v;
+  }
+  md->numqbc = k + 1;
+  md->inodeq[k] = node; md->qdis0[k] = q; md->tdis[k] = t; md->sdis[k] = s;
+  // QAREA / ANGLEQ of a node river (mod_force.F:3470-3511)
+  const int o = md->nbsn_off[node];
+  const int i1 = md->nbsn[o + 1], i2 = md->nbsn[o + 0], i3 = md->nbsn[o + md->ntsn[node] - 2];
+  const double dy12 = md->vy[i1] - md->vy[i2], dy32 = md->vy[i3] - md->vy[i2];
+  const double dx12 = md->vx[i1] - md->vx[i2], dx32 = md->vx[i3] - md->vx[i2];
+  double a1 = std::atan2(dy12, dx12);
+  const double a2 = std::atan2(dy32, dx32);
+  if (a1 < a2) a1 = a1 + 2.0 * 3.1415927;
+  const double dxy = std::sqrt(dx12 * dx12 + dy12 * dy12) + std::sqrt(dx32 * dx32 + dy32 * dy32);
+  md->qarea[k] = 0.5 * dxy * md->h[node];
+  md->angleq[k] = (a1 - a2) / 2. + a2;
+  for (int l = 0; l < KBM1; ++l) md->vqdist[(size_t)l * (k + 1) + k] = md->dz[(size_t)l * m1 + node];
   return 0;
 }
 
@@ -734,26 +784,8 @@ extern "C" int fv_model_build(const fv_case* c, fv_model** out) {
   for (int i = 0; i <= M; ++i) md->nn_hvc[i] = i ? 0.2 : 0.0;
   // river at the west-end node (middle of the west wall)
   if (c->river) {
-    md->numqbc = 1;
-    std::free(md->inodeq); std::free(md->qarea); std::free(md->angleq); std::free(md->qdis0);
-    std::free(md->tdis); std::free(md->sdis); std::free(md->vqdist);
-    md->inodeq = (int32_t*)xalloc(4); md->qarea = (double*)xalloc(8); md->angleq = (double*)xalloc(8);
-    md->qdis0 = (double*)xalloc(8); md->tdis = (double*)xalloc(8); md->sdis = (double*)xalloc(8);
-    md->vqdist = (double*)xalloc(8 * KBM1);
-    int jn = (ny / 2) * nx + 1;
-    md->inodeq[0] = jn; md->qdis0[0] = c->river_q; md->tdis[0] = c->river_t; md->sdis[0] = c->river_s;
-    if (md->isonb[jn] != 1) { fv_model_free(md); return fail("river node is not on the solid boundary"); }
-    // mod_force.F:3470-3511
-    const int o = md->nbsn_off[jn];
-    int i1 = md->nbsn[o + 1], i2 = md->nbsn[o + 0], i3 = md->nbsn[o + md->ntsn[jn] - 2];
-    double dy12 = md->vy[i1] - md->vy[i2], dy32 = md->vy[i3] - md->vy[i2];
-    double dx12 = md->vx[i1] - md->vx[i2], dx32 = md->vx[i3] - md->vx[i2];
-    double a1 = std::atan2(dy12, dx12), a2 = std::atan2(dy32, dx32);
-    if (a1 < a2) a1 = a1 + 2.0 * 3.1415927;
-    double dxy = std::sqrt(dx12 * dx12 + dy12 * dy12) + std::sqrt(dx32 * dx32 + dy32 * dy32);
-    md->qarea[0] = 0.5 * dxy * md->h[jn];
-    md->angleq[0] = (a1 - a2) / 2. + a2;
-    for (int k = 0; k < KBM1; ++k) md->vqdist[k] = md->dz[(size_t)k * m1 + jn];
+    const int jn = (ny / 2) * nx + 1;
+    if (fv_model_set_river(md, jn, c->river_q, c->river_t, c->river_s)) { fv_model_free(md); return 1; }
   }
   // wind field at elements
   if (c->wind) {
@@ -786,6 +818,24 @@ extern "C" int fv_model_build(const fv_case* c, fv_model** out) {
   rc = fv_model_cold_start(md);
   if (rc) { fv_model_free(md); return rc; }
   *out = md;
+  return 0;
+}
+
+extern "C" int fv_model_set_tide(fv_model* md, int32_t ntide, const double* period,
+                                 const double* amp, const double* phase_deg) {
+  if (ntide < 0 || ntide > 16) return fail("ntide out of range");
+  std::free(md->apt); std::free(md->phai); std::free(md->period);
+  md->ntide = ntide;
+  md->apt = (double*)xalloc(sizeof(double) * md->iobcn * ntide);
+  md->phai = (double*)xalloc(sizeof(double) * md->iobcn * ntide);
+  md->period = (double*)xalloc(sizeof(double) * ntide);
+  for (int t = 0; t < ntide; ++t) {
+    md->period[t] = period[t];
+    for (int k = 0; k < md->iobcn; ++k) {
+      md->apt[(size_t)t * md->iobcn + k] = amp[t];
+      md->phai[(size_t)t * md->iobcn + k] = phase_deg[t];
+    }
+  }
   return 0;
 }
 

@@ -1,0 +1,54 @@
+"""The C-ABI libraries load and export every function include/fvgpu.h
+declares (no compute calls: this runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "fvgpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s+\*?([a-z][a-z0-9_]+)\s*\(",
+                       src, flags=re.M)
+    return sorted(set(n for n in names if n.startswith(("fv_", "fvgpu_"))))
+
+
+def test_header_lists_entry_points():
+    names = declared_functions()
+    for must in ("fvgpu_create", "fvgpu_internal_step", "fvgpu_external_step", "fvgpu_get_state",
+                 "fvgpu_put_state", "fvgpu_diagnostics", "fvgpu_destroy", "fv_model_build"):
+        assert must in names
+
+
+def test_libraries_export_every_declared_symbol(built):
+    from myfvcomcode_amd import build
+    grid = os.path.join(ROOT, "myfvcomcode_amd", "lib", "libfvgrid.so")
+    gpu = os.path.join(ROOT, "myfvcomcode_amd", "lib", "libfvgpu.so")
+    if not os.path.exists(gpu):
+        build.build_gpu()
+    lg, lgpu = C.CDLL(grid), C.CDLL(gpu)
+    missing = []
+    for n in declared_functions():
+        lib = lgpu if n.startswith("fvgpu_") else lg
+        if not hasattr(lib, n):
+            missing.append(n)
+    assert not missing, missing
+    assert hasattr(lgpu, "fv_last_error")
+
+
+def test_gpu_library_fails_loudly_without_device(built):
+    """Without a HIP device fvgpu_create must return an error, never fall
+    back to a CPU path."""
+    import myfvcomcode_amd as F
+    L = F.gpu_lib()
+    n = C.c_int32(0)
+    rc = L.fvgpu_device_count(C.byref(n))
+    if rc == 0 and n.value > 0:
+        pytest.skip("a GPU is present")
+    m = F.Model.build(0, nx=9, ny=5, kb=4)
+    with pytest.raises(RuntimeError):
+        F.DeviceModel(m)
