@@ -20,7 +20,8 @@
  * (mod_main.F:1783,1791) are stored as CSR in the reference's ring order.
  *
  * Every entry point returns 0 on success and a non-zero status otherwise;
- * fv_last_error() returns the message of the last failure on this thread.
+ * fv_last_error() (libfvgrid) and fvgpu_last_error() (libfvgpu) return the
+ * message of the last failure on this thread.
  */
 #ifndef FVGPU_H
 #define FVGPU_H
@@ -238,6 +239,9 @@ int  fv_model_cold_start(fv_model* mdl);
 
 /* ------------------------------- libfvgpu ------------------------------- */
 typedef struct fvgpu_handle fvgpu_handle;
+
+/* Message of the last failed libfvgpu call on this thread. */
+const char* fvgpu_last_error(void);
 
 /* Diagnostics gathered on the device (SHUTDOWN_CHECK, DEPTH_CHECK and the
  * FP64 budgets; FVCOM_source/mod_utils.F:320-432, depth_check.F).        */

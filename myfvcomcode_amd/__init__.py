@@ -103,7 +103,8 @@ def gpu_lib():
             raise RuntimeError("libfvgpu.so (HIP) is not built; run __graft_entry__.build()")
         L = C.CDLL(path)
         vp = C.c_void_p
-        L.fv_last_error.restype = C.c_char_p
+        L.fvgpu_last_error.restype = C.c_char_p
+        L.fv_last_error = L.fvgpu_last_error
         L.fvgpu_create.argtypes = [vp, C.c_int32, C.POINTER(vp)]
         L.fvgpu_internal_step.argtypes = [vp]
         L.fvgpu_run.argtypes = [vp, C.c_int64, C.c_int32]

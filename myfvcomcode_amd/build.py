@@ -49,9 +49,27 @@ def build_gpu(verbose=False):
     return out
 
 
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose=False):
+    """The Fortran driver (fvcom_gpu) over the ISO_C_BINDING module."""
+    fdir = os.path.join(HERE, "fortran")
+    mdir = os.path.join(LIB, "_fmod")
+    os.makedirs(mdir, exist_ok=True)
+    out = os.path.join(LIB, "fvcom_gpu")
+    _run([FC, "-O2", "-J", mdir, "-c", os.path.join(fdir, "fvgpu_mod.f90"),
+          "-o", os.path.join(mdir, "fvgpu_mod.o")], verbose)
+    _run([FC, "-O2", "-J", mdir, os.path.join(fdir, "fvcom_driver.f90"), os.path.join(mdir, "fvgpu_mod.o"),
+          "-L" + LIB, "-lfvgpu", "-lfvgrid", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(verbose=False):
     build_grid(verbose)
     build_gpu(verbose)
+    build_fortran(verbose)
 
 
 if __name__ == "__main__":

@@ -488,7 +488,7 @@ double* lookup(fvgpu_handle* h, const std::string& n, size_t* len) {
 // ---------------------------------------------------------------- C-ABI
 extern "C" {
 
-const char* fv_last_error(void) { return g_err.c_str(); }
+const char* fvgpu_last_error(void) { return g_err.c_str(); }
 
 int fvgpu_device_count(int32_t* n) {
   int c = 0;

@@ -37,7 +37,7 @@ def test_libraries_export_every_declared_symbol(built):
         if not hasattr(lib, n):
             missing.append(n)
     assert not missing, missing
-    assert hasattr(lgpu, "fv_last_error")
+    assert hasattr(lgpu, "fvgpu_last_error") and hasattr(lg, "fv_last_error")
 
 
 def test_gpu_library_fails_loudly_without_device(built):
