@@ -283,6 +283,10 @@ int  fvgpu_set_profiling(fvgpu_handle* h, int32_t on);
 /* Restrict the timing events to one kernel family (NULL or "": all).     */
 int  fvgpu_profile_only(fvgpu_handle* h, const char* name);
 int  fvgpu_synchronize(fvgpu_handle* h);
+/* Measurement aid: average time of a plain device copy of n doubles
+ * (k_copy, the same launch shape as the step's copies) over reps launches;
+ * calibrates HBM byte counters and gives the achievable copy bandwidth.   */
+int  fvgpu_stream_copy(fvgpu_handle* h, int64_t n, int32_t reps, double* avg_us);
 void fvgpu_destroy(fvgpu_handle* h);
 int  fvgpu_device_count(int32_t* n);
 

@@ -116,6 +116,7 @@ def gpu_lib():
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
         L.fvgpu_profile_only.argtypes = [vp, C.c_char_p]
         L.fvgpu_synchronize.argtypes = [vp]
+        L.fvgpu_stream_copy.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
         L.fvgpu_destroy.argtypes = [vp]
         L.fvgpu_device_count.argtypes = [C.POINTER(C.c_int32)]
         _gpu = L
@@ -317,6 +318,13 @@ class DeviceModel:
         L = gpu_lib()
         _check(L, L.fvgpu_profile_only(self._h, (kernel or "").encode()), "fvgpu_profile_only")
         _check(L, L.fvgpu_set_profiling(self._h, 1 if on else 0), "fvgpu_set_profiling")
+
+    def stream_copy(self, n_doubles, reps=20):
+        """Average microseconds of a plain copy of n doubles (k_copy)."""
+        L = gpu_lib()
+        us = C.c_double()
+        _check(L, L.fvgpu_stream_copy(self._h, int(n_doubles), int(reps), C.byref(us)), "fvgpu_stream_copy")
+        return us.value
 
     def synchronize(self):
         L = gpu_lib()

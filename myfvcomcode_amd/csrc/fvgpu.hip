@@ -16,7 +16,7 @@
 //    and one internal step can be captured as a replayable hipGraph;
 //  * time-dependent scalars (RAMP, ExtTime) live in a device clock updated
 //    by single-thread kernels, never by host writes between launches.
-#include "fvgpu_kernels_tracer.h"
+#include "fvgpu_kernels_column.h"
 #include "../../include/fvgpu.h"
 
 #include <cmath>
@@ -200,7 +200,9 @@ void enqueue_internal_step(fvgpu_handle* h) {
   LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
   LAUNCH("viscofh", k_viscofh, nb((long)g.M * g.KBM1), 256, g, a);
   LAUNCH("advuv", k_advuv, nb(g.N + 1), 256, g, a);
-  LAUNCH("vdif_uv", k_vdif_uv, nb(g.N + 1), 256, g, a);
+#define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
+  FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
+  LAUNCH("uv_finish", k_uv_finish, nb((long)g.N * g.KBM1), 256, g, a);
   LAUNCH("wreal", k_wreal, nb(g.N + 1), 256, g, a);
   const double qmean1 = (double)1.E-8f;
   if (h->md->p.vertical_closure) {
@@ -212,7 +214,8 @@ void enqueue_internal_step(fvgpu_handle* h) {
       LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2, a.q2f);
       LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2l, a.q2lf);
     }
-    LAUNCH("vdif_q", k_vdif_q, nb(g.M + 1), 256, g, a);
+#define VDIF_Q_CALL(KM) LAUNCH("vdif_q", k_vdif_q_r<KM>, nb(g.M + 1), 256, g, a, a.gx)
+    FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
     LAUNCH("copy", k_copy, nb(mk), 256, a.q2l, a.q2lf, mk);
   }
@@ -227,7 +230,8 @@ void enqueue_internal_step(fvgpu_handle* h) {
     LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * g.KBM1), 256, g, a, f1, fm, 0.0, 1, g.KBM1);
     LAUNCH("adv_ts", k_adv_ts, nb(g.M + 1), 256, g, a, f1, ff, fdis, which);
     if (h->md->p.scalar_positivity) LAUNCH("fct_ts", k_fct_ts, nb(g.M + 1), 256, g, f1, ff);
-    LAUNCH("vdif_ts", k_vdif_ts, nb(g.M + 1), 256, g, a, ff, which);
+#define VDIF_TS_CALL(KM) LAUNCH("vdif_ts", k_vdif_ts_r<KM>, nb(g.M + 1), 256, g, a, ff, which)
+    FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
     LAUNCH("bcond_ts", k_bcond_ts, nb(g.IOBCN + 1), 256, g, a, which);
     LAUNCH("copy", k_copy, nb(mk), 256, f1, ff, mk);
     LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, f1, which == 1 ? a.t : a.s, g.KB);
@@ -597,6 +601,30 @@ int fvgpu_set_profiling(fvgpu_handle* h, int32_t on) {
 
 int fvgpu_profile_only(fvgpu_handle* h, const char* name) {
   h->profile_only = name ? name : "";
+  return 0;
+}
+
+int fvgpu_stream_copy(fvgpu_handle* h, int64_t n, int32_t reps, double* avg_us) {
+  double *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc((void**)&a, n * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&b, n * sizeof(double)));
+  hipLaunchKernelGGL(k_zero, dim3(nb(n)), dim3(256), 0, h->stream, a, (long)n);
+  hipLaunchKernelGGL(k_copy, dim3(nb(n)), dim3(256), 0, h->stream, b, a, (long)n);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy, dim3(nb(n)), dim3(256), 0, h->stream, b, a, (long)n);
+  HIPCHK(hipEventRecord(e1, h->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_us = 1000.0 * ms / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
   return 0;
 }
 

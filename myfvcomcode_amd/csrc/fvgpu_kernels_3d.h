@@ -474,65 +474,6 @@ __global__ void k_advuv(DevGrid g, IntArgs a) {
   a.viscofm[EI(g, i, g.KB)] = 0.0 / art;
 }
 
-// VDIF_UV (vdif_uv.F:34-189) then BCOND_GCN(3,0) (bcond_gcn.F:424-530).
-__global__ void k_vdif_uv(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > g.N) return;
-  const int KB = g.KB, KBM1 = g.KBM1, KBM2 = g.KBM2;
-  const double dti = g.dti, umol = g.umol, d1 = a.d1[i];
-  double A[FV_MAXKB + 1], C[FV_MAXKB + 1], VHU[FV_MAXKB + 1], VHV[FV_MAXKB + 1], VHPU[FV_MAXKB + 1], VHPV[FV_MAXKB + 1];
-  double* UF = a.uf; double* VF = a.vf;
-  if (d1 > 0.0) {
-    for (int k = 1; k <= KB; ++k) C[k] = a.km1[EI(g, i, k)];
-    for (int k = 2; k <= KBM1; ++k) {
-      A[k - 1] = -dti * (C[k] + umol) / (DZ1(i, k - 1) * DZZ1(i, k - 1) * d1 * d1);
-      C[k] = -dti * (C[k] + umol) / (DZ1(i, k) * DZZ1(i, k - 1) * d1 * d1);
-    }
-    const int it1 = g.isonb[NV_(i, 1)], it2 = g.isonb[NV_(i, 2)], it3 = g.isonb[NV_(i, 3)];
-    const double tmp = (it1 == 2 || it2 == 2 || it3 == 2) ? 0.0 : 1.0;
-    const double ramp = a.clk->ramp_int;
-    const double wus = g.wind_on ? -g.wsx[i] * ramp * 0.001 : 0.0;
-    const double wvs = g.wind_on ? -g.wsy[i] * ramp * 0.001 : 0.0;
-    VHU[1] = A[1] / (A[1] - 1.);
-    VHV[1] = A[1] / (A[1] - 1.);
-    VHPU[1] = (-dti * wus * tmp / (-DZ1(i, 1) * d1) - UF[EI(g, i, 1)]) / (A[1] - 1.);
-    VHPV[1] = (-dti * wvs * tmp / (-DZ1(i, 1) * d1) - VF[EI(g, i, 1)]) / (A[1] - 1.);
-    for (int k = 2; k <= KBM2; ++k) {
-      VHPU[k] = 1.0 / (A[k] + C[k] * (1. - VHU[k - 1]) - 1.);
-      VHPV[k] = 1.0 / (A[k] + C[k] * (1. - VHV[k - 1]) - 1.);
-      VHU[k] = A[k] * VHPU[k];
-      VHV[k] = A[k] * VHPV[k];
-      VHPU[k] = (C[k] * VHPU[k - 1] - UF[EI(g, i, k)]) * VHPU[k];
-      VHPV[k] = (C[k] * VHPV[k - 1] - VF[EI(g, i, k)]) * VHPV[k];
-    }
-    const double ub = a.u[EI(g, i, KBM1)], vb = a.v[EI(g, i, KBM1)];
-    const double tps = a.cbc[i] * sqrt(ub * ub + vb * vb);
-    a.tps[i] = tps;
-    UF[EI(g, i, KBM1)] = (C[KBM1] * VHPU[KBM2] - UF[EI(g, i, KBM1)]) /
-                         (tps * dti / (-DZ1(i, KBM1) * d1) - 1. - (VHU[KBM2] - 1.) * C[KBM1]);
-    VF[EI(g, i, KBM1)] = (C[KBM1] * VHPV[KBM2] - VF[EI(g, i, KBM1)]) /
-                         (tps * dti / (-DZ1(i, KBM1) * d1) - 1. - (VHV[KBM2] - 1.) * C[KBM1]);
-    for (int k = 2; k <= KBM1; ++k) {
-      const int ki = KB - k;
-      UF[EI(g, i, ki)] = (VHU[ki] * UF[EI(g, i, ki + 1)] + VHPU[ki]);
-      VF[EI(g, i, ki)] = (VHV[ki] * VF[EI(g, i, ki + 1)] + VHPV[ki]);
-    }
-  }
-  const double sp = g.cc_sponge[i];
-  for (int k = 1; k <= KBM1; ++k) {
-    double uu = UF[EI(g, i, k)], vv = VF[EI(g, i, k)];
-    uu = uu / (1.0 + sp * uu * uu);
-    vv = vv / (1.0 + sp * vv * vv);
-    wall_project(g, i, uu, vv);
-    UF[EI(g, i, k)] = uu;
-    VF[EI(g, i, k)] = vv;
-  }
-  if (d1 > 0.0) {
-    a.wubot[i] = -a.tps[i] * a.u[EI(g, i, KBM1)];
-    a.wvbot[i] = -a.tps[i] * a.v[EI(g, i, KBM1)];
-  }
-}
-
 // WREAL (wreal.F:32-140).
 __global__ void k_wreal(DevGrid g, IntArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

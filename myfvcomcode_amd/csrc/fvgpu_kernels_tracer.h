@@ -122,153 +122,6 @@ __global__ void k_fct_q(DevGrid g, const double* q, double* qf) {
   }
 }
 
-// VDIF_Q, the MY2.5 closure (vdif_q.F:34-551), one node column per thread.
-__global__ void k_vdif_q(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > g.M) return;
-  const int KB = g.KB, KBM1 = g.KBM1;
-  const double A1 = 0.92, B1 = 16.6, A2 = 0.74, B2 = 10.1, C1 = 0.08, E1 = 1.80, E2 = 1.33, SEF = 1.00;
-  const double KAPPA = 0.40, SMALL = 1.E-8, GH_MAX = .0233, GHC = -6.0, SURFL = 2.E+5, GH_MIN = -.281;
-  const double p9 = (double)0.9f, p1 = (double)0.1f;
-  double* Q2F = a.q2f; double* Q2LF = a.q2lf; double* L = a.l; double* KM = a.km; double* KH = a.kh; double* KQ = a.kq;
-  if (i == 0) {
-    L[NI(g, 0, 1)] = 0.0;
-    L[NI(g, 0, KB)] = 0.0;
-    for (int k = 1; k <= KB; ++k) {
-      const double kn0 = L[NI(g, 0, k)] * sqrt(fabs(a.q2[NI(g, 0, k)] + SMALL));
-      KQ[NI(g, 0, k)] = 0.5 * (kn0 * KAPPA * 0.0 + KQ[NI(g, 0, k)]);
-      KM[NI(g, 0, k)] = 0.5 * (kn0 * 0.0 + KM[NI(g, 0, k)]);
-      KH[NI(g, 0, k)] = 0.5 * (kn0 * 0.0 * g.vprnu + KH[NI(g, 0, k)]);
-    }
-    return;
-  }
-  const double dti = g.dti, umol = g.umol, d = a.d[i];
-  double A[FV_MAXKB + 1], Cc[FV_MAXKB + 1], VH[FV_MAXKB + 1], VHP[FV_MAXKB + 1], PROD[FV_MAXKB + 1];
-  double BOY[FV_MAXKB + 1], GH[FV_MAXKB + 1], STF[FV_MAXKB + 1], DTEF[FV_MAXKB + 1];
-  double Q2[FV_MAXKB + 1], LL[FV_MAXKB + 1];
-  for (int k = 2; k <= KBM1; ++k) {
-    const long q = NI(g, i, k);
-    if (Q2F[q] <= SMALL || Q2LF[q] <= SMALL) { Q2F[q] = SMALL; Q2LF[q] = SMALL; }
-  }
-  for (int k = 1; k <= KB; ++k) {
-    A[k] = Cc[k] = VH[k] = VHP[k] = PROD[k] = BOY[k] = GH[k] = DTEF[k] = 0.0;
-    STF[k] = 1.0;
-    Q2[k] = fabs(a.q2[NI(g, i, k)] + SMALL);
-    LL[k] = L[NI(g, i, k)];
-  }
-  for (int k = 2; k <= KBM1; ++k) {
-    A[k] = -0.5 * dti * (KQ[NI(g, i, k + 1)] + KQ[NI(g, i, k)] + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k) * d * d);
-    Cc[k] = -0.5 * dti * (KQ[NI(g, i, k - 1)] + KQ[NI(g, i, k)] + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k - 1) * d * d);
-  }
-  const int o = g.nbve_off[i], nt = g.ntve[i];
-  const double ramp = a.clk->ramp_int;
-  double wus = 0.0, wvs = 0.0, wub = 0.0, wvb = 0.0;
-  for (int j = 0; j < nt; ++j) {
-    const int e = g.nbve[o + j];
-    wus = wus + (g.wind_on ? -g.wsx[e] * ramp * 0.001 : 0.0);
-    wvs = wvs + (g.wind_on ? -g.wsy[e] * ramp * 0.001 : 0.0);
-  }
-  const double ntd = (double)nt;
-  wus = wus / ntd; wvs = wvs / ntd;
-  const double utau2 = sqrt(sq(wus) + sq(wvs));
-  VHP[1] = g.cvhp1 * utau2;
-  VH[1] = 0.0;
-  const double l0 = SURFL * utau2 / g.grav_n[i];
-  for (int j = 0; j < nt; ++j) {
-    const int e = g.nbve[o + j];
-    wub = wub + a.wubot[e];
-    wvb = wvb + a.wvbot[e];
-  }
-  wub = wub / ntd; wvb = wvb / ntd;
-  Q2F[NI(g, i, KB)] = sqrt(sq(wub) + sq(wvb)) * g.const1;
-  for (int k = 2; k <= KBM1; ++k) {
-    BOY[k] = g.grav_n[i] * (a.rho1[NI(g, i, k - 1)] - a.rho1[NI(g, i, k)]) / (DZZ(i, k - 1) * d);
-    PROD[k] = KM[NI(g, i, k)] * 0.0 * fv_pow15(.5 * (-BOY[k] + fabs(BOY[k])));
-  }
-  if (d > 0.0) {
-    double ukm = 0.0, vkm = 0.0;
-    for (int j = 0; j < nt; ++j) { const int e = g.nbve[o + j]; ukm = ukm + a.u[e]; vkm = vkm + a.v[e]; }
-    ukm = ukm / ntd; vkm = vkm / ntd;
-    for (int k = 2; k <= KBM1; ++k) {
-      const long eo = (long)(k - 1) * g.n1;
-      double uk = 0.0, vk = 0.0, ukm1 = 0.0, vkm1 = 0.0;
-      for (int j = 0; j < nt; ++j) {
-        const int e = g.nbve[o + j];
-        uk = uk + a.u[eo + e]; vk = vk + a.v[eo + e];
-        ukm1 = ukm1 + a.u[eo - g.n1 + e]; vkm1 = vkm1 + a.v[eo - g.n1 + e];
-      }
-      uk = uk / ntd; vk = vk / ntd; ukm1 = ukm1 / ntd; vkm1 = vkm1 / ntd;
-      PROD[k] = PROD[k] + KM[NI(g, i, k)] * SEF * (sq(uk - ukm1) + sq(vk - vkm1)) / sq(DZZ(i, k - 1) * d);
-      PROD[k] = PROD[k] + KH[NI(g, i, k)] * BOY[k];
-    }
-    (void)ukm; (void)vkm;
-  }
-  for (int k = 2; k <= KBM1; ++k) {
-    double q2l = fabs(a.q2l[NI(g, i, k)]);
-    LL[k] = q2l / Q2[k];
-    if (ZN(i, k) > -0.5) LL[k] = fmax(LL[k], KAPPA * l0);
-    if (BOY[k] < 0.0) {
-      const double lmax = sqrt(fabs(GH_MIN) * Q2[k] / (-BOY[k] + SMALL));
-      LL[k] = fmin(LL[k], lmax);
-    }
-  }
-  for (int k = 1; k <= KBM1; ++k) GH[k] = (sq(LL[k]) / Q2[k]) * BOY[k];
-  for (int k = 1; k <= KB; ++k) GH[k] = fmin(GH[k], GH_MAX);
-  LL[1] = KAPPA * l0;
-  LL[KB] = 0.0;
-  GH[1] = 0.0;
-  GH[KB] = 0.0;
-  for (int k = 1; k <= KB; ++k) {
-    STF[k] = 1.0;
-    if (GH[k] < 0.0) STF[k] = 1.0 - p9 * fv_pow15(GH[k] / GHC);
-    if (GH[k] < GHC) STF[k] = p1;
-  }
-  for (int k = 1; k <= KB; ++k) DTEF[k] = sqrt(Q2[k]) / (B1 * LL[k] + SMALL) * STF[k];
-  for (int k = 2; k <= KBM1; ++k) {
-    VHP[k] = 1. / (A[k] + Cc[k] * (1. - VH[k - 1]) - (2. * dti * DTEF[k] + 1.));
-    VH[k] = A[k] * VHP[k];
-    VHP[k] = (-2. * dti * PROD[k] + Cc[k] * VHP[k - 1] - Q2F[NI(g, i, k)]) * VHP[k];
-  }
-  for (int k = 1; k <= KBM1; ++k) {
-    const int ki = KB - k;
-    Q2F[NI(g, i, ki)] = VH[ki] * Q2F[NI(g, i, ki + 1)] + VHP[ki];
-  }
-  VH[1] = 0.0;
-  VHP[1] = 0.0;
-  const double z1 = ZN(i, 1), zkb = ZN(i, KB);
-  for (int k = 2; k <= KBM1; ++k) {
-    if (d > 0.0) {
-      const double zk = ZN(i, k);
-      DTEF[k] = DTEF[k] * (1. + E2 * sq((1. / fabs(zk - z1) + 1. / fabs(zk - zkb)) * LL[k] / (d * KAPPA)));
-      VHP[k] = 1. / (A[k] + Cc[k] * (1. - VH[k - 1]) - (dti * DTEF[k] + 1.));
-      VH[k] = A[k] * VHP[k];
-      VHP[k] = (dti * (-PROD[k] * LL[k] * E1) + Cc[k] * VHP[k - 1] - Q2LF[NI(g, i, k)]) * VHP[k];
-    }
-  }
-  for (int k = 1; k <= KBM1; ++k) {
-    const int ki = KB - k;
-    Q2LF[NI(g, i, ki)] = VH[ki] * Q2LF[NI(g, i, ki + 1)] + VHP[ki];
-  }
-  for (int k = 2; k <= KBM1; ++k) {
-    const long q = NI(g, i, k);
-    if (Q2F[q] <= SMALL || Q2LF[q] <= SMALL) { Q2F[q] = SMALL; Q2LF[q] = SMALL; }
-  }
-  const double coef4 = 18.0 * A1 * A1 + 9.0 * A1 * A2, coef5 = 9.0 * A1 * A2;
-  for (int k = 1; k <= KB; ++k) {
-    const double coef1 = A2 * (1.0 - 6.0 * A1 / B1 * STF[k]);
-    const double coef2 = 3.0 * A2 * B2 / STF[k] + 18.0 * A1 * A2;
-    const double coef3 = A1 * (1.0 - 3.0 * C1 - 6.0 * A1 / B1 * STF[k]);
-    const double sh = coef1 / (1.0 - coef2 * GH[k]);
-    const double sm = (coef3 + sh * coef4 * GH[k]) / (1. - coef5 * GH[k]);
-    const double kn = LL[k] * sqrt(Q2[k]);
-    const long q = NI(g, i, k);
-    KQ[q] = 0.5 * (kn * KAPPA * sm + KQ[q]);
-    KM[q] = 0.5 * (kn * sm + KM[q]);
-    KH[q] = 0.5 * (kn * sh * g.vprnu + KH[q]);
-    L[q] = LL[k];
-  }
-}
-
 // ADV_T / ADV_S (adv_t.F:34-1077, adv_s.F:34-1029) for one node column.
 // which = 1 temperature, 2 salinity; both set PFPXB/PFPYB/AH_BOTTOM
 // (adv_t.F:377-388, adv_s.F:71-84).
@@ -364,50 +217,6 @@ __global__ void k_fct_ts(DevGrid g, const double* f1, double* ff) {
     if (tmin - x > 0.0) x = tmin;
     if (x - tmax > 0.0) x = tmax;
     ff[ko + i] = x;
-  }
-}
-
-// VDIF_TS(NCON,F) (vdif_ts.F:42-385), FP64 internals, no surface flux.
-__global__ void k_vdif_ts(DevGrid g, IntArgs a, double* f, int ncon) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > g.M) return;
-  const int KB = g.KB, KBM1 = g.KBM1, KBM2 = g.KBM2;
-  const double dti = g.dti, umolpr = g.umol * 1.E0, d = a.d[i];
-  double AF[FV_MAXKB + 1], CF[FV_MAXKB + 1], VHF[FV_MAXKB + 1], VHPF[FV_MAXKB + 1];
-  double khbottom = 0.0;
-  for (int k = 2; k <= KBM1; ++k) {
-    const double fkh = a.kh[NI(g, i, k)];
-    if (k == KBM1) khbottom = fkh;
-    AF[k - 1] = -dti * (fkh + umolpr) / (DZ(i, k - 1) * DZZ(i, k - 1) * d * d);
-    CF[k] = -dti * (fkh + umolpr) / (DZ(i, k) * DZZ(i, k - 1) * d * d);
-  }
-  VHF[1] = AF[1] / (AF[1] - 1.);
-  VHPF[1] = -dti * (0.0 + 0.0 - 0.0 + 0.0 - 0.0) / (-DZ(i, 1) * d) - f[NI(g, i, 1)];
-  VHPF[1] = VHPF[1] / (AF[1] - 1.);
-  for (int k = 2; k <= KBM2; ++k) {
-    VHPF[k] = 1. / (AF[k] + CF[k] * (1. - VHF[k - 1]) - 1.);
-    VHF[k] = AF[k] * VHPF[k];
-    VHPF[k] = (CF[k] * VHPF[k - 1] - f[NI(g, i, k)] + dti * (0.0 - 0.0) / (d * DZ(i, k))) * VHPF[k];
-  }
-  if (g.isonb[i] != 2) {
-    const double tmp1 = a.pfpxb[i] * g.cos_sita[i] + a.pfpyb[i] * g.sin_sita[i];
-    const double tmp2 = a.ah_bottom[i] * g.phpn[i];
-    const double tmp3 = khbottom + umolpr + a.ah_bottom[i] * g.phpn[i] * g.phpn[i];
-    double tmp = tmp1 * tmp2 / tmp3 * (khbottom + umolpr);
-    if (g.noflux_bot) {
-      if (ncon == 1) { if (tmp1 > 0.0) tmp = 0.0; }
-      else { if (tmp1 < 0.0) tmp = 0.0; tmp = -tmp; }
-    } else {
-      tmp = 0.0;
-    }
-    double fk = (CF[KBM1] * VHPF[KBM2] - f[NI(g, i, KBM1)] - 0.0 + dti * (0.0 - 0.0 - tmp) / (d * DZ(i, KBM1))) /
-                (CF[KBM1] * (1. - VHF[KBM2]) - 1.);
-    f[NI(g, i, KBM1)] = fk;
-    for (int k = 2; k <= KBM1; ++k) {
-      const int ki = KB - k;
-      fk = (VHF[ki] * fk + VHPF[ki]);
-      f[NI(g, i, ki)] = fk;
-    }
   }
 }
 
