@@ -41,8 +41,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time graph replays instead of eager launches")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the path would shard by domain "
-              "decomposition (see DESIGN.md) but no multi-GPU run is provided", file=sys.stderr)
+        print("bench.py: runs on one GPU only; the time step would shard by domain "
+              "decomposition with halo exchange, which is not provided here", file=sys.stderr)
         sys.exit(2)
 
     import myfvcomcode_amd as F
@@ -130,7 +130,7 @@ def cpu_baseline(steps, kb):
     sdph_1m = rate * dti / (24.0 * 1000000 * 40)
     return {"value": sdph_1m, "unit": "sim-days/wall-hour (scaled to 1M x 40)", "cores": 1, "kind": "port",
             "elem_layer_steps_per_s": rate,
-            "sample": "oracle (serial FP64 C restatement, -O3) on the config-1 generator at 251x126 nodes "
+            "sample": "oracle (serial FP64 C restatement, -O3) on the same jittered-mesh generator at 251x126 nodes "
                       "(%d elements x %d layers), %d internal steps, %.1f s" % (n, kbm1, steps, wall)}
 
 

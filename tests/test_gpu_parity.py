@@ -25,7 +25,7 @@ def _pair(config=0, **kw):
 
 def test_external_step_parity(built):
     m, dev, orc = _pair(0)
-    print("case: config 0, one EXTERNAL_STEP", flush=True)
+    print("case: 81x21 estuary channel, one EXTERNAL_STEP", flush=True)
     for _ in range(3):
         dev.external_step(1)
         orc.external_step(1)
@@ -37,7 +37,7 @@ def test_external_step_parity(built):
 
 def test_internal_step_parity_one(built):
     m, dev, orc = _pair(0)
-    print("case: config 0, one INTERNAL_STEP", flush=True)
+    print("case: 81x21 estuary channel, one INTERNAL_STEP", flush=True)
     dev.internal_step()
     orc.internal_step()
     rep = {}
@@ -50,7 +50,7 @@ def test_internal_step_parity_one(built):
 def test_internal_step_parity_many(built):
     m, dev, orc = _pair(0)
     n = 30
-    print("case: config 0, %d INTERNAL_STEPs (graph)" % n, flush=True)
+    print("case: 81x21 estuary channel, %d INTERNAL_STEPs (graph)" % n, flush=True)
     dev.run(n, graph=True)
     orc.run(n)
     rep = {}
@@ -69,7 +69,7 @@ def test_jittered_wind_case_parity(built, graph):
     M2/S2/K1; eager and graph-replayed steps both match the oracle."""
     m, dev, orc = _pair(1, nx=101, ny=51, kb=11)
     n = 12
-    print("case: config-1 generator 101x51 KB=11, %d steps, graph=%s" % (n, graph), flush=True)
+    print("case: jittered seamount mesh 101x51 KB=11, %d steps, graph=%s" % (n, graph), flush=True)
     dev.run(n, graph=graph)
     orc.run(n)
     rep = {}

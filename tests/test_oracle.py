@@ -22,7 +22,7 @@ from oracle import Oracle, lib as oracle_lib
 
 
 def closed_basin(nx=21, ny=7, kb=6, h=None, **pkw):
-    """The config-0 channel mesh without open-boundary nodes."""
+    """The 81x21 estuary channel mesh without open-boundary nodes."""
     c = F.Model.build(0, nx=nx, ny=ny, kb=kb, river=0)
     x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
     nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
