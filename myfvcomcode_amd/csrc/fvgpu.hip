@@ -16,7 +16,7 @@
 //    and one internal step can be captured as a replayable hipGraph;
 //  * time-dependent scalars (RAMP, ExtTime) live in a device clock updated
 //    by single-thread kernels, never by host writes between launches.
-#include "fvgpu_kernels_column.h"
+#include "fvgpu_kernels_cv.h"
 #include "../../include/fvgpu.h"
 
 #include <cmath>
@@ -196,7 +196,8 @@ void enqueue_internal_step(fvgpu_handle* h) {
   if (g.IOBCN > 0) LAUNCH("zero", k_zero, nb(g.IOBCN), 256, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step(h, iext);
   LAUNCH("adjust2", k_adjust2, nb(g.N + 1), 256, g, a);
-  LAUNCH("vertvl", k_vertvl, nb(g.M + 1), 256, g, a);
+#define VERTVL_CALL(KM) LAUNCH("vertvl", k_vertvl_r<KM>, nb(g.M + 1), 256, g, a)
+  FV_KMAX_DISPATCH(g.KBM1, VERTVL_CALL);
   LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
   LAUNCH("viscofh", k_viscofh, nb((long)g.M * g.KBM1), 256, g, a);
   LAUNCH("advuv", k_advuv, nb(g.N + 1), 256, g, a);
@@ -206,10 +207,14 @@ void enqueue_internal_step(fvgpu_handle* h) {
   LAUNCH("wreal", k_wreal, nb(g.N + 1), 256, g, a);
   const double qmean1 = (double)1.E-8f;
   if (h->md->p.vertical_closure) {
-    LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * (g.KBM1 - 1)), 256, g, a, a.q2, nullptr, qmean1, 2, g.KBM1);
-    LAUNCH("adv_q", k_adv_q, nb(g.M + 1), 256, g, a, a.q2, a.q2f);
-    LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * (g.KBM1 - 1)), 256, g, a, a.q2l, nullptr, qmean1, 2, g.KBM1);
-    LAUNCH("adv_q", k_adv_q, nb(g.M + 1), 256, g, a, a.q2l, a.q2lf);
+    for (int which = 0; which < 2; ++which) {
+      double* q = which ? a.q2l : a.q2;
+      double* qf = which ? a.q2lf : a.q2f;
+      LAUNCH("cvrec", k_cvrec<1>, nb(g.M + 1), 256, g, a, q, nullptr, qmean1, 2, g.KBM1);
+      LAUNCH("cvflux_q", k_cvflux_q, nb(g.NCV + 1), 256, g, a);
+#define ADV_Q_CALL(SM) LAUNCH("adv_q", k_adv_q_gather<SM>, nb(g.M + 1), 256, g, a, q, qf)
+      FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
+    }
     if (h->md->p.scalar_positivity) {
       LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2, a.q2f);
       LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2l, a.q2lf);
@@ -227,8 +232,11 @@ void enqueue_internal_step(fvgpu_handle* h) {
     double* ff = which == 1 ? a.tf1 : a.sf1;
     const double* fm = which == 1 ? a.tmean1 : a.smean1;
     const double* fdis = which == 1 ? g.tdis : g.sdis;
-    LAUNCH("ring_prep", k_ring_prep, nb((long)g.M * g.KBM1), 256, g, a, f1, fm, 0.0, 1, g.KBM1);
-    LAUNCH("adv_ts", k_adv_ts, nb(g.M + 1), 256, g, a, f1, ff, fdis, which);
+    LAUNCH("cvrec", k_cvrec<0>, nb(g.M + 1), 256, g, a, f1, fm, 0.0, 1, g.KBM1);
+    LAUNCH("cvflux_ts", k_cvflux_ts, nb(g.NCV + 1), 256, g, a);
+#define ADV_TS_CALL(SM) LAUNCH("adv_ts", k_adv_ts_gather<SM>, nb(g.M + 1), 256, g, a, f1, ff, fdis, which)
+    FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
+    if (g.IOBCN > 0) LAUNCH("adv_ts_obc", k_adv_ts_obc, nb((long)g.IOBCN * g.KBM1), 256, g, a);
     if (h->md->p.scalar_positivity) LAUNCH("fct_ts", k_fct_ts, nb(g.M + 1), 256, g, f1, ff);
 #define VDIF_TS_CALL(KM) LAUNCH("vdif_ts", k_vdif_ts_r<KM>, nb(g.M + 1), 256, g, a, ff, which)
     FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
@@ -261,8 +269,33 @@ int build_lists(fvgpu_handle* h) {
     ent[off[ia] + fill[ia]++] = (c << 1) | 0;
     ent[off[ib] + fill[ib]++] = (c << 1) | 1;
   }
+  for (int i = 1; i <= M; ++i)
+    if (md->ntsn[i] > FV_MAX_RING || md->ntsn[i] < 1) return fail("node ring NTSN outside [1, FV_MAX_RING]");
   if (dupload(h, &g.cv_off, off.data(), off.size())) return 1;
   if (dupload(h, &g.cv_ent, ent.data(), ent.size())) return 1;
+  {
+    int smax = 0;
+    for (int i = 0; i <= M; ++i) smax = std::max(smax, cnt[i]);
+    if (smax > FV_MAX_CV) return fail("a node has more control-volume sub-edges than FV_MAX_CV");
+    g.cv_max = smax;
+    const size_t m1 = (size_t)M + 1, ns = (size_t)std::max(smax, 1) * m1;
+    std::vector<int> le(ns, 0), lel(ns, 0);
+    std::vector<double> sdx(ns, 0.0), sdy(ns, 0.0);
+    for (int i = 0; i <= M; ++i)
+      for (int j = 0; j < cnt[i]; ++j) {
+        const int e = ent[off[i] + j], c = e >> 1;
+        const size_t q = (size_t)j * m1 + i;
+        le[q] = e;
+        lel[q] = md->ntrg[c];
+        sdx[q] = (e & 1) ? md->dltxe[c] : -md->dltxe[c];
+        sdy[q] = (e & 1) ? md->dltye[c] : -md->dltye[c];
+      }
+    if (dupload(h, &g.cv_cnt, cnt.data(), m1)) return 1;
+    if (dupload(h, &g.cvl_ent, le.data(), ns)) return 1;
+    if (dupload(h, &g.cvl_elem, lel.data(), ns)) return 1;
+    if (dupload(h, &g.cvl_sdx, sdx.data(), ns)) return 1;
+    if (dupload(h, &g.cvl_sdy, sdy.data(), ns)) return 1;
+  }
   // element -> its three edges sorted by edge index
   std::vector<int> ed(3 * (size_t)(N + 1), 0), efill(N + 1, 0);
   for (int e = 1; e <= NE; ++e) {
@@ -437,8 +470,9 @@ int upload_state(fvgpu_handle* h) {
   RW(q2, md->q2, mk); RW(q2l, md->q2l, mk); RW(q2f, nullptr, mk); RW(q2lf, nullptr, mk);
   RW(l, md->l, mk); RW(km, md->km, mk); RW(kh, md->kh, mk); RW(kq, md->kq, mk); RW(wts, md->wts, mk);
   RW(viscofh, nullptr, mk);
-  RW(gx, nullptr, mk); RW(gy, nullptr, mk); RW(gxd, nullptr, mk); RW(gyd, nullptr, mk);
-  RW(fmin_, nullptr, mk); RW(fmax_, nullptr, mk);
+  RW(gx, nullptr, mk);
+  if (dalloc(h, &a.rec, mk)) return 1;
+  RW(cvflux, nullptr, ((size_t)g.NCV + 1) * g.KB);
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
   RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
   RW(uard, nullptr, n1); RW(vard, nullptr, n1); RW(fluxobn, nullptr, n1);

@@ -7,7 +7,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #define FV_MAXKB 64
+#define FV_MAX_CV 32   // control-volume sub-edges per node (twice the node degree)
 
 struct DevGrid {
   int M, N, NE, NCV, KB, KBM1, KBM2, IOBCN, NUMQBC, NTIDE;
@@ -33,6 +36,12 @@ struct DevGrid {
   const double *z, *zz, *dz, *dzz, *z1, *zz1, *dz1, *dzz1;
   // gather lists
   const int *cv_off, *cv_ent;      // node -> (cv<<1 | side), sorted by cv
+  // the same lists slot-major (entry j of node i at j*m1+i, j < cv_cnt[i]):
+  // consecutive nodes read consecutive addresses for each slot.  cvl_sdx/sdy
+  // are DLTXE/DLTYE negated on side 0 (XFLUX(IA) -= f  ==  XFLUX(IA) += (-f)).
+  int cv_max;
+  const int *cv_cnt, *cvl_ent, *cvl_elem;
+  const double *cvl_sdx, *cvl_sdy;
   const int *ed_ent;               // element -> 3 x (edge<<1 | side), sorted by edge
   const int *fo_off, *fo_obc;      // element -> FLUXOBN contributions (obc<<1 | slot)
   const int *node_obc;             // node -> OBC index or -1
@@ -45,6 +54,18 @@ struct DevGrid {
   const int *inodeq;
   const double *qarea, *cos_angleq, *sin_angleq, *qdis0, *tdis, *sdis, *vqdist;
 };
+
+// Workgroup index in XCD-contiguous order.  The dispatcher deals consecutive
+// workgroups round-robin to the 8 XCDs, each with its own L2; this remap gives
+// XCD x one contiguous range of indices, so workgroups over neighbouring mesh
+// entities (which gather the same node/element data) share an L2.  It is a
+// bijection on [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int per = nb >> 3, rem = nb & 7, x = b & 7;
+  return x * per + (x < rem ? x : rem) + (b >> 3);
+}
+__device__ __forceinline__ long xcd_thread() { return (long)xcd_block() * blockDim.x + threadIdx.x; }
 
 // 1-based layer accessors
 __device__ __forceinline__ long NI(const DevGrid& g, int i, int k) { return (long)(k - 1) * g.m1 + i; }
@@ -91,3 +112,35 @@ struct DevClock {
   int depth_fail;
   int pad;
 };
+
+// Compile-time loop: f(std::integral_constant<int, K>) for K = 0..N-1, fully
+// expanded so that register arrays are indexed by constants.
+template <typename F, int... Ks>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Ks...>) {
+  (f(std::integral_constant<int, Ks>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Host-side choice of the register-array bound KMAX >= KBM1 for the
+// column kernels templated on it.
+#define FV_KMAX_DISPATCH(KBM1, CALL)                 \
+  do {                                               \
+    if ((KBM1) <= 8) { CALL(8); }                    \
+    else if ((KBM1) <= 16) { CALL(16); }             \
+    else if ((KBM1) <= 24) { CALL(24); }             \
+    else if ((KBM1) <= 32) { CALL(32); }             \
+    else if ((KBM1) <= 40) { CALL(40); }             \
+    else if ((KBM1) <= 48) { CALL(48); }             \
+    else { CALL(64); }                               \
+  } while (0)
+
+// Host-side choice of the per-node sub-edge bound SMAX >= cv_max.
+#define FV_SMAX_DISPATCH(SMAX_, CALL)                \
+  do {                                               \
+    if ((SMAX_) <= 16) { CALL(16); }                 \
+    else if ((SMAX_) <= 24) { CALL(24); }            \
+    else { CALL(32); }                               \
+  } while (0)

@@ -76,15 +76,14 @@ __device__ __forceinline__ double tide_force(const DevGrid& g, int obc, double r
 }
 
 __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)xcd_thread();
   if (i > g.M) return;
   double xflux = 0.0;
-  for (int p = g.cv_off[i]; p < g.cv_off[i + 1]; ++p) {
-    const int ent = g.cv_ent[p], c = ent >> 1;
-    const int i1 = g.ntrg[c];
-    const double ex = a.cur.d1[i1] * (-a.cur.ua[i1] * g.dltye[c] + a.cur.va[i1] * g.dltxe[c]);
-    if (ent & 1) xflux = xflux + ex;
-    else xflux = xflux - ex;
+  const int n = g.cv_cnt[i];
+  for (int j = 0; j < n; ++j) {
+    const long q = (long)j * g.m1 + i;
+    const int i1 = g.cvl_elem[q];
+    xflux = xflux + a.cur.d1[i1] * (-a.cur.ua[i1] * g.cvl_sdy[q] + a.cur.va[i1] * g.cvl_sdx[q]);
   }
   const int obc = g.node_obc[i];
   double xobc = 0.0;

@@ -7,6 +7,14 @@
 #pragma once
 #include "fvgpu_kernels_2d.h"
 
+// Per node-layer inputs of a control-volume face flux, packed so that one
+// gather of a neighbour node reads one 64-byte record: the field, its ring
+// gradient (adv_t.F:219-376), ring bounds (adv_t.F:469-480), the gradient of
+// the anomaly, and the node's horizontal viscosity times NN_HVC.
+struct __align__(16) CvRec {
+  double f, gx, gy, fmin, fmax, gxd, gyd, vhn;
+};
+
 struct IntArgs {
   // elements, 3-D
   double *u, *v, *w, *ww, *uf, *vf, *drhox, *drhoy, *rho, *rmean, *t, *s, *km1, *viscofm;
@@ -14,7 +22,9 @@ struct IntArgs {
   double *t1, *s1, *tf1, *sf1, *rho1, *rmean1, *tmean1, *smean1, *q2, *q2l, *q2f, *q2lf;
   double *l, *km, *kh, *kq, *wts, *viscofh;
   // node-layer scratch: ring gradients and bounds
-  double *gx, *gy, *gxd, *gyd, *fmin_, *fmax_;
+  double* gx;            // node-layer scratch (PROD of VDIF_Q)
+  CvRec* rec;            // node-layer records of the field being advected
+  double* cvflux;        // per sub-edge, per layer face flux (NCV+1) x KB
   // 2-D
   double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;
   double *el, *d, *et, *dt, *egf, *dtfa, *elf;          // nodes (elf = EL after the ext. loop)
@@ -265,54 +275,6 @@ __global__ void k_adjust2(DevGrid g, IntArgs a) {
     a.u[EI(g, i, k)] = a.u[EI(g, i, k)] - (utmp - uard) / dt1;
     a.v[EI(g, i, k)] = a.v[EI(g, i, k)] - (vtmp - vard) / dt1;
   }
-}
-
-// VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer, column
-// integration of omega, linear mass-error correction, DTFA.
-__global__ void k_vertvl(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > g.M) return;
-  if (i == 0) {
-    for (int k = 1; k <= g.KB; ++k) a.wts[NI(g, 0, k)] = 0.0;
-    a.dtfa[0] = a.d[0];
-    return;
-  }
-  const int c0 = g.cv_off[i], c1 = g.cv_off[i + 1];
-  const bool obc = g.isonb[i] == 2;
-  const int rv = g.node_riv[i];
-  const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
-  double w = 0.0;
-  a.wts[NI(g, i, 1)] = 0.0;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    double xflux = 0.0;
-    const long ko = (long)(k - 1) * g.n1;
-    for (int p = c0; p < c1; ++p) {
-      const int ent = g.cv_ent[p], c = ent >> 1;
-      const int i1 = g.ntrg[c];
-      const double dij = a.dt1[i1] * DZ1(i1, k);
-      const double ex = dij * (-a.u[ko + i1] * g.dltye[c] + a.v[ko + i1] * g.dltxe[c]);
-      if (ent & 1) xflux = xflux + ex;
-      else xflux = xflux - ex;
-    }
-    if (obc) xflux = 0.0;
-    if (rv >= 0)
-      for (int j = rv; j < g.NUMQBC; ++j)
-        if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
-    w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
-    a.wts[NI(g, i, k + 1)] = w;
-  }
-  const double wbottom = 0.0;
-  double dtfa = a.d[i];
-  if (fabs(w - wbottom) > 1.0E-8 && !obc) {
-    double tmp1 = a.elf[i] * (double)g.KBM1 - (w - wbottom) * g.dti / DZ(i, 1);
-    tmp1 = tmp1 / (double)g.KBM1;
-    dtfa = tmp1 + g.h[i];
-    for (int k = 2; k <= g.KB; ++k) {
-      const double ratio = (double)((float)(k - 1) / (float)g.KBM1);
-      a.wts[NI(g, i, k)] = a.wts[NI(g, i, k)] - ratio * (w - wbottom);
-    }
-  }
-  a.dtfa[i] = dtfa;
 }
 
 // N2E3D (mod_main.F:1633-1645) over layers 1..kmax.

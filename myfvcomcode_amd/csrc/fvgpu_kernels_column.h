@@ -12,19 +12,6 @@
 #pragma once
 #include "fvgpu_kernels_tracer.h"
 
-#include <utility>
-
-// Compile-time loop: f(std::integral_constant<int, K>) for K = 0..N-1, fully
-// expanded so that register arrays are indexed by constants.
-template <typename F, int... Ks>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Ks...>) {
-  (f(std::integral_constant<int, Ks>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 // VDIF_UV (vdif_uv.F:34-189): one thread per (element column, component);
 // blockIdx.y = 0 solves U, 1 solves V.  The coefficient VHU = VHV is
 // evaluated by both with the same operations.  The sponge and the wall
@@ -322,15 +309,3 @@ __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* 
     L[NI(g, i, k)] = lv.ll;
   }
 }
-
-// Dispatch on the smallest register bound KMAX >= KBM1.
-#define FV_KMAX_DISPATCH(KBM1, CALL)                 \
-  do {                                               \
-    if ((KBM1) <= 8) { CALL(8); }                    \
-    else if ((KBM1) <= 16) { CALL(16); }             \
-    else if ((KBM1) <= 24) { CALL(24); }             \
-    else if ((KBM1) <= 32) { CALL(32); }             \
-    else if ((KBM1) <= 40) { CALL(40); }             \
-    else if ((KBM1) <= 48) { CALL(48); }             \
-    else { CALL(64); }                               \
-  } while (0)

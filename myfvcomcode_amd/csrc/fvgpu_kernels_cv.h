@@ -1,0 +1,364 @@
+// fvgpu_kernels_cv.h -- control-volume flux kernels of the internal mode in
+// two phases, and VERTVL with its sub-edge sums in registers.
+//
+// The reference's scalar advection (ADV_T/ADV_S, ADV_Q) loops over the
+// control-volume sub-edges and scatters each face flux to its two nodes.
+// Here:
+//   1. k_cvrec     one thread per node: ring gradient, ring bounds and the
+//                  viscosity factor of every layer, packed per node-layer
+//                  (CvRec), the node's ring kept in registers across layers;
+//   2. k_cvflux_*  one thread per sub-edge: the face flux of every layer,
+//                  evaluated once, written to cvflux[k][c] (coalesced);
+//   3. k_*_gather  one thread per node: the node's sub-edge fluxes summed in
+//                  increasing sub-edge order (the reference's scatter order,
+//                  so the sum is bit-identical), then the vertical flux and
+//                  the update.
+// Node lists are read slot-major (cvl_ent) so consecutive nodes read
+// consecutive addresses, and workgroups are placed XCD-contiguously.
+#pragma once
+#include "fvgpu_kernels_column.h"
+
+#define FV_MAX_RING 16   // NTSN bound of the register ring in k_cvrec_*
+
+// Ring gradient / bounds of field f at node i for the layers [k0, k1]
+// (adv_t.F:219-376 and :469-480; adv_q.F the same on layer interfaces).
+// kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
+// DZ-weighted interface viscosity (adv_q.F) times NN_HVC.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const double* f, const double* mean,
+                                               double cmean, int k0, int k1) {
+  const int i = (int)xcd_thread();
+  if (i < 1 || i > g.M) return;
+  const int o = g.nbsn_off[i], nt = g.ntsn[i];
+  int rn[FV_MAX_RING];
+  double rdx[FV_MAX_RING], rdy[FV_MAX_RING];
+  static_for<FV_MAX_RING>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    rn[j] = j < nt ? g.nbsn[o + j] : 0;
+    rdx[j] = j < nt - 1 ? g.dltxtrie[o + j] : 0.0;
+    rdy[j] = j < nt - 1 ? g.dltytrie[o + j] : 0.0;
+  });
+  const double art2 = g.art2[i], nn = g.nn_hvc[i];
+  for (int k = k0; k <= k1; ++k) {
+    const long ko = (long)(k - 1) * g.m1;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    double f1 = f[ko + rn[0]];
+    double m1v = mean ? mean[ko + rn[0]] : cmean;
+    double lo = f1, hi = f1;
+    static_for<FV_MAX_RING - 1>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j < nt - 1) {
+        const double f2 = f[ko + rn[j + 1]];
+        const double m2v = mean ? mean[ko + rn[j + 1]] : cmean;
+        const double ffd = 0.5 * (f1 + f2 - m1v - m2v);
+        const double ff1 = 0.5 * (f1 + f2);
+        s1 = s1 + ff1 * rdy[j];
+        s2 = s2 + ff1 * rdx[j];
+        s3 = s3 + ffd * rdy[j];
+        s4 = s4 + ffd * rdx[j];
+        if (j >= 1) { lo = fmin(lo, f1); hi = fmax(hi, f1); }
+        f1 = f2;
+        m1v = m2v;
+      }
+    });
+    const double c = f[ko + i];
+    double vhn;
+    if (KIND == 0) {
+      vhn = a.viscofh[ko + i] * nn;
+    } else {
+      const double vfa = (a.viscofh[ko + i] * DZ(i, k - 1) + a.viscofh[ko - g.m1 + i] * DZ(i, k)) /
+                         (DZ(i, k) + DZ(i, k - 1));
+      vhn = vfa * nn;
+    }
+    CvRec r;
+    r.f = c;
+    r.gx = s1 / art2;
+    r.gy = s2 / art2;
+    r.fmin = fmin(lo, c);
+    r.fmax = fmax(hi, c);
+    r.gxd = s3 / art2;
+    r.gyd = s4 / art2;
+    r.vhn = vhn;
+    a.rec[ko + i] = r;
+  }
+}
+
+// Face flux of ADV_T/ADV_S through sub-edge c at layer k (adv_t.F:440-500).
+struct TsFace { double exflux, fxx, fyy; };
+__device__ __forceinline__ TsFace ts_face(const DevGrid& g, const IntArgs& a, int c, int k) {
+  const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
+  const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
+  const double dltx = g.dltxe[c], dlty = g.dltye[c];
+  const double dtij = a.dt1[i1] * DZ1(i1, k);
+  const double un = a.v[eo + i1] * dltx - a.u[eo + i1] * dlty;
+  const CvRec ra = a.rec[ko + ia], rb = a.rec[ko + ib];
+  double fij1 = ra.f + DLTXNCVE_(c, 1) * ra.gx + DLTYNCVE_(c, 1) * ra.gy;
+  double fij2 = rb.f + DLTXNCVE_(c, 2) * rb.gx + DLTYNCVE_(c, 2) * rb.gy;
+  if (fij1 < ra.fmin) fij1 = ra.fmin;
+  if (fij1 > ra.fmax) fij1 = ra.fmax;
+  if (fij2 < rb.fmin) fij2 = rb.fmin;
+  if (fij2 > rb.fmax) fij2 = rb.fmax;
+  const double viscof = g.fact * 0.5 * (ra.vhn + rb.vhn) + g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]);
+  const double txx = 0.5 * (ra.gxd + rb.gxd) * viscof;
+  const double tyy = 0.5 * (ra.gyd + rb.gyd) * viscof;
+  TsFace r;
+  r.fxx = -dtij * txx * dlty;
+  r.fyy = dtij * tyy * dltx;
+  const double sgn = fsign1(un);
+  r.exflux = -un * dtij * ((1.0 + sgn) * fij2 + (1.0 - sgn) * fij1) * 0.5 + r.fxx + r.fyy;
+  return r;
+}
+
+// Phase 2 of ADV_T/ADV_S, per sub-edge and layer: the same operations as
+// ts_face with the geometry held in registers across layers.
+__global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
+  const int c = (int)xcd_thread();
+  if (c < 1 || c > g.NCV) return;
+  const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
+  const double dltx = g.dltxe[c], dlty = g.dltye[c];
+  const double x1 = DLTXNCVE_(c, 1), y1 = DLTYNCVE_(c, 1), x2 = DLTXNCVE_(c, 2), y2 = DLTYNCVE_(c, 2);
+  const double visc0 = g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]);
+  const double dt1 = a.dt1[i1];
+  const long cs = (long)g.NCV + 1;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
+    const double dtij = dt1 * DZ1(i1, k);
+    const double un = a.v[eo + i1] * dltx - a.u[eo + i1] * dlty;
+    const CvRec ra = a.rec[ko + ia], rb = a.rec[ko + ib];
+    double fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
+    double fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
+    if (fij1 < ra.fmin) fij1 = ra.fmin;
+    if (fij1 > ra.fmax) fij1 = ra.fmax;
+    if (fij2 < rb.fmin) fij2 = rb.fmin;
+    if (fij2 > rb.fmax) fij2 = rb.fmax;
+    const double viscof = g.fact * 0.5 * (ra.vhn + rb.vhn) + visc0;
+    const double txx = 0.5 * (ra.gxd + rb.gxd) * viscof;
+    const double tyy = 0.5 * (ra.gyd + rb.gyd) * viscof;
+    const double fxx = -dtij * txx * dlty;
+    const double fyy = dtij * tyy * dltx;
+    const double sgn = fsign1(un);
+    a.cvflux[(long)(k - 1) * cs + c] = -un * dtij * ((1.0 + sgn) * fij2 + (1.0 - sgn) * fij1) * 0.5 + fxx + fyy;
+  }
+}
+
+// Advective part of the horizontal flux at the open-boundary nodes
+// (XFLUX_OBC of adv_t.F:507-520), one thread per OBC node and layer.
+__global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.IOBCN * g.KBM1) return;
+  const int obc = (int)(t % g.IOBCN), k = (int)(t / g.IOBCN) + 1;
+  const int i = g.i_obc_n[obc];
+  const int n = g.cv_cnt[i];
+  double xadv = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const int e = g.cvl_ent[(long)j * g.m1 + i];
+    const TsFace r = ts_face(g, a, e >> 1, k);
+    if (e & 1) xadv = xadv - (r.exflux - r.fxx - r.fyy);
+    else xadv = xadv + (r.exflux - r.fxx - r.fyy);
+  }
+  a.xflux_obc[(long)(k - 1) * g.IOBCN + obc] = xadv;
+}
+
+// Phase 3 of ADV_T/ADV_S (adv_t.F:507-1052): ordered sub-edge sums, bottom
+// gradients for VDIF_TS, vertical flux, rivers and the update
+// TF1 = (T1 - XFLUX/ART1*DTI/(DT*DZ))*DT/DTFA.
+template <int SMAX>
+__global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, const double* f1, double* ff,
+                                                        const double* fdis, int which) {
+  const int i = (int)xcd_thread();
+  if (i < 1 || i > g.M) return;
+  const int n = g.cv_cnt[i];
+  int ce[SMAX];
+  static_for<SMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    ce[j] = j < n ? g.cvl_ent[(long)j * g.m1 + i] : 0;
+  });
+  const double art1 = g.art1[i];
+  const bool isobc = g.isonb[i] == 2;
+  const long cs = (long)g.NCV + 1;
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.m1;
+    const double* fl = a.cvflux + (long)(k - 1) * cs;
+    double xflux = 0.0;
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j < n) {
+        const double x = fl[ce[j] >> 1];
+        if (ce[j] & 1) xflux = xflux - x;
+        else xflux = xflux + x;
+      }
+    });
+    if (k == g.KBM1) {  // ADV_T and ADV_S both set the bottom-gradient terms VDIF_TS reads
+      const CvRec r = a.rec[ko + i];
+      a.pfpxb[i] = r.gx;
+      a.pfpyb[i] = r.gy;
+      a.ah_bottom[i] = (g.fact * a.viscofh[ko + i] + g.fm1) * g.nn_hvc[i];
+    }
+    const double fk = f1[ko + i], dzk = DZ(i, k);
+    double temp;
+    if (k == 1) {
+      const double dzp = DZ(i, k + 1);
+      temp = -a.wts[ko + g.m1 + i] * (fk * dzp + f1[ko + g.m1 + i] * dzk) / (dzk + dzp);
+    } else if (k == g.KBM1) {
+      const double dzm = DZ(i, k - 1);
+      temp = a.wts[ko + i] * (fk * dzm + f1[ko - g.m1 + i] * dzk) / (dzk + dzm);
+    } else {
+      const double dzp = DZ(i, k + 1), dzm = DZ(i, k - 1);
+      temp = a.wts[ko + i] * (fk * dzm + f1[ko - g.m1 + i] * dzk) / (dzk + dzm) -
+             a.wts[ko + g.m1 + i] * (fk * dzp + f1[ko + g.m1 + i] * dzk) / (dzk + dzp);
+    }
+    if (isobc) xflux = temp * art1;
+    else xflux = xflux + temp * art1;
+    if (!g.river_ts_specified) {
+      const int rv = g.node_riv[i];
+      if (rv >= 0)
+        for (int j = rv; j < g.NUMQBC; ++j)
+          if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j] * fdis[j];
+    }
+    if (which == 1) xflux = xflux - 0.0 * art1;
+    ff[ko + i] = (fk - xflux / art1 * (g.dti / (a.dt[i] * dzk))) * (a.dt[i] / a.dtfa[i]);
+  }
+}
+
+// Phase 2 of ADV_Q (adv_q.F:200-330): face flux on layer interfaces
+// k = 2..KBM1, velocities interpolated to the interface with DZ1 weights.
+__global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
+  const int c = (int)xcd_thread();
+  if (c < 1 || c > g.NCV) return;
+  const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
+  const double dltx = g.dltxe[c], dlty = g.dltye[c];
+  const double x1 = DLTXNCVE_(c, 1), y1 = DLTYNCVE_(c, 1), x2 = DLTXNCVE_(c, 2), y2 = DLTYNCVE_(c, 2);
+  const double visc0 = g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]);
+  const double dt1 = a.dt1[i1];
+  const long cs = (long)g.NCV + 1;
+  double um = a.u[i1], vm = a.v[i1];   // layer 1
+  for (int k = 2; k <= g.KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
+    const double uk = a.u[eo + i1], vk = a.v[eo + i1];
+    const double dz1k = DZ1(i1, k), dz1m = DZ1(i1, k - 1);
+    const double uq = (uk * dz1m + um * dz1k) / (dz1k + dz1m);
+    const double vq = (vk * dz1m + vm * dz1k) / (dz1k + dz1m);
+    um = uk;
+    vm = vk;
+    const double dtij = dt1 * DZZ1(i1, k - 1);
+    const double un = vq * dltx - uq * dlty;
+    const CvRec ra = a.rec[ko + ia], rb = a.rec[ko + ib];
+    double fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
+    double fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
+    if (fij1 < ra.fmin) fij1 = ra.fmin;
+    if (fij1 > ra.fmax) fij1 = ra.fmax;
+    if (fij2 < rb.fmin) fij2 = rb.fmin;
+    if (fij2 > rb.fmax) fij2 = rb.fmax;
+    const double viscof = (g.fact * 0.5 * (ra.vhn + rb.vhn) + visc0) / g.hprnu;
+    const double txx = 0.5 * (ra.gxd + rb.gxd) * viscof;
+    const double tyy = 0.5 * (ra.gyd + rb.gyd) * viscof;
+    const double fxx = -dtij * txx * dlty;
+    const double fyy = dtij * tyy * dltx;
+    const double sgn = fsign1(un);
+    a.cvflux[(long)(k - 1) * cs + c] = -un * dtij * ((1.0 + sgn) * fij2 + (1.0 - sgn) * fij1) * 0.5 + fxx + fyy;
+  }
+}
+
+// Phase 3 of ADV_Q (adv_q.F:340-390): ordered sums, vertical flux, update.
+template <int SMAX>
+__global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, const double* q, double* qf) {
+  const int i = (int)xcd_thread();
+  if (i > g.M) return;
+  if (i == 0) {
+    for (int k = 1; k <= g.KB; ++k) qf[NI(g, 0, k)] = 0.0;
+    return;
+  }
+  const int n = g.cv_cnt[i];
+  int ce[SMAX];
+  static_for<SMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    ce[j] = j < n ? g.cvl_ent[(long)j * g.m1 + i] : 0;
+  });
+  const double art1 = g.art1[i], dti = g.dti, dtn = a.dt[i], dn = a.d[i];
+  const long cs = (long)g.NCV + 1;
+  qf[NI(g, i, 1)] = 0.0;
+  qf[NI(g, i, g.KB)] = 0.0;
+  for (int k = 2; k <= g.KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.m1;
+    const double* fl = a.cvflux + (long)(k - 1) * cs;
+    double xflux = 0.0;
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j < n) {
+        const double x = fl[ce[j] >> 1];
+        if (ce[j] & 1) xflux = xflux - x;
+        else xflux = xflux + x;
+      }
+    });
+    const double temp = a.wts[ko - g.m1 + i] * q[ko - g.m1 + i] - a.wts[ko + g.m1 + i] * q[ko + g.m1 + i];
+    xflux = xflux + temp * art1 * DZZ(i, k - 1) / (DZ(i, k - 1) + DZ(i, k));
+    qf[ko + i] = (q[ko + i] - xflux / art1 * (dti / (dtn * DZZ(i, k - 1)))) * (dtn / dn);
+  }
+}
+
+// VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer with the
+// node's sub-edges outermost and the layer sums in registers, then the
+// column integration of omega, the linear mass-error correction and DTFA.
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
+  const int i = (int)xcd_thread();
+  if (i > g.M) return;
+  if (i == 0) {
+    for (int k = 1; k <= g.KB; ++k) a.wts[NI(g, 0, k)] = 0.0;
+    a.dtfa[0] = a.d[0];
+    return;
+  }
+  const int KBM1 = g.KBM1;
+  double acc[KMAX];
+  static_for<KMAX>([&](auto kc) { acc[decltype(kc)::value] = 0.0; });
+  const int n = g.cv_cnt[i];
+  for (int j = 0; j < n; ++j) {
+    const long q = (long)j * g.m1 + i;
+    const int i1 = g.cvl_elem[q];
+    const double sdx = g.cvl_sdx[q], sdy = g.cvl_sdy[q], dt1 = a.dt1[i1];
+    static_for<KMAX>([&](auto kc) {
+      constexpr int k = decltype(kc)::value + 1;
+      if (k <= KBM1) {
+        const long ko = (long)(k - 1) * g.n1;
+        const double dij = dt1 * DZ1(i1, k);
+        acc[k - 1] = acc[k - 1] + dij * (-a.u[ko + i1] * sdy + a.v[ko + i1] * sdx);
+      }
+    });
+  }
+  const bool obc = g.isonb[i] == 2;
+  const int rv = g.node_riv[i];
+  const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
+  double w = 0.0;
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) {
+      double xflux = obc ? 0.0 : acc[k - 1];
+      if (rv >= 0)
+        for (int j = rv; j < g.NUMQBC; ++j)
+          if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+      w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
+      acc[k - 1] = w;
+    }
+  });
+  const double wbottom = 0.0;
+  double dtfa = a.d[i];
+  const bool corr = fabs(w - wbottom) > 1.0E-8 && !obc;
+  if (corr) {
+    double tmp1 = a.elf[i] * (double)KBM1 - (w - wbottom) * g.dti / DZ(i, 1);
+    tmp1 = tmp1 / (double)KBM1;
+    dtfa = tmp1 + g.h[i];
+  }
+  a.wts[NI(g, i, 1)] = 0.0;
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 2;
+    if (k <= KBM1 + 1) {
+      double wk = acc[k - 2];
+      if (corr) {
+        const double ratio = (double)((float)(k - 1) / (float)KBM1);
+        wk = wk - ratio * (w - wbottom);
+      }
+      a.wts[NI(g, i, k)] = wk;
+    }
+  });
+  a.dtfa[i] = dtfa;
+}
