@@ -16,7 +16,7 @@
 //    and one internal step can be captured as a replayable hipGraph;
 //  * time-dependent scalars (RAMP, ExtTime) live in a device clock updated
 //    by single-thread kernels, never by host writes between launches.
-#include "fvgpu_kernels_cv.h"
+#include "fvgpu_kernels_edge.h"
 #include "../../include/fvgpu.h"
 
 #include <cmath>
@@ -165,6 +165,8 @@ ExtArgs make_ext_args(fvgpu_handle* h, int K, const Ext2DState& cur, const Ext2D
   a.adx2d = h->ia.adx2d; a.ady2d = h->ia.ady2d; a.drx2d = h->ia.drx2d; a.dry2d = h->ia.dry2d;
   a.wubot = h->ia.wubot; a.wvbot = h->ia.wvbot;
   a.qdis = h->ia.qdis;
+  a.ef = h->ia.ef2;
+  a.elij = h->ia.elij2;
   a.clk = h->clk;
   return a;
 }
@@ -178,6 +180,7 @@ void enqueue_external_step(fvgpu_handle* h, int iext) {
   for (int K = 1; K <= 4; ++K) {
     ExtArgs a = make_ext_args(h, K, *cur[K], *nxt[K]);
     LAUNCH("ext_elevation", k_ext_elevation, nb(g.M + 1), 256, g, a);
+    LAUNCH("advave_edge", k_advave_edge, nb(g.NE + 1), 256, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
     LAUNCH("ext_momentum", k_ext_momentum, nb(g.N + 1), 256, g, a);
   }
 }
@@ -188,9 +191,11 @@ void enqueue_internal_step(fvgpu_handle* h) {
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
   LAUNCH("adjust1_brough", k_adjust1_brough, nb(g.N + 1), 256, g, a);
-  LAUNCH("adv3d_sum", k_adv3d_sum, nb(g.N + 1), 256, g, a);
+  LAUNCH("uvedge", k_uvedge<0>, nb(g.NE + 1), 256, g, a, a.u, a.v);
+  LAUNCH("adv3d", k_adv3d_gather, nb(g.N + 1), 256, g, a);
   LAUNCH("baropg", k_baropg, nb(g.N + 1), 256, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) LAUNCH("rho1_roundtrip", k_rho1_roundtrip, nb(g.M + 1), 256, g, a);
+  LAUNCH("advave_edge", k_advave_edge, nb(g.NE + 1), 256, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
   LAUNCH("advave_int", k_advave_int, nb(g.N + 1), 256, g, a);
   LAUNCH("zero", k_zero, nb(g.m1), 256, a.egf, g.m1);
   if (g.IOBCN > 0) LAUNCH("zero", k_zero, nb(g.IOBCN), 256, a.uard_obcn, (long)g.IOBCN);
@@ -200,7 +205,8 @@ void enqueue_internal_step(fvgpu_handle* h) {
   FV_KMAX_DISPATCH(g.KBM1, VERTVL_CALL);
   LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
   LAUNCH("viscofh", k_viscofh, nb((long)g.M * g.KBM1), 256, g, a);
-  LAUNCH("advuv", k_advuv, nb(g.N + 1), 256, g, a);
+  LAUNCH("uvedge", k_uvedge<1>, nb(g.NE + 1), 256, g, a, a.u, a.v);
+  LAUNCH("advuv", k_advuv_gather, nb(g.N + 1), 256, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   LAUNCH("uv_finish", k_uv_finish, nb((long)g.N * g.KBM1), 256, g, a);
@@ -471,6 +477,10 @@ int upload_state(fvgpu_handle* h) {
   RW(l, md->l, mk); RW(km, md->km, mk); RW(kh, md->kh, mk); RW(kq, md->kq, mk); RW(wts, md->wts, mk);
   RW(viscofh, nullptr, mk);
   RW(gx, nullptr, mk);
+  if (dalloc(h, &a.ef2, (size_t)g.NE + 1)) return 1;
+  RW(elij2, nullptr, (size_t)g.NE + 1);
+  if (dalloc(h, &a.ef3, ((size_t)g.NE + 1) * g.KB)) return 1;
+  RW(efvisc, nullptr, ((size_t)g.NE + 1) * g.KB);
   if (dalloc(h, &a.rec, mk)) return 1;
   RW(cvflux, nullptr, ((size_t)g.NCV + 1) * g.KB);
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);

@@ -1,16 +1,18 @@
 // fvgpu_kernels_2d.h -- the barotropic (external) mode on the device.
 //
 // One Runge-Kutta stage of EXTERNAL_STEP (FVCOM_source/external_step.F:159-438)
-// is two kernels:
+// is three kernels:
 //   k_ext_elevation  node-parallel: EXTEL_EDGE (extel_edge.F:32-229) as a
 //                    gather over each node's control-volume sub-edges, the
 //                    OBC tide and RK blend (BCOND_GCN(1,K), bcond_gcn.F:116;
 //                    BCOND_ASL, mod_obcs.F:509-592), the OBC flux of FLUX_OBN
 //                    (mod_obcs.F:1181-1215), D = H + EL, DEPTH_CHECK, and the
 //                    stage-3 EGF / stage-4 UARD_OBCN accumulations.
-//   k_ext_momentum   element-parallel: N2E2D of the new elevation,
-//                    ADVAVE_EDGE_GCN (advave_edge_gcn.F:546-992) gathered over
-//                    the element's three edges, EXTUV_EDGE (extuv_edge.F:33-196),
+//   k_advave_edge    edge-parallel: the ADVAVE_EDGE_GCN edge terms
+//                    (advave_edge_gcn.F:546-884) of the stage input, once per edge;
+//   k_ext_momentum   element-parallel: N2E2D of the new elevation, the
+//                    element's three edge terms gathered in edge order,
+//                    EXTUV_EDGE (extuv_edge.F:33-196),
 //                    the wall projection BCOND_GCN(2,K) (bcond_gcn.F:325-420),
 //                    D1 = H1 + EL1 and the stage-3 UARD/VARD accumulation.
 // The per-node and per-element gather lists are sorted by sub-edge / edge
@@ -24,6 +26,13 @@ struct Ext2DState {      // one buffer set of the 2-D state
   double *ua, *va, *el1, *d1;  // elements
 };
 
+// Per edge (and layer) momentum-flux terms of the edge loops of
+// ADVAVE_EDGE_GCN / ADVECTION_EDGE_GCN / ADV_UV_EDGE_GCN, written once per
+// edge by an edge kernel and gathered by the two adjacent elements.
+struct __align__(16) EdgeFlux4 {
+  double xadv, yadv, fx, fy;
+};
+
 struct ExtArgs {
   Ext2DState cur;        // stage input
   Ext2DState base;       // RK base (ELRK, UARK, VARK, ELRK1)
@@ -33,6 +42,8 @@ struct ExtArgs {
   double *xflux_obcn, *fobn, *uard_obcn, *egf, *fluxobn, *uard, *vard;
   const double *adx2d, *ady2d, *drx2d, *dry2d, *wubot, *wvbot;
   const double *qdis;    // QDIS (internal-mode river discharge)
+  EdgeFlux4* ef;         // per-edge ADVAVE terms of the stage input
+  double* elij;          // per-edge mean elevation of the stage input
   DevClock* clk;
 };
 
@@ -115,14 +126,16 @@ __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   if (i >= 1 && dnew < g.min_depth) a.clk->depth_fail = 1;
 }
 
-// Edge flux of ADVAVE_EDGE_GCN for edge `e`, evaluated exactly as the
-// reference's edge loop body (advave_edge_gcn.F:546-884).
-struct AdvaveFlux { double xterm, yterm, pterm_x, pterm_y; };
-
-__device__ __forceinline__ void advave_edge(const DevGrid& g, const double* ua, const double* va,
-                                            const double* d, const double* el, const double* d1,
-                                            int e, int self, int side, double& xf, double& yf,
-                                            double& px, double& py) {
+// ADVAVE_EDGE_GCN edge loop body (advave_edge_gcn.F:546-884) for edge e of
+// the state (ua, va, d, el), one thread per edge: least-squares velocity
+// gradients on both sides, face velocities, upwind advective flux, the
+// Smagorinsky stresses; the side-dependent factors (EPOR, IUCP, ISBC, the
+// pressure term) are applied by the gathering element.
+__global__ __launch_bounds__(256) void k_advave_edge(DevGrid g, const double* ua, const double* va,
+                                                     const double* d, const double* el,
+                                                     EdgeFlux4* ef, double* elij_out) {
+  const int e = (int)xcd_thread();
+  if (e < 1 || e > g.NE) return;
   const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
   const double dij = 0.5 * (d[j1] + d[j2]);
   const double elij = 0.5 * (el[j1] + el[j2]);
@@ -152,37 +165,43 @@ __device__ __forceinline__ void advave_edge(const DevGrid& g, const double* ua, 
   const double txxij = (cofa1 + cofa3) * viscof;
   const double tyyij = (cofa6 + cofa8) * viscof;
   const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
-  const double fxx = dij * (txxij * dly - txyij * dlx);
-  const double fyy = dij * (txyij * dly - tyyij * dlx);
+  EdgeFlux4 r;
+  r.fx = dij * (txxij * dly - txyij * dlx);
+  r.fy = dij * (txyij * dly - tyyij * dlx);
   const double sgn = fsign1(un_tmp);
-  const double xadv = dij * un_tmp * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
-  const double yadv = dij * un_tmp * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
-  const double isbc_tmp = (double)g.isbc[e];
-  const double ep = g.epor[self], iu = (double)g.iucp[self];
-  const double gd = g.grav_e[self] * d1[self] * elij;
-  if (side == 0) {
-    xf = xf + (xadv + fxx * ep) * (1.0 - isbc_tmp) * iu;
-    yf = yf + (yadv + fyy * ep) * (1.0 - isbc_tmp) * iu;
-    px = px - gd * dly;
-    py = py + gd * dlx;
-  } else {
-    xf = xf - (xadv + fxx * ep) * (1.0 - isbc_tmp) * iu;
-    yf = yf - (yadv + fyy * ep) * (1.0 - isbc_tmp) * iu;
-    px = px + gd * dly;
-    py = py - gd * dlx;
-  }
+  r.xadv = dij * un_tmp * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
+  r.yadv = dij * un_tmp * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
+  ef[e] = r;
+  elij_out[e] = elij;
 }
 
-// ADVAVE_EDGE_GCN for element i: edge gather + OBC flux + rivers.
-__device__ __forceinline__ void advave_element(const DevGrid& g, int i, const double* ua, const double* va,
-                                               const double* d, const double* el, const double* d1,
-                                               const double* fluxobn, const double* qdis,
-                                               double& xf, double& yf, double& px, double& py) {
+// ADVAVE_EDGE_GCN for element i: its three edges' terms gathered in edge
+// order (the reference's scatter order), OBC flux and rivers
+// (advave_edge_gcn.F:746-992).
+__device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const EdgeFlux4* ef, const double* elij,
+                                              const double* ua, const double* va, const double* d1,
+                                              const double* fluxobn, const double* qdis,
+                                              double& xf, double& yf, double& px, double& py) {
   xf = 0.0; yf = 0.0; px = 0.0; py = 0.0;
+  const double ep = g.epor[i], iu = (double)g.iucp[i], gdi = g.grav_e[i] * d1[i];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const int ent = g.ed_ent[3 * i + q];
-    advave_edge(g, ua, va, d, el, d1, ent >> 1, i, ent & 1, xf, yf, px, py);
+    const int ent = g.ed_ent[3 * i + q], e = ent >> 1;
+    const EdgeFlux4 r = ef[e];
+    const double isbc_tmp = (double)g.isbc[e];
+    const double gd = gdi * elij[e];
+    const double dlx = g.dltxc[e], dly = g.dltyc[e];
+    if ((ent & 1) == 0) {
+      xf = xf + (r.xadv + r.fx * ep) * (1.0 - isbc_tmp) * iu;
+      yf = yf + (r.yadv + r.fy * ep) * (1.0 - isbc_tmp) * iu;
+      px = px - gd * dly;
+      py = py + gd * dlx;
+    } else {
+      xf = xf - (r.xadv + r.fx * ep) * (1.0 - isbc_tmp) * iu;
+      yf = yf - (r.yadv + r.fy * ep) * (1.0 - isbc_tmp) * iu;
+      px = px + gd * dly;
+      py = py - gd * dlx;
+    }
   }
   if (g.isbce[i] == 2) {
     xf = (xf + fluxobn[i] * ua[i]) * (double)g.iucp[i];
@@ -197,7 +216,7 @@ __device__ __forceinline__ void advave_element(const DevGrid& g, int i, const do
 }
 
 __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)xcd_thread();
   if (i < 1 || i > g.N) return;
   const double one_third = 1.0 / 3.0;
   const double elf1 = one_third * (a.nxt.el[NV_(i, 1)] + a.nxt.el[NV_(i, 2)] + a.nxt.el[NV_(i, 3)]);
@@ -211,7 +230,7 @@ __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
     a.fluxobn[i] = f;
   }
   double xf, yf, px, py;
-  advave_element(g, i, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.cur.d1, a.fluxobn, a.qdis, xf, yf, px, py);
+  advave_gather(g, i, a.ef, a.elij, a.cur.ua, a.cur.va, a.cur.d1, a.fluxobn, a.qdis, xf, yf, px, py);
   // EXTUV_EDGE
   const double ua = a.cur.ua[i], va = a.cur.va[i], d1 = a.cur.d1[i], art = g.art[i], cor = g.cor[i];
   const double ramp = a.clk->ramp_ext;

@@ -23,6 +23,10 @@ struct IntArgs {
   double *l, *km, *kh, *kq, *wts, *viscofh;
   // node-layer scratch: ring gradients and bounds
   double* gx;            // node-layer scratch (PROD of VDIF_Q)
+  EdgeFlux4* ef2;        // per-edge ADVAVE terms (2-D)
+  double* elij2;         // per-edge mean elevation (2-D)
+  EdgeFlux4* ef3;        // per edge-layer momentum terms (3-D), (NE+1) x KB
+  double* efvisc;        // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB
   CvRec* rec;            // node-layer records of the field being advected
   double* cvflux;        // per sub-edge, per layer face flux (NCV+1) x KB
   // 2-D
@@ -85,82 +89,6 @@ __global__ void k_adjust1_brough(DevGrid g, IntArgs a) {
   const double btps = cbc * sqrt(ub * ub + vb * vb);
   a.wubot[i] = -btps * ub;
   a.wvbot[i] = -btps * vb;
-}
-
-// ADVECTION_EDGE_GCN (advection_edge_gcn.F:397-734) summed over the column
-// into ADX2D/ADY2D (internal_step.F:374-384).
-__device__ __forceinline__ void adv3d_edge(const DevGrid& g, const double* U, const double* V,
-                                           const double* dtn, int e, int self, int side, int k,
-                                           double& xf, double& yf) {
-  const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
-  const int k1 = NBE_(ia, 1), k2 = NBE_(ia, 2), k3 = NBE_(ia, 3), k4 = NBE_(ib, 1), k5 = NBE_(ib, 2), k6 = NBE_(ib, 3);
-  const double xija = g.xijc[e] - g.xc[ia], yija = g.yijc[e] - g.yc[ia];
-  const double xijb = g.xijc[e] - g.xc[ib], yijb = g.yijc[e] - g.yc[ib];
-  const double dij = 0.5 * (dtn[j1] * DZ(j1, k) + dtn[j2] * DZ(j2, k));
-  const long ko = (long)(k - 1) * g.n1;
-  const double uia = U[ko + ia], vIA = V[ko + ia], uib = U[ko + ib], vIB = V[ko + ib];
-  const double cofa1 = A1U_(ia, 1) * uia + A1U_(ia, 2) * U[ko + k1] + A1U_(ia, 3) * U[ko + k2] + A1U_(ia, 4) * U[ko + k3];
-  const double cofa2 = A2U_(ia, 1) * uia + A2U_(ia, 2) * U[ko + k1] + A2U_(ia, 3) * U[ko + k2] + A2U_(ia, 4) * U[ko + k3];
-  const double cofa5 = A1U_(ia, 1) * vIA + A1U_(ia, 2) * V[ko + k1] + A1U_(ia, 3) * V[ko + k2] + A1U_(ia, 4) * V[ko + k3];
-  const double cofa6 = A2U_(ia, 1) * vIA + A2U_(ia, 2) * V[ko + k1] + A2U_(ia, 3) * V[ko + k2] + A2U_(ia, 4) * V[ko + k3];
-  const double uij1 = uia + cofa1 * xija + cofa2 * yija;
-  const double vij1 = vIA + cofa5 * xija + cofa6 * yija;
-  const double cofa3 = A1U_(ib, 1) * uib + A1U_(ib, 2) * U[ko + k4] + A1U_(ib, 3) * U[ko + k5] + A1U_(ib, 4) * U[ko + k6];
-  const double cofa4 = A2U_(ib, 1) * uib + A2U_(ib, 2) * U[ko + k4] + A2U_(ib, 3) * U[ko + k5] + A2U_(ib, 4) * U[ko + k6];
-  const double cofa7 = A1U_(ib, 1) * vIB + A1U_(ib, 2) * V[ko + k4] + A1U_(ib, 3) * V[ko + k5] + A1U_(ib, 4) * V[ko + k6];
-  const double cofa8 = A2U_(ib, 1) * vIB + A2U_(ib, 2) * V[ko + k4] + A2U_(ib, 3) * V[ko + k5] + A2U_(ib, 4) * V[ko + k6];
-  const double uij2 = uib + cofa3 * xijb + cofa4 * yijb;
-  const double vij2 = vIB + cofa7 * xijb + cofa8 * yijb;
-  const double uij = 0.5 * (uij1 + uij2), vij = 0.5 * (vij1 + vij2);
-  const double dlx = g.dltxc[e], dly = g.dltyc[e];
-  const double un_tmp = vij * dlx - uij * dly;
-  const double viscof1 = g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + 0.5 * sq(cofa2 + cofa5));
-  const double viscof2 = g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + 0.5 * sq(cofa4 + cofa7));
-  const double viscof = (g.fact * 0.5 * (viscof1 * g.cc_hvc[ia] + viscof2 * g.cc_hvc[ib]) +
-                         g.fm1 * 0.5 * (g.cc_hvc[ia] + g.cc_hvc[ib])) / g.hprnu;
-  const double txxij = (cofa1 + cofa3) * viscof, tyyij = (cofa6 + cofa8) * viscof;
-  const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
-  const double fxx = dij * (txxij * dly - txyij * dlx);
-  const double fyy = dij * (txyij * dly - tyyij * dlx);
-  const double sgn = fsign1(un_tmp);
-  const double xadv = dij * un_tmp * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
-  const double yadv = dij * un_tmp * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
-  const double isbc_tmp = (double)g.isbc[e];
-  const double tp = (double)(1 - g.isbc[e]) * g.epor[self];
-  const double ep = g.epor[self];
-  if (side == 0) {
-    xf = xf + xadv * tp + (fxx + 3.0 * fxx * isbc_tmp) * ep;
-    yf = yf + yadv * tp + (fyy + 3.0 * fyy * isbc_tmp) * ep;
-  } else {
-    xf = xf - xadv * tp - (fxx + 3.0 * fxx * isbc_tmp) * ep;
-    yf = yf - yadv * tp - (fyy + 3.0 * fyy * isbc_tmp) * ep;
-  }
-}
-
-__global__ void k_adv3d_sum(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > g.N) return;
-  int ent[3];
-  for (int q = 0; q < 3; ++q) ent[q] = g.ed_ent[3 * i + q];
-  double sx = 0.0, sy = 0.0;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    double xf = 0.0, yf = 0.0;
-    for (int q = 0; q < 3; ++q) adv3d_edge(g, a.u, a.v, a.dt, ent[q] >> 1, i, ent[q] & 1, k, xf, yf);
-    if (g.isbce[i] == 2) { xf = 0.0; yf = 0.0; }
-    for (int p = g.re_off[i]; p < g.re_off[i + 1]; ++p) {
-      const int r = g.re_riv[p];
-      const int j = g.inodeq[r];
-      const double vlctyq = a.qdis[r] / g.qarea[r];
-      const double vq = g.vqdist[(long)(k - 1) * g.NUMQBC + r];
-      const double temp = 0.5 * a.qdis[r] * vq * vq * vlctyq / DZ(j, k);
-      xf = xf - temp * g.cos_angleq[r];
-      yf = yf - temp * g.sin_angleq[r];
-    }
-    sx = sx + xf;
-    sy = sy + yf;
-  }
-  a.adx2d[i] = sx;
-  a.ady2d[i] = sy;
 }
 
 // BAROPG (baropg.F:33-166) with the RMEAN round trip of RHO applied to the
@@ -249,7 +177,7 @@ __global__ void k_advave_int(DevGrid g, IntArgs a) {
   a.vard[i] = 0.0;
   if (i < 1) return;
   double xf, yf, px, py;
-  advave_element(g, i, a.ua, a.va, a.d, a.el, a.d1, a.fluxobn, a.qdis, xf, yf, px, py);
+  advave_gather(g, i, a.ef2, a.elij2, a.ua, a.va, a.d1, a.fluxobn, a.qdis, xf, yf, px, py);
   a.adx2d[i] = a.adx2d[i] - xf;
   a.ady2d[i] = a.ady2d[i] - yf;
 }
@@ -329,111 +257,6 @@ __global__ void k_viscofh(DevGrid g, IntArgs a) {
   const double tmp1 = sq(pupx) + sq(pvpy);
   const double tmp2 = 0.5 * sq(pupy + pvpx);
   a.viscofh[NI(g, i, k)] = sqrt(tmp1 + tmp2) * art1;
-}
-
-// ADV_UV_EDGE_GCN (adv_uv_edge_gcn.F:31-1198) for one element column.
-__global__ void k_advuv(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > g.N) return;
-  if (i == 0) {
-    for (int k = 1; k <= g.KB; ++k) a.viscofm[EI(g, 0, k)] = 0.0;
-    return;
-  }
-  int ent[3];
-  for (int q = 0; q < 3; ++q) ent[q] = g.ed_ent[3 * i + q];
-  const double art = g.art[i], dt1 = a.dt1[i], d1 = a.d1[i], cor = g.cor[i];
-  const double* U = a.u; const double* V = a.v; const double* W = a.w;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    const long ko = (long)(k - 1) * g.n1;
-    double xf = 0.0, yf = 0.0, px = 0.0, py = 0.0, vm = 0.0;
-    for (int q = 0; q < 3; ++q) {
-      const int e = ent[q] >> 1, side = ent[q] & 1;
-      const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
-      const double elij = 0.5 * (a.egf[j1] + a.egf[j2]);
-      const int k1 = NBE_(ia, 1), k2 = NBE_(ia, 2), k3 = NBE_(ia, 3), k4 = NBE_(ib, 1), k5 = NBE_(ib, 2), k6 = NBE_(ib, 3);
-      const double dij = 0.5 * (a.dt[j1] * DZ(j1, k) + a.dt[j2] * DZ(j2, k));
-      const double xija = g.xijc[e] - g.xc[ia], yija = g.yijc[e] - g.yc[ia];
-      const double xijb = g.xijc[e] - g.xc[ib], yijb = g.yijc[e] - g.yc[ib];
-      const double uia = U[ko + ia], vIA = V[ko + ia], uib = U[ko + ib], vIB = V[ko + ib];
-      const double cofa1 = A1U_(ia, 1) * uia + A1U_(ia, 2) * U[ko + k1] + A1U_(ia, 3) * U[ko + k2] + A1U_(ia, 4) * U[ko + k3];
-      const double cofa2 = A2U_(ia, 1) * uia + A2U_(ia, 2) * U[ko + k1] + A2U_(ia, 3) * U[ko + k2] + A2U_(ia, 4) * U[ko + k3];
-      const double cofa5 = A1U_(ia, 1) * vIA + A1U_(ia, 2) * V[ko + k1] + A1U_(ia, 3) * V[ko + k2] + A1U_(ia, 4) * V[ko + k3];
-      const double cofa6 = A2U_(ia, 1) * vIA + A2U_(ia, 2) * V[ko + k1] + A2U_(ia, 3) * V[ko + k2] + A2U_(ia, 4) * V[ko + k3];
-      const double uij1 = uia + cofa1 * xija + cofa2 * yija;
-      const double vij1 = vIA + cofa5 * xija + cofa6 * yija;
-      const double cofa3 = A1U_(ib, 1) * uib + A1U_(ib, 2) * U[ko + k4] + A1U_(ib, 3) * U[ko + k5] + A1U_(ib, 4) * U[ko + k6];
-      const double cofa4 = A2U_(ib, 1) * uib + A2U_(ib, 2) * U[ko + k4] + A2U_(ib, 3) * U[ko + k5] + A2U_(ib, 4) * U[ko + k6];
-      const double cofa7 = A1U_(ib, 1) * vIB + A1U_(ib, 2) * V[ko + k4] + A1U_(ib, 3) * V[ko + k5] + A1U_(ib, 4) * V[ko + k6];
-      const double cofa8 = A2U_(ib, 1) * vIB + A2U_(ib, 2) * V[ko + k4] + A2U_(ib, 3) * V[ko + k5] + A2U_(ib, 4) * V[ko + k6];
-      const double uij2 = uib + cofa3 * xijb + cofa4 * yijb;
-      const double vij2 = vIB + cofa7 * xijb + cofa8 * yijb;
-      const double uij = 0.5 * (uij1 + uij2), vij = 0.5 * (vij1 + vij2);
-      const double dlx = g.dltxc[e], dly = g.dltyc[e];
-      const double exflux = dij * (-uij * dly + vij * dlx);
-      const double viscof1 = g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + 0.5 * sq(cofa2 + cofa5));
-      const double viscof2 = g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + 0.5 * sq(cofa4 + cofa7));
-      const double viscof = (g.fact * 0.5 * (viscof1 * g.cc_hvc[ia] + viscof2 * g.cc_hvc[ib]) +
-                             g.fm1 * 0.5 * (g.cc_hvc[ia] + g.cc_hvc[ib])) / g.hprnu;
-      vm = vm + viscof;
-      const double txxij = (cofa1 + cofa3) * viscof, tyyij = (cofa6 + cofa8) * viscof;
-      const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
-      const double fxx = dij * (txxij * dly - txyij * dlx);
-      const double fyy = dij * (txyij * dly - tyyij * dlx);
-      const double sgn = fsign1(exflux);
-      const double xadv = exflux * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
-      const double yadv = exflux * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
-      const double isbc_tmp = (double)g.isbc[e];
-      const double tp = (double)(1 - g.isbc[e]) * g.epor[i];
-      const double ep = g.epor[i];
-      const double gd = g.grav_e[i] * dt1 * DZ1(i, k) * elij;
-      if (side == 0) {
-        xf = xf + xadv * tp + (fxx + 3.0 * fxx * isbc_tmp) * ep;
-        yf = yf + yadv * tp + (fyy + 3.0 * fyy * isbc_tmp) * ep;
-        px = px - gd * dly;
-        py = py + gd * dlx;
-      } else {
-        xf = xf - xadv * tp - (fxx + 3.0 * fxx * isbc_tmp) * ep;
-        yf = yf - yadv * tp - (fyy + 3.0 * fyy * isbc_tmp) * ep;
-        px = px + gd * dly;
-        py = py - gd * dlx;
-      }
-    }
-    xf = xf + px;
-    yf = yf + py;
-    // vertical advection (adv_uv_edge_gcn.F:927-946)
-    const double uk = U[ko + i], vk = V[ko + i], dzk = DZ1(i, k);
-    double xfluxv, yfluxv;
-    if (k == 1) {
-      const long kp = ko + g.n1; const double dzp = DZ1(i, k + 1);
-      xfluxv = -W[kp + i] * (uk * dzp + U[kp + i] * dzk) / (dzk + dzp);
-      yfluxv = -W[kp + i] * (vk * dzp + V[kp + i] * dzk) / (dzk + dzp);
-    } else if (k == g.KBM1) {
-      const long km = ko - g.n1; const double dzm = DZ1(i, k - 1);
-      xfluxv = W[ko + i] * (uk * dzm + U[km + i] * dzk) / (dzk + dzm);
-      yfluxv = W[ko + i] * (vk * dzm + V[km + i] * dzk) / (dzk + dzm);
-    } else {
-      const long kp = ko + g.n1, km = ko - g.n1;
-      const double dzp = DZ1(i, k + 1), dzm = DZ1(i, k - 1);
-      xfluxv = W[ko + i] * (uk * dzm + U[km + i] * dzk) / (dzk + dzm) - W[kp + i] * (uk * dzp + U[kp + i] * dzk) / (dzk + dzp);
-      yfluxv = W[ko + i] * (vk * dzm + V[km + i] * dzk) / (dzk + dzm) - W[kp + i] * (vk * dzp + V[kp + i] * dzk) / (dzk + dzp);
-    }
-    xf = xf + xfluxv * art + a.drhox[ko + i] - cor * vk * dt1 * dzk * art;
-    yf = yf + yfluxv * art + a.drhoy[ko + i] + cor * uk * dt1 * dzk * art;
-    if (g.isbce[i] == 2) { xf = 0.0; yf = 0.0; }
-    for (int p = g.re_off[i]; p < g.re_off[i + 1]; ++p) {
-      const int r = g.re_riv[p];
-      const int j = g.inodeq[r];
-      const double vlctyq = a.qdis[r] / g.qarea[r];
-      const double vq = g.vqdist[(long)(k - 1) * g.NUMQBC + r];
-      const double temp = 0.5 * a.qdis[r] * vq * vq * vlctyq / DZ(j, k);
-      xf = xf - temp * g.cos_angleq[r];
-      yf = yf - temp * g.sin_angleq[r];
-    }
-    a.uf[ko + i] = uk * dt1 / d1 - g.dti * xf / art / (d1 * dzk);
-    a.vf[ko + i] = vk * dt1 / d1 - g.dti * yf / art / (d1 * dzk);
-    a.viscofm[ko + i] = vm / art;
-  }
-  a.viscofm[EI(g, i, g.KB)] = 0.0 / art;
 }
 
 // WREAL (wreal.F:32-140).
