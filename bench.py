@@ -7,10 +7,11 @@ DTE = 2 s, ISPLIT = 10 (DTI = 20 s), synthetic seeded data.
 
 A step is one INTERNAL_STEP (ISPLIT external steps included).  The timed
 region runs K steps with the state resident in HBM; HIP events around every
-launch of the dominant kernel family give its average device time for the
-roofline figure.  The CPU baseline is the serial FP64 oracle on a bounded
-sample of the same workload (the same generator at 251x126 nodes, 62,500
-elements x 40 layers), scaled to the metric's unit.
+launch give each kernel family's average device time for the
+roofline figures (the dominant family in "roofline", every modelled family
+in "roofline_by_kernel").  The CPU baseline is the serial FP64 oracle on a
+bounded sample of the same workload (the same generator at 251x126 nodes,
+62,500 elements x 40 layers), scaled to the metric's unit.
 
 Prints one JSON line.
 """
@@ -30,15 +31,13 @@ PEAK_HBM = 8.0e12
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--kernel", default="ext_momentum", help="kernel family timed with HIP events")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nx", type=int, default=1001)
     ap.add_argument("--ny", type=int, default=501)
     ap.add_argument("--kb", type=int, default=41)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="time graph replays instead of eager launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; the time step would shard by domain "
@@ -55,18 +54,20 @@ def main():
     print("bench: built %d nodes / %d elements x %d layers in %.1f s" % (model.m, n, kbm1, time.time() - t0),
           file=sys.stderr, flush=True)
     dev = F.DeviceModel(model)
-    dev.run(args.warmup, graph=args.graph)
+    dev.run(args.warmup, graph=False)
     dev.synchronize()
     print("bench: warmup done", file=sys.stderr, flush=True)
 
-    dev.set_profiling(True, kernel=args.kernel)
+    # timed region: K internal steps, HIP events around every launch on the
+    # handle's stream give each kernel family's average device time
+    dev.set_profiling(True)
     dev.synchronize()
     t0 = time.perf_counter()
     dev.run(args.steps, graph=False)
     dev.synchronize()
     wall = time.perf_counter() - t0
     dev.set_profiling(False)
-    avg_us, count = dev.kernel_time(args.kernel)
+    stats = dev.kernel_stats()
     diag = dev.diagnostics()
     if diag.nonfinite:
         print("bench: non-finite state", file=sys.stderr)
@@ -74,15 +75,29 @@ def main():
 
     rate = n * kbm1 * args.steps / wall                      # element-layer-steps/s
     sdph = rate * dti / (24.0 * n * kbm1)                     # simulated days per wall-hour
-    bytes_launch = roofline.bytes_per_launch(args.kernel, model)
-    achieved = bytes_launch / (avg_us * 1e-6) / 1e9
+
+    per_kernel = {}
+    tot_b = tot_t = 0.0
+    for name, (avg_us, count) in stats.items():
+        if name not in roofline.FAMILIES or avg_us <= 0:
+            continue
+        b = roofline.bytes_per_launch(name, model)
+        gbps = b / (avg_us * 1e-6) / 1e9
+        per_kernel[name] = {"avg_us": round(avg_us, 2), "launches_per_step": count / args.steps,
+                            "bytes": b, "achieved_GBps": round(gbps, 1), "frac": round(gbps * 1e9 / PEAK_HBM, 4)}
+        tot_b += b * count
+        tot_t += avg_us * 1e-6 * count
+    # dominant kernel = the modelled family with the largest device time per step
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
+    pk = per_kernel[dom]
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "%s_traffic.json" % args.kernel)
+    tfile = os.path.join(ROOT, "profiles", "%s_traffic.json" % dom)
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("bytes_per_launch")
-        except Exception:
+        except (OSError, ValueError):
             traffic = None
+    step_kernel_ms = sum(v[0] * v[1] for v in stats.values()) / 1000.0 / args.steps
 
     cpu = None
     if not args.no_cpu:
@@ -103,12 +118,17 @@ def main():
         "data": "synthetic (seeded jittered mesh, seamounts, M2/S2/K1 tide, Gaussian random wind)",
         "config": {"workload": "FVCOM INTERNAL_STEP, %dx%d-node jittered mesh, %d elements x %d layers, "
                                "DTE=2 s ISPLIT=10, MY2.5, plain FP64" % (args.nx, args.ny, n, kbm1),
-                   "elements": n, "layers": kbm1, "dti_s": dti, "mode": "graph" if args.graph else "eager"},
+                   "elements": n, "layers": kbm1, "dti_s": dti, "mode": "eager"},
         "elem_layer_steps_per_s": rate,
-        "roofline": {"bound": "hbm", "kernel": args.kernel, "achieved": achieved, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved * 1e9 / PEAK_HBM, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_us": avg_us,
-                     "launches": count},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": pk["achieved_GBps"], "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": pk["frac"], "traffic": traffic,
+                     "algorithmic_bytes_per_launch": pk["bytes"], "avg_launch_us": pk["avg_us"],
+                     "launches_per_step": pk["launches_per_step"]},
+        "roofline_step": {"modelled_bytes_per_step": tot_b / args.steps,
+                          "achieved_GBps": round(tot_b / tot_t / 1e9, 1) if tot_t else None,
+                          "frac": round(tot_b / tot_t / PEAK_HBM, 4) if tot_t else None,
+                          "kernel_ms_per_step": round(step_kernel_ms, 3)},
+        "roofline_by_kernel": per_kernel,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

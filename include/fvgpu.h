@@ -233,6 +233,13 @@ int  fv_model_set_tide(fv_model* mdl, int32_t ntide, const double* period,
  * mod_force.F:3470-3511): discharge q (m^3/s), temperature t, salinity s,
  * vertical distribution proportional to DZ.                             */
 int  fv_model_set_river(fv_model* mdl, int32_t node, double q, double t, double s);
+/* Batched instances: the disjoint union of nparts models (same KB, NTIDE,
+ * tidal periods and run parameters) as one model, instance p's entities
+ * renumbered after those of instances 0..p-1 (no counterpart in the
+ * reference, whose EnKF members run one after another through restart
+ * files, FVCOM_source/fvcom.F:2006-2128).  Every instance of the union
+ * advances exactly as it would alone.                                    */
+int  fv_model_union(const fv_model* const* parts, int32_t nparts, fv_model** out);
 /* Cold start: EL=0, rest, DENS2 density, RMEAN/TMEAN/SMEAN, default
  * turbulence (mod_startup.F:1062-1181, 1332-1408).                       */
 int  fv_model_cold_start(fv_model* mdl);
@@ -279,6 +286,9 @@ int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
  * named kernel family in microseconds, measured with HIP events on the
  * handle's stream.  Returns non-zero if the name is unknown.             */
 int  fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t* count);
+/* Enumerate the timed kernel families (idx = 0, 1, ... until non-zero):
+ * name (valid until the next profiling call), average us and launches.  */
+int  fvgpu_kernel_stat(fvgpu_handle* h, int32_t idx, const char** name, double* avg_us, int64_t* count);
 int  fvgpu_set_profiling(fvgpu_handle* h, int32_t on);
 /* Restrict the timing events to one kernel family (NULL or "": all).     */
 int  fvgpu_profile_only(fvgpu_handle* h, const char* name);

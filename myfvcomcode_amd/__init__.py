@@ -88,6 +88,7 @@ def grid_lib():
         L.fv_model_cold_start.argtypes = [C.c_void_p]
         L.fv_model_set_tide.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_set_river.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
+        L.fv_model_union.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
         L.fv_extent_len.restype = C.c_int64
         L.fv_extent_len.argtypes = [C.c_void_p, C.c_int32]
         _grid = L
@@ -113,6 +114,8 @@ def gpu_lib():
         L.fvgpu_put_state.argtypes = [vp, C.c_char_p, vp, C.c_int64]
         L.fvgpu_diagnostics.argtypes = [vp, C.POINTER(FvDiag)]
         L.fvgpu_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.fvgpu_kernel_stat.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int64)]
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
         L.fvgpu_profile_only.argtypes = [vp, C.c_char_p]
         L.fvgpu_synchronize.argtypes = [vp]
@@ -175,6 +178,22 @@ class Model:
                                   C.byref(params), C.byref(out))
         _check(L, rc, "fv_model_from_mesh")
         return cls(out.value)
+
+    @classmethod
+    def union(cls, parts):
+        """Batched instances: the disjoint union of ``parts`` as one model.
+        Returns (model, offsets) with offsets[p] = (node offset, element
+        offset, node count, element count) of instance p."""
+        L = grid_lib()
+        arr = (C.c_void_p * len(parts))(*[p._ptr.value for p in parts])
+        out = C.c_void_p()
+        _check(L, L.fv_model_union(arr, len(parts), C.byref(out)), "fv_model_union")
+        offs, om, on = [], 0, 0
+        for p in parts:
+            offs.append((om, on, p.m, p.n))
+            om += p.m
+            on += p.n
+        return cls(out.value), offs
 
     @property
     def ptr(self):
@@ -312,6 +331,21 @@ class DeviceModel:
         _check(L, L.fvgpu_kernel_time(self._h, name.encode(), C.byref(us), C.byref(cnt)),
                "fvgpu_kernel_time")
         return us.value, cnt.value
+
+    def kernel_stats(self):
+        """{family: (average us, launches)} of the events recorded so far."""
+        L = gpu_lib()
+        out = {}
+        i = 0
+        while True:
+            nm = C.c_char_p()
+            us = C.c_double()
+            cnt = C.c_int64()
+            if L.fvgpu_kernel_stat(self._h, i, C.byref(nm), C.byref(us), C.byref(cnt)) != 0:
+                break
+            out[nm.value.decode()] = (us.value, cnt.value)
+            i += 1
+        return out
 
     def set_profiling(self, on, kernel=None):
         """HIP events around launches (all families, or only ``kernel``)."""

@@ -191,7 +191,7 @@ void enqueue_internal_step(fvgpu_handle* h) {
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
   LAUNCH("adjust1_brough", k_adjust1_brough, nb(g.N + 1), 256, g, a);
-  LAUNCH("uvedge", k_uvedge<0>, nb(g.NE + 1), 256, g, a, a.u, a.v);
+  LAUNCH("uvedge_adv", k_uvedge<0>, nb(g.NE + 1), 256, g, a, a.u, a.v);
   LAUNCH("adv3d", k_adv3d_gather, nb(g.N + 1), 256, g, a);
   LAUNCH("baropg", k_baropg, nb(g.N + 1), 256, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) LAUNCH("rho1_roundtrip", k_rho1_roundtrip, nb(g.M + 1), 256, g, a);
@@ -204,29 +204,30 @@ void enqueue_internal_step(fvgpu_handle* h) {
 #define VERTVL_CALL(KM) LAUNCH("vertvl", k_vertvl_r<KM>, nb(g.M + 1), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VERTVL_CALL);
   LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
-  LAUNCH("viscofh", k_viscofh, nb((long)g.M * g.KBM1), 256, g, a);
-  LAUNCH("uvedge", k_uvedge<1>, nb(g.NE + 1), 256, g, a, a.u, a.v);
+#define VISCOFH_CALL(RM) LAUNCH("viscofh", k_viscofh_r<RM>, nb(g.M + 1), 256, g, a)
+  FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
+  LAUNCH("uvedge_uv", k_uvedge<1>, nb(g.NE + 1), 256, g, a, a.u, a.v);
   LAUNCH("advuv", k_advuv_gather, nb(g.N + 1), 256, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   LAUNCH("uv_finish", k_uv_finish, nb((long)g.N * g.KBM1), 256, g, a);
   LAUNCH("wreal", k_wreal, nb(g.N + 1), 256, g, a);
   const double qmean1 = (double)1.E-8f;
+  const int fct = h->md->p.scalar_positivity ? 1 : 0;
   if (h->md->p.vertical_closure) {
     for (int which = 0; which < 2; ++which) {
       double* q = which ? a.q2l : a.q2;
       double* qf = which ? a.q2lf : a.q2f;
-      LAUNCH("cvrec", k_cvrec<1>, nb(g.M + 1), 256, g, a, q, nullptr, qmean1, 2, g.KBM1);
+#define CVREC_Q_CALL(RM) LAUNCH("cvrec_q", (k_cvrec<1, RM>), nb(g.M + 1), 256, g, a, q, nullptr, qmean1, 2, g.KBM1)
+      FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
       LAUNCH("cvflux_q", k_cvflux_q, nb(g.NCV + 1), 256, g, a);
-#define ADV_Q_CALL(SM) LAUNCH("adv_q", k_adv_q_gather<SM>, nb(g.M + 1), 256, g, a, q, qf)
+#define ADV_Q_CALL(SM) LAUNCH("adv_q", k_adv_q_gather<SM>, nb(g.M + 1), 256, g, a, q, qf, fct)
       FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
     }
-    if (h->md->p.scalar_positivity) {
-      LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2, a.q2f);
-      LAUNCH("fct_q", k_fct_q, nb(g.M + 1), 256, g, a.q2l, a.q2lf);
-    }
-#define VDIF_Q_CALL(KM) LAUNCH("vdif_q", k_vdif_q_r<KM>, nb(g.M + 1), 256, g, a, a.gx)
-    FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
+#define VDIF_Q_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx)
+#define VDIF_Q_CALL16(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), nb(g.M + 1), 256, g, a, a.gx)
+    if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
+    else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
     LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
     LAUNCH("copy", k_copy, nb(mk), 256, a.q2l, a.q2lf, mk);
   }
@@ -238,12 +239,12 @@ void enqueue_internal_step(fvgpu_handle* h) {
     double* ff = which == 1 ? a.tf1 : a.sf1;
     const double* fm = which == 1 ? a.tmean1 : a.smean1;
     const double* fdis = which == 1 ? g.tdis : g.sdis;
-    LAUNCH("cvrec", k_cvrec<0>, nb(g.M + 1), 256, g, a, f1, fm, 0.0, 1, g.KBM1);
+#define CVREC_TS_CALL(RM) LAUNCH("cvrec_ts", (k_cvrec<0, RM>), nb(g.M + 1), 256, g, a, f1, fm, 0.0, 1, g.KBM1)
+    FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
     LAUNCH("cvflux_ts", k_cvflux_ts, nb(g.NCV + 1), 256, g, a);
-#define ADV_TS_CALL(SM) LAUNCH("adv_ts", k_adv_ts_gather<SM>, nb(g.M + 1), 256, g, a, f1, ff, fdis, which)
+#define ADV_TS_CALL(SM) LAUNCH("adv_ts", k_adv_ts_gather<SM>, nb(g.M + 1), 256, g, a, f1, ff, fdis, which, fct)
     FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
     if (g.IOBCN > 0) LAUNCH("adv_ts_obc", k_adv_ts_obc, nb((long)g.IOBCN * g.KBM1), 256, g, a);
-    if (h->md->p.scalar_positivity) LAUNCH("fct_ts", k_fct_ts, nb(g.M + 1), 256, g, f1, ff);
 #define VDIF_TS_CALL(KM) LAUNCH("vdif_ts", k_vdif_ts_r<KM>, nb(g.M + 1), 256, g, a, ff, which)
     FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
     LAUNCH("bcond_ts", k_bcond_ts, nb(g.IOBCN + 1), 256, g, a, which);
@@ -275,8 +276,14 @@ int build_lists(fvgpu_handle* h) {
     ent[off[ia] + fill[ia]++] = (c << 1) | 0;
     ent[off[ib] + fill[ib]++] = (c << 1) | 1;
   }
-  for (int i = 1; i <= M; ++i)
+  g.ring_max = 1;
+  g.ve_max = 1;
+  for (int i = 1; i <= M; ++i) {
     if (md->ntsn[i] > FV_MAX_RING || md->ntsn[i] < 1) return fail("node ring NTSN outside [1, FV_MAX_RING]");
+    if (md->ntve[i] > FV_MAX_RING || md->ntve[i] < 1) return fail("node ring NTVE outside [1, FV_MAX_RING]");
+    g.ring_max = std::max(g.ring_max, md->ntsn[i]);
+    g.ve_max = std::max(g.ve_max, md->ntve[i]);
+  }
   if (dupload(h, &g.cv_off, off.data(), off.size())) return 1;
   if (dupload(h, &g.cv_ent, ent.data(), ent.size())) return 1;
   {
@@ -482,6 +489,7 @@ int upload_state(fvgpu_handle* h) {
   if (dalloc(h, &a.ef3, ((size_t)g.NE + 1) * g.KB)) return 1;
   RW(efvisc, nullptr, ((size_t)g.NE + 1) * g.KB);
   if (dalloc(h, &a.rec, mk)) return 1;
+  if (dalloc(h, &a.rlohi, mk)) return 1;
   RW(cvflux, nullptr, ((size_t)g.NCV + 1) * g.KB);
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
   RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
@@ -629,8 +637,20 @@ int fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64
 int fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);  // fvgpu_diag.hip
 
 int fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t* count) {
+  if (flush_profile(h)) return 1;
   auto it = h->stats.find(name);
   if (it == h->stats.end()) return fail(std::string("no timing for ") + name);
+  *count = it->second.count;
+  *avg_us = it->second.count ? 1000.0 * it->second.total_ms / it->second.count : 0.0;
+  return 0;
+}
+
+int fvgpu_kernel_stat(fvgpu_handle* h, int32_t idx, const char** name, double* avg_us, int64_t* count) {
+  if (flush_profile(h)) return 1;
+  if (idx < 0 || idx >= (int32_t)h->stats.size()) return fail("kernel statistics index out of range");
+  auto it = h->stats.begin();
+  std::advance(it, idx);
+  *name = it->first.c_str();
   *count = it->second.count;
   *avg_us = it->second.count ? 1000.0 * it->second.total_ms / it->second.count : 0.0;
   return 0;
@@ -639,7 +659,7 @@ int fvgpu_kernel_time(fvgpu_handle* h, const char* name, double* avg_us, int64_t
 int fvgpu_set_profiling(fvgpu_handle* h, int32_t on) {
   if (h->profiling && flush_profile(h)) return 1;
   h->profiling = on != 0;
-  h->stats.clear();
+  if (h->profiling) h->stats.clear();   // statistics stay readable after switching off
   return 0;
 }
 

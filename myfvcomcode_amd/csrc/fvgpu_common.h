@@ -11,6 +11,7 @@
 
 #define FV_MAXKB 64
 #define FV_MAX_CV 32   // control-volume sub-edges per node (twice the node degree)
+#define FV_MAX_RING 16 // NTSN / NTVE bound of the node rings held in registers
 
 struct DevGrid {
   int M, N, NE, NCV, KB, KBM1, KBM2, IOBCN, NUMQBC, NTIDE;
@@ -40,6 +41,7 @@ struct DevGrid {
   // consecutive nodes read consecutive addresses for each slot.  cvl_sdx/sdy
   // are DLTXE/DLTYE negated on side 0 (XFLUX(IA) -= f  ==  XFLUX(IA) += (-f)).
   int cv_max;
+  int ring_max, ve_max;            // max NTSN, max NTVE
   const int *cv_cnt, *cvl_ent, *cvl_elem;
   const double *cvl_sdx, *cvl_sdy;
   const int *ed_ent;               // element -> 3 x (edge<<1 | side), sorted by edge
@@ -143,4 +145,11 @@ __device__ __forceinline__ void static_for(F&& f) {
     if ((SMAX_) <= 16) { CALL(16); }                 \
     else if ((SMAX_) <= 24) { CALL(24); }            \
     else { CALL(32); }                               \
+  } while (0)
+
+// Host-side choice of the node-ring bound RMAX >= ring_max (NTSN or NTVE).
+#define FV_RMAX_DISPATCH(RM_, CALL)                  \
+  do {                                               \
+    if ((RM_) <= 10) { CALL(10); }                   \
+    else { CALL(16); }                               \
   } while (0)

@@ -28,6 +28,7 @@ struct IntArgs {
   EdgeFlux4* ef3;        // per edge-layer momentum terms (3-D), (NE+1) x KB
   double* efvisc;        // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB
   CvRec* rec;            // node-layer records of the field being advected
+  double2* rlohi;        // node-layer min/max of the field over the NBSN ring (FCT bounds)
   double* cvflux;        // per sub-edge, per layer face flux (NCV+1) x KB
   // 2-D
   double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;
@@ -207,56 +208,12 @@ __global__ void k_adjust2(DevGrid g, IntArgs a) {
 
 // N2E3D (mod_main.F:1633-1645) over layers 1..kmax.
 __global__ void k_n2e3d(DevGrid g, const double* nvar, double* evar, int kmax) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = xcd_thread();
   const long tot = (long)g.N * kmax;
   if (t >= tot) return;
   const int i = (int)(t % g.N) + 1, k = (int)(t / g.N) + 1;
   const double one_third = 1.0 / 3.0;
   evar[EI(g, i, k)] = one_third * (nvar[NI(g, NV_(i, 1), k)] + nvar[NI(g, NV_(i, 2), k)] + nvar[NI(g, NV_(i, 3), k)]);
-}
-
-// VISCOF_H (viscofh.F:33-127), one node-layer per thread.
-__global__ void k_viscofh(DevGrid g, IntArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.M * g.KBM1) return;
-  const int i = (int)(t % g.M) + 1, k = (int)(t / g.M) + 1;
-  const long ko = (long)(k - 1) * g.n1;
-  const int o = g.nbve_off[i], nt = g.ntve[i];
-  const bool bnd = g.isonb[i] != 0;
-  double pupx = 0.0, pupy = 0.0, pvpx = 0.0, pvpy = 0.0;
-  // first element, interior elements, last element: for NTVE==1 the single
-  // element enters twice, as in the reference's three-part loop
-  {
-    const int i1 = g.nbve[o];
-    const double uu = a.u[ko + i1], vv = a.v[ko + i1];
-    pupx = pupx + uu * g.dltyecec[o]; pupy = pupy + uu * g.dltxecec[o];
-    pvpx = pvpx + vv * g.dltyecec[o]; pvpy = pvpy + vv * g.dltxecec[o];
-    if (bnd) {
-      pupx = pupx + uu * g.dltynec[o]; pupy = pupy + uu * g.dltxnec[o];
-      pvpx = pvpx + vv * g.dltynec[o]; pvpy = pvpy + vv * g.dltxnec[o];
-    }
-  }
-  for (int j = 1; j <= nt - 2; ++j) {
-    const int i1 = g.nbve[o + j];
-    const double uu = a.u[ko + i1], vv = a.v[ko + i1];
-    pupx = pupx + uu * g.dltyecec[o + j]; pupy = pupy + uu * g.dltxecec[o + j];
-    pvpx = pvpx + vv * g.dltyecec[o + j]; pvpy = pvpy + vv * g.dltxecec[o + j];
-  }
-  {
-    const int j = nt - 1, i1 = g.nbve[o + j];
-    const double uu = a.u[ko + i1], vv = a.v[ko + i1];
-    pupx = pupx + uu * g.dltyecec[o + j]; pupy = pupy + uu * g.dltxecec[o + j];
-    pvpx = pvpx + vv * g.dltyecec[o + j]; pvpy = pvpy + vv * g.dltxecec[o + j];
-    if (bnd) {
-      pupx = pupx + uu * (-g.dltynec[o + j]); pupy = pupy + uu * (-g.dltxnec[o + j]);
-      pvpx = pvpx + vv * (-g.dltynec[o + j]); pvpy = pvpy + vv * (-g.dltxnec[o + j]);
-    }
-  }
-  const double art1 = g.art1[i];
-  pupx = pupx / art1; pupy = pupy / art1; pvpx = pvpx / art1; pvpy = pvpy / art1;
-  const double tmp1 = sq(pupx) + sq(pvpy);
-  const double tmp2 = 0.5 * sq(pupy + pvpx);
-  a.viscofh[NI(g, i, k)] = sqrt(tmp1 + tmp2) * art1;
 }
 
 // WREAL (wreal.F:32-140).

@@ -167,9 +167,9 @@ __device__ __forceinline__ Q25Level q25_level(const DevGrid& g, const IntArgs& a
 
 // VDIF_Q, MY2.5 (vdif_q.F:34-551).  PROD (which needs ring averages of U/V)
 // is formed once per level into the node-layer scratch array `prod`.
-template <int KMAX>
+template <int KMAX, int RMAX>
 __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* prod) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)xcd_thread();
   if (i > g.M) return;
   const int KB = g.KB, KBM1 = g.KBM1;
   const double A1 = 0.92, B1 = 16.6, A2 = 0.74, B2 = 10.1, C1 = 0.08, E1 = 1.80, E2 = 1.33, SEF = 1.00;
@@ -212,22 +212,34 @@ __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* 
   }
   wub = wub / ntd; wvb = wvb / ntd;
   Q2F[NI(g, i, KB)] = sqrt(sq(wub) + sq(wvb)) * g.const1;
-  // PROD: buoyancy part times zero, shear and buoyancy production
+  // PROD: buoyancy part times zero, shear and buoyancy production.  The
+  // ring averages of U/V at layer k-1 are those of the previous level.
+  int re[RMAX];
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    re[j] = j < nt ? g.nbve[o + j] : 0;
+  });
+  auto ring_avg = [&](const double* f, long eo) {
+    double x = 0.0;
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j < nt) x = x + f[eo + re[j]];
+    });
+    return x / ntd;
+  };
+  double ukm1 = 0.0, vkm1 = 0.0;
+  if (d > 0.0) { ukm1 = ring_avg(a.u, 0); vkm1 = ring_avg(a.v, 0); }
   for (int k = 2; k <= KBM1; ++k) {
     const long q = NI(g, i, k);
     const double boy = g.grav_n[i] * (a.rho1[q - m1] - a.rho1[q]) / (DZZ(i, k - 1) * d);
     double pr = KM[q] * 0.0 * fv_pow15(.5 * (-boy + fabs(boy)));
     if (d > 0.0) {
       const long eo = (long)(k - 1) * g.n1;
-      double uk = 0.0, vk = 0.0, ukm1 = 0.0, vkm1 = 0.0;
-      for (int j = 0; j < nt; ++j) {
-        const int e = g.nbve[o + j];
-        uk = uk + a.u[eo + e]; vk = vk + a.v[eo + e];
-        ukm1 = ukm1 + a.u[eo - g.n1 + e]; vkm1 = vkm1 + a.v[eo - g.n1 + e];
-      }
-      uk = uk / ntd; vk = vk / ntd; ukm1 = ukm1 / ntd; vkm1 = vkm1 / ntd;
+      const double uk = ring_avg(a.u, eo), vk = ring_avg(a.v, eo);
       pr = pr + KM[q] * SEF * (sq(uk - ukm1) + sq(vk - vkm1)) / sq(DZZ(i, k - 1) * d);
       pr = pr + KH[q] * boy;
+      ukm1 = uk;
+      vkm1 = vk;
     }
     prod[q] = pr;
   }

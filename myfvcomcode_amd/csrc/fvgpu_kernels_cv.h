@@ -18,21 +18,22 @@
 #pragma once
 #include "fvgpu_kernels_column.h"
 
-#define FV_MAX_RING 16   // NTSN bound of the register ring in k_cvrec_*
 
 // Ring gradient / bounds of field f at node i for the layers [k0, k1]
-// (adv_t.F:219-376 and :469-480; adv_q.F the same on layer interfaces).
+// (adv_t.F:219-376 and :469-480; adv_q.F the same on layer interfaces),
+// and the ring-only min/max that FCT_T/FCT_S/FCT_Q2 clip to
+// (fct_t.F:74-80).  RMAX >= NTSN bounds the ring held in registers.
 // kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
 // DZ-weighted interface viscosity (adv_q.F) times NN_HVC.
-template <int KIND>
+template <int KIND, int RMAX>
 __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const double* f, const double* mean,
                                                double cmean, int k0, int k1) {
   const int i = (int)xcd_thread();
   if (i < 1 || i > g.M) return;
   const int o = g.nbsn_off[i], nt = g.ntsn[i];
-  int rn[FV_MAX_RING];
-  double rdx[FV_MAX_RING], rdy[FV_MAX_RING];
-  static_for<FV_MAX_RING>([&](auto jc) {
+  int rn[RMAX];
+  double rdx[RMAX], rdy[RMAX];
+  static_for<RMAX>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
     rn[j] = j < nt ? g.nbsn[o + j] : 0;
     rdx[j] = j < nt - 1 ? g.dltxtrie[o + j] : 0.0;
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const doubl
     double f1 = f[ko + rn[0]];
     double m1v = mean ? mean[ko + rn[0]] : cmean;
     double lo = f1, hi = f1;
-    static_for<FV_MAX_RING - 1>([&](auto jc) {
+    static_for<RMAX - 1>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j < nt - 1) {
         const double f2 = f[ko + rn[j + 1]];
@@ -80,6 +81,8 @@ __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const doubl
     r.gyd = s4 / art2;
     r.vhn = vhn;
     a.rec[ko + i] = r;
+    // the ring alone (entry NTSN repeats entry 1): the FCT bounds
+    a.rlohi[ko + i] = make_double2(lo, hi);
   }
 }
 
@@ -164,9 +167,10 @@ __global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
 // TF1 = (T1 - XFLUX/ART1*DTI/(DT*DZ))*DT/DTFA.
 template <int SMAX>
 __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, const double* f1, double* ff,
-                                                        const double* fdis, int which) {
+                                                        const double* fdis, int which, int fct) {
   const int i = (int)xcd_thread();
   if (i < 1 || i > g.M) return;
+  const bool clip = fct && !g.fct_skip[i];
   const int n = g.cv_cnt[i];
   int ce[SMAX];
   static_for<SMAX>([&](auto jc) {
@@ -195,18 +199,24 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
       a.ah_bottom[i] = (g.fact * a.viscofh[ko + i] + g.fm1) * g.nn_hvc[i];
     }
     const double fk = f1[ko + i], dzk = DZ(i, k);
-    double temp;
-    if (k == 1) {
-      const double dzp = DZ(i, k + 1);
-      temp = -a.wts[ko + g.m1 + i] * (fk * dzp + f1[ko + g.m1 + i] * dzk) / (dzk + dzp);
-    } else if (k == g.KBM1) {
+    // DZ-weighted values at the layer's upper (nu/du) and lower (nd/dd)
+    // interfaces: the vertical flux (adv_t.F:873-881) and the FCT bounds
+    // (fct_t.F:82-101) use the same expressions
+    double temp, nu = 0.0, du = 1.0, nd = 0.0, dd = 1.0;
+    const bool has_up = k != 1, has_dn = k == 1 || k != g.KBM1;   // the reference's branch order
+    if (has_up) {
       const double dzm = DZ(i, k - 1);
-      temp = a.wts[ko + i] * (fk * dzm + f1[ko - g.m1 + i] * dzk) / (dzk + dzm);
-    } else {
-      const double dzp = DZ(i, k + 1), dzm = DZ(i, k - 1);
-      temp = a.wts[ko + i] * (fk * dzm + f1[ko - g.m1 + i] * dzk) / (dzk + dzm) -
-             a.wts[ko + g.m1 + i] * (fk * dzp + f1[ko + g.m1 + i] * dzk) / (dzk + dzp);
+      nu = fk * dzm + f1[ko - g.m1 + i] * dzk;
+      du = dzk + dzm;
     }
+    if (has_dn) {
+      const double dzp = DZ(i, k + 1);
+      nd = fk * dzp + f1[ko + g.m1 + i] * dzk;
+      dd = dzk + dzp;
+    }
+    if (k == 1) temp = -a.wts[ko + g.m1 + i] * nd / dd;
+    else if (k == g.KBM1) temp = a.wts[ko + i] * nu / du;
+    else temp = a.wts[ko + i] * nu / du - a.wts[ko + g.m1 + i] * nd / dd;
     if (isobc) xflux = temp * art1;
     else xflux = xflux + temp * art1;
     if (!g.river_ts_specified) {
@@ -216,7 +226,16 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
           if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j] * fdis[j];
     }
     if (which == 1) xflux = xflux - 0.0 * art1;
-    ff[ko + i] = (fk - xflux / art1 * (g.dti / (a.dt[i] * dzk))) * (a.dt[i] / a.dtfa[i]);
+    double x = (fk - xflux / art1 * (g.dti / (a.dt[i] * dzk))) * (a.dt[i] / a.dtfa[i]);
+    if (clip) {  // FCT_T / FCT_S (fct_t.F:74-101)
+      const double2 lh = a.rlohi[ko + i];
+      double tmin = lh.x, tmax = lh.y;
+      if (has_up) { const double v = nu / du; tmax = fmax(tmax, v); tmin = fmin(tmin, v); }
+      if (has_dn) { const double v = nd / dd; tmax = fmax(tmax, v); tmin = fmin(tmin, v); }
+      if (tmin - x > 0.0) x = tmin;
+      if (x - tmax > 0.0) x = tmax;
+    }
+    ff[ko + i] = x;
   }
 }
 
@@ -261,7 +280,7 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
 
 // Phase 3 of ADV_Q (adv_q.F:340-390): ordered sums, vertical flux, update.
 template <int SMAX>
-__global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, const double* q, double* qf) {
+__global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, const double* q, double* qf, int fct) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
   if (i == 0) {
@@ -276,6 +295,7 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
   });
   const double art1 = g.art1[i], dti = g.dti, dtn = a.dt[i], dn = a.d[i];
   const long cs = (long)g.NCV + 1;
+  const bool clip = fct && !g.fct_skip[i];
   qf[NI(g, i, 1)] = 0.0;
   qf[NI(g, i, g.KB)] = 0.0;
   for (int k = 2; k <= g.KBM1; ++k) {
@@ -292,7 +312,23 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
     });
     const double temp = a.wts[ko - g.m1 + i] * q[ko - g.m1 + i] - a.wts[ko + g.m1 + i] * q[ko + g.m1 + i];
     xflux = xflux + temp * art1 * DZZ(i, k - 1) / (DZ(i, k - 1) + DZ(i, k));
-    qf[ko + i] = (q[ko + i] - xflux / art1 * (dti / (dtn * DZZ(i, k - 1)))) * (dtn / dn);
+    const double qk = q[ko + i];
+    double x = (qk - xflux / art1 * (dti / (dtn * DZZ(i, k - 1)))) * (dtn / dn);
+    if (clip) {  // FCT_Q2 / FCT_Q2L (fct_q2.F:60-95)
+      const double2 lh = a.rlohi[ko + i];
+      double qmin = lh.x, qmax = lh.y;
+      if (k != 2) {
+        const double v = (qk * DZZ(i, k - 2) + q[ko - g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k - 1) + DZZ(i, k - 2));
+        qmax = fmax(qmax, v); qmin = fmin(qmin, v);
+      }
+      if (k == 2 || k != g.KBM1) {
+        const double v = (qk * DZZ(i, k) + q[ko + g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k) + DZZ(i, k - 1));
+        qmax = fmax(qmax, v); qmin = fmin(qmin, v);
+      }
+      if (qmin - x > 0.0) x = qmin;
+      if (x - qmax > 0.0) x = qmax;
+    }
+    qf[ko + i] = x;
   }
 }
 
@@ -361,4 +397,65 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
     }
   });
   a.dtfa[i] = dtfa;
+}
+
+// VISCOF_H (viscofh.F:33-127): nodal strain-rate magnitude from the NBVE
+// element ring, one thread per node with the ring (RMAX >= NTVE) held in
+// registers across the layers.  The first and last ring elements carry the
+// boundary terms of nodes on the boundary; with NTVE = 1 the single element
+// enters twice, as in the reference's three-part loop.
+template <int RMAX>
+__global__ __launch_bounds__(256) void k_viscofh_r(DevGrid g, IntArgs a) {
+  const int i = (int)xcd_thread();
+  if (i < 1 || i > g.M) return;
+  const int o = g.nbve_off[i], nt = g.ntve[i];
+  const bool bnd = g.isonb[i] != 0;
+  int re[RMAX];
+  double rx[RMAX], ry[RMAX];
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    re[j] = j < nt ? g.nbve[o + j] : 0;
+    rx[j] = j < nt ? g.dltxecec[o + j] : 0.0;
+    ry[j] = j < nt ? g.dltyecec[o + j] : 0.0;
+  });
+  const int jl = nt - 1;
+  const int el = g.nbve[o + jl];
+  const double rxl = g.dltxecec[o + jl], ryl = g.dltyecec[o + jl];
+  const double nx0 = bnd ? g.dltxnec[o] : 0.0, ny0 = bnd ? g.dltynec[o] : 0.0;
+  const double nxl = bnd ? g.dltxnec[o + jl] : 0.0, nyl = bnd ? g.dltynec[o + jl] : 0.0;
+  const double art1 = g.art1[i];
+  for (int k = 1; k <= g.KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.n1;
+    double pupx = 0.0, pupy = 0.0, pvpx = 0.0, pvpy = 0.0;
+    {
+      const double uu = a.u[ko + re[0]], vv = a.v[ko + re[0]];
+      pupx = pupx + uu * ry[0]; pupy = pupy + uu * rx[0];
+      pvpx = pvpx + vv * ry[0]; pvpy = pvpy + vv * rx[0];
+      if (bnd) {
+        pupx = pupx + uu * ny0; pupy = pupy + uu * nx0;
+        pvpx = pvpx + vv * ny0; pvpy = pvpy + vv * nx0;
+      }
+    }
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j >= 1 && j <= nt - 2) {
+        const double uu = a.u[ko + re[j]], vv = a.v[ko + re[j]];
+        pupx = pupx + uu * ry[j]; pupy = pupy + uu * rx[j];
+        pvpx = pvpx + vv * ry[j]; pvpy = pvpy + vv * rx[j];
+      }
+    });
+    {
+      const double uu = a.u[ko + el], vv = a.v[ko + el];
+      pupx = pupx + uu * ryl; pupy = pupy + uu * rxl;
+      pvpx = pvpx + vv * ryl; pvpy = pvpy + vv * rxl;
+      if (bnd) {
+        pupx = pupx + uu * (-nyl); pupy = pupy + uu * (-nxl);
+        pvpx = pvpx + vv * (-nyl); pvpy = pvpy + vv * (-nxl);
+      }
+    }
+    pupx = pupx / art1; pupy = pupy / art1; pvpx = pvpx / art1; pvpy = pvpy / art1;
+    const double tmp1 = sq(pupx) + sq(pvpy);
+    const double tmp2 = 0.5 * sq(pupy + pvpx);
+    a.viscofh[NI(g, i, k)] = sqrt(tmp1 + tmp2) * art1;
+  }
 }

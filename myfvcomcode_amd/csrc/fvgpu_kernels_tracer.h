@@ -5,62 +5,6 @@
 #pragma once
 #include "fvgpu_kernels_3d.h"
 
-// FCT_Q2 / FCT_Q2L (fct_q2.F:31-104).
-__global__ void k_fct_q(DevGrid g, const double* q, double* qf) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > g.M || g.fct_skip[i]) return;
-  const int o = g.nbsn_off[i], nt = g.ntsn[i];
-  for (int k = 2; k <= g.KBM1; ++k) {
-    const long ko = (long)(k - 1) * g.m1;
-    double qmax = q[ko + g.nbsn[o]], qmin = qmax;
-    for (int j = 1; j < nt; ++j) { const double x = q[ko + g.nbsn[o + j]]; qmax = fmax(qmax, x); qmin = fmin(qmin, x); }
-    const double qk = q[ko + i];
-    if (k == 2) {
-      const double v = (qk * DZZ(i, k) + q[ko + g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k) + DZZ(i, k - 1));
-      qmax = fmax(qmax, v); qmin = fmin(qmin, v);
-    } else if (k == g.KBM1) {
-      const double v = (qk * DZZ(i, k - 2) + q[ko - g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k - 1) + DZZ(i, k - 2));
-      qmax = fmax(qmax, v); qmin = fmin(qmin, v);
-    } else {
-      const double v1 = (qk * DZZ(i, k - 2) + q[ko - g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k - 1) + DZZ(i, k - 2));
-      const double v2 = (qk * DZZ(i, k) + q[ko + g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k) + DZZ(i, k - 1));
-      qmax = fmax(qmax, fmax(v1, v2)); qmin = fmin(qmin, fmin(v1, v2));
-    }
-    double x = qf[ko + i];
-    if (qmin - x > 0.0) x = qmin;
-    if (x - qmax > 0.0) x = qmax;
-    qf[ko + i] = x;
-  }
-}
-
-// FCT_T / FCT_S (fct_t.F:33-108).
-__global__ void k_fct_ts(DevGrid g, const double* f1, double* ff) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > g.M || g.fct_skip[i]) return;
-  const int o = g.nbsn_off[i], nt = g.ntsn[i];
-  for (int k = 1; k <= g.KBM1; ++k) {
-    const long ko = (long)(k - 1) * g.m1;
-    double tmax = f1[ko + g.nbsn[o]], tmin = tmax;
-    for (int j = 1; j < nt; ++j) { const double x = f1[ko + g.nbsn[o + j]]; tmax = fmax(tmax, x); tmin = fmin(tmin, x); }
-    const double fk = f1[ko + i];
-    if (k == 1) {
-      const double v = (fk * DZ(i, k + 1) + f1[ko + g.m1 + i] * DZ(i, k)) / (DZ(i, k) + DZ(i, k + 1));
-      tmax = fmax(tmax, v); tmin = fmin(tmin, v);
-    } else if (k == g.KBM1) {
-      const double v = (fk * DZ(i, k - 1) + f1[ko - g.m1 + i] * DZ(i, k)) / (DZ(i, k) + DZ(i, k - 1));
-      tmax = fmax(tmax, v); tmin = fmin(tmin, v);
-    } else {
-      const double v1 = (fk * DZ(i, k - 1) + f1[ko - g.m1 + i] * DZ(i, k)) / (DZ(i, k) + DZ(i, k - 1));
-      const double v2 = (fk * DZ(i, k + 1) + f1[ko + g.m1 + i] * DZ(i, k)) / (DZ(i, k) + DZ(i, k + 1));
-      tmax = fmax(tmax, fmax(v1, v2)); tmin = fmin(tmin, fmin(v1, v2));
-    }
-    double x = ff[ko + i];
-    if (tmin - x > 0.0) x = tmin;
-    if (x - tmax > 0.0) x = tmax;
-    ff[ko + i] = x;
-  }
-}
-
 // BCOND_TS(NCON) (bcond_ts.F:28-268), one thread per OBC node; thread
 // IOBCN.. handles the specified river values first.
 __global__ void k_bcond_ts(DevGrid g, IntArgs a, int ncon) {

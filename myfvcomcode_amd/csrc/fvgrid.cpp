@@ -903,3 +903,137 @@ extern "C" int fv_model_cold_start(fv_model* md) {
   for (size_t q = 0; q < n1 * KB; ++q) { md->u[q] = md->v[q] = md->w[q] = md->ww[q] = 0.0; }
   return 0;
 }
+
+// --------------------------------------------------------------------------
+// Batched instances as one mesh: the disjoint union of independent models.
+// Every kernel of the time step is local to a node's or element's own
+// neighbourhood, and each gather list is sorted by sub-edge / edge index, so
+// instance p of the union advances exactly as it would alone.  Entities are
+// renumbered by offsets (instance p's node i becomes offM[p] + i); the dummy
+// entry 0 is shared.
+// --------------------------------------------------------------------------
+namespace {
+
+enum RefKind { REF_NONE = 0, REF_NODE, REF_ELEM };
+
+RefKind field_ref(const char* name) {
+  static const char* node_refs[] = {"nbsn", "inodeq", "i_obc_n", "next_obc", "ienode", "niec", "nv"};
+  static const char* elem_refs[] = {"nbe", "nbve", "iec", "ntrg", "adjc_obc"};
+  for (const char* n : node_refs) if (std::strcmp(n, name) == 0) return REF_NODE;
+  for (const char* n : elem_refs) if (std::strcmp(n, name) == 0) return REF_ELEM;
+  return REF_NONE;
+}
+
+// entity count of an extent with a dummy 0 (or a plain list) and its components
+struct ExtShape { int kind; int comps; };   // kind 0: 0..cnt with dummy, 1: list 0..cnt-1, 2: CSR offsets, 3: shared
+ExtShape ext_shape(int ext, const fv_model* md) {
+  switch (ext) {
+    case FVX_M1: case FVX_N1: case FVX_NE1: case FVX_CV1: return {0, 1};
+    case FVX_N1x3: return {0, 3};
+    case FVX_N1x4: return {0, 4};
+    case FVX_NE1x2: case FVX_CV1x2: return {0, 2};
+    case FVX_M1K: case FVX_N1K: return {0, md->kb};
+    case FVX_M2: return {2, 1};
+    case FVX_RSN: case FVX_RVE: case FVX_OBC: case FVX_RIV: return {1, 1};
+    case FVX_OBCx2: return {1, 2};
+    case FVX_OBCxT: return {1, md->ntide};
+    case FVX_RIVxK: return {1, md->kbm1};
+    case FVX_TIDE: return {3, 1};
+  }
+  return {3, 1};
+}
+
+int64_t ext_count(int ext, const fv_model* md) {
+  switch (ext) {
+    case FVX_M1: case FVX_M1K: case FVX_M2: return md->m;
+    case FVX_N1: case FVX_N1x3: case FVX_N1x4: case FVX_N1K: return md->n;
+    case FVX_NE1: case FVX_NE1x2: return md->ne;
+    case FVX_CV1: case FVX_CV1x2: return md->ncv;
+    case FVX_RSN: return md->nring_sn;
+    case FVX_RVE: return md->nring_ve;
+    case FVX_OBC: case FVX_OBCx2: case FVX_OBCxT: return md->iobcn;
+    case FVX_RIV: case FVX_RIVxK: return md->numqbc;
+    case FVX_TIDE: return md->ntide;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int fv_model_union(const fv_model* const* parts, int32_t nparts, fv_model** out) {
+  if (nparts < 1 || !parts) return fail("fv_model_union: no parts");
+  const fv_model* p0 = parts[0];
+  for (int p = 0; p < nparts; ++p) {
+    const fv_model* q = parts[p];
+    if (q->kb != p0->kb || q->ntide != p0->ntide)
+      return fail("fv_model_union: parts differ in KB or NTIDE");
+    if (std::memcmp(&q->p, &p0->p, sizeof(fv_params)) != 0)
+      return fail("fv_model_union: parts differ in run parameters");
+    for (int t = 0; t < p0->ntide; ++t)
+      if (q->period[t] != p0->period[t]) return fail("fv_model_union: parts differ in tidal periods");
+    if (q->nbsn_off[0] != 0 || q->nbsn_off[1] != 0 || q->nbve_off[0] != 0 || q->nbve_off[1] != 0)
+      return fail("fv_model_union: node 0 must have empty rings");
+  }
+  fv_model* md = (fv_model*)xalloc(sizeof(fv_model));
+  *md = fv_model{};
+  md->kb = p0->kb; md->kbm1 = p0->kbm1; md->kbm2 = p0->kbm2; md->ntide = p0->ntide; md->p = p0->p;
+  std::vector<int64_t> offM(nparts), offN(nparts);
+  for (int p = 0; p < nparts; ++p) {
+    const fv_model* q = parts[p];
+    offM[p] = md->m; offN[p] = md->n;
+    md->m += q->m; md->n += q->n; md->ne += q->ne; md->ncv += q->ncv; md->ncv_i += q->ncv_i;
+    md->iobcn += q->iobcn; md->numqbc += q->numqbc; md->nring_sn += q->nring_sn; md->nring_ve += q->nring_ve;
+  }
+  alloc_all(md);
+  for (int f = 0; f < kNFields; ++f) {
+    const int ext = kFields[f].ext;
+    const size_t esz = kFields[f].dtype == 0 ? 8 : 4;
+    const RefKind ref = kFields[f].dtype == 1 ? field_ref(kFields[f].name) : REF_NONE;
+    char* dst = *(char**)((char*)md + kFields[f].off);
+    const ExtShape sh = ext_shape(ext, md);
+    const int64_t tot = ext_count(ext, md);
+    if (sh.kind == 3) {   // shared (tidal periods)
+      std::memcpy(dst, *(char* const*)((const char*)p0 + kFields[f].off), (size_t)fv_extent_len(md, ext) * esz);
+      continue;
+    }
+    if (sh.kind == 2) {   // CSR offsets over nodes: ring starts shifted by the preceding parts' entries
+      int32_t* o = (int32_t*)dst;
+      const bool sn = std::strcmp(kFields[f].name, "nbsn_off") == 0;
+      int32_t shift = 0;
+      o[0] = 0;
+      for (int p = 0; p < nparts; ++p) {
+        const int32_t* src = *(int32_t* const*)((const char*)parts[p] + kFields[f].off);
+        for (int i = 1; i <= parts[p]->m; ++i) o[offM[p] + i] = src[i] + shift;
+        shift += sn ? parts[p]->nring_sn : parts[p]->nring_ve;
+      }
+      o[md->m + 1] = shift;
+      continue;
+    }
+    const int base = sh.kind == 0 ? 1 : 0;            // first copied entry of each component
+    const int64_t stride = tot + (sh.kind == 0 ? 1 : 0);
+    int64_t at = 0;                                   // entities written so far (per component)
+    for (int p = 0; p < nparts; ++p) {
+      const fv_model* q = parts[p];
+      const char* src = *(char* const*)((const char*)q + kFields[f].off);
+      const int64_t cnt = ext_count(ext, q);
+      const int64_t sstride = cnt + (sh.kind == 0 ? 1 : 0);
+      const int32_t add = ref == REF_NODE ? (int32_t)offM[p] : ref == REF_ELEM ? (int32_t)offN[p] : 0;
+      for (int c = 0; c < sh.comps; ++c) {
+        if (p == 0 && sh.kind == 0)   // the shared dummy entry 0 of every component
+          std::memcpy(dst + (size_t)(c * stride) * esz, src + (size_t)(c * sstride) * esz, esz);
+        for (int64_t i = 0; i < cnt; ++i) {
+          const size_t d = (size_t)(c * stride + base + at + i), s = (size_t)(c * sstride + base + i);
+          if (ref != REF_NONE) {
+            const int32_t v = ((const int32_t*)src)[s];
+            ((int32_t*)dst)[d] = v > 0 ? v + add : v;
+          } else {
+            std::memcpy(dst + d * esz, src + s * esz, esz);
+          }
+        }
+      }
+      at += cnt;
+    }
+  }
+  *out = md;
+  return 0;
+}

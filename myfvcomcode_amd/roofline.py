@@ -1,10 +1,16 @@
 """Algorithmic (compulsory) HBM bytes per kernel launch.
 
-Bytes = distinct array entries a launch must read + entries it writes, with
-static geometry and topology counted, intermediate arrays counted only when
-they cross a kernel boundary.  One function per kernel family, in terms of
-the mesh sizes, so that the benchmark and the documentation use the same
-model.  FP64 reals are 8 bytes, indices 4 bytes.
+Bytes = distinct array entries a launch must read + entries it writes.
+Static geometry and topology are counted; intermediate arrays are counted
+only where they cross a kernel boundary in this design (the per-sub-edge and
+per-edge flux buffers, the node-layer records).  One function per kernel
+family (the names the library's timing events use), in terms of the mesh
+sizes, so that the benchmark and the documentation use the same model.
+FP64 reals are 8 bytes, indices 4 bytes; `real` is the size of the state
+reals (8 for the FP64 path).
+
+Per-stage conditional traffic (EGF at RK stage 3, UARD at stage 3, the
+OBC and river branches) is left out: it is small and not every launch has it.
 """
 
 R8 = 8
@@ -17,40 +23,172 @@ def sizes(model):
                 RSN=h.nring_sn, RVE=h.nring_ve)
 
 
+# ----------------------------------------------------------------- 2-D mode
 def ext_elevation(s, real=R8):
-    """k_ext_elevation: EXTEL_EDGE gather + elevation update, one launch."""
+    """EXTEL_EDGE gather + elevation update (one RK stage)."""
     M, N, NCV = s["M"], s["N"], s["NCV"]
-    reads = (N * 3 * real                 # D1, UA, VA of the stage input
-             + NCV * (I4 + 2 * real)      # NTRG, DLTXE, DLTYE
-             + (M + 2) * I4 + 2 * NCV * I4  # node->sub-edge CSR
-             + M * (3 * real + 2 * I4))   # ELRK, ART1, H; node OBC / river flags
-    writes = M * 2 * real                 # ELF, D
+    reads = (M * (3 * I4 + 3 * real)          # cv_cnt, node_obc, node_riv; ELRK, ART1, H
+             + 2 * NCV * (I4 + 2 * real)      # slot-major sub-edge lists: element, signed DLTXE/DLTYE
+             + N * 3 * real)                  # D1, UA, VA of the stage input
+    writes = M * 2 * real                     # EL, D of the stage output
+    return reads + writes
+
+
+def advave_edge(s, real=R8):
+    """ADVAVE_EDGE_GCN edge terms, one thread per edge."""
+    M, N, NE = s["M"], s["N"], s["NE"]
+    reads = (NE * (4 * I4 + 4 * real)         # IEC, IENODE; XIJC, YIJC, DLTXC, DLTYC
+             + N * (3 * I4 + 8 * real + 4 * real + 2 * real)  # NBE; A1U, A2U; XC, YC, ART, CC_HVC; UA, VA
+             + M * 2 * real)                  # D, EL
+    writes = NE * (4 * real + real)           # EdgeFlux4 + ELIJ
     return reads + writes
 
 
 def ext_momentum(s, real=R8):
-    """k_ext_momentum: ADVAVE_EDGE_GCN gather + EXTUV_EDGE, one launch."""
+    """Edge-term gather + EXTUV_EDGE + wall projection (one RK stage)."""
     M, N, NE = s["M"], s["N"], s["NE"]
-    elem_reals = (3        # UA, VA, D1 stage input
-                  + 3      # UARK, VARK, ELRK1
-                  + 6      # ADX2D, ADY2D, DRX2D, DRY2D, WUBOT, WVBOT
-                  + 11     # ART, H1, COR, CC_HVC, CC_SPONGE, EPOR, GRAV_E, XC, YC, WSX, WSY
-                  + 8)     # A1U(4), A2U(4)
-    elem_ints = 3 + 3 + 3 + 2 + 1 + 1   # NV, NBE, edge list, ISBCE/IUCP, FO/river offsets
-    node_reals = 3         # ELF (new), EL, D (stage input)
-    edge = NE * (4 * I4 + I4 + 4 * real)  # IEC, IENODE, ISBC, XIJC, YIJC, DLTXC, DLTYC
-    writes = N * 4 * real  # UAF, VAF, EL1, D1
-    return N * (elem_reals * real + elem_ints * I4) + M * node_reals * real + edge + writes
+    elem_reals = 20   # EPOR GRAV_E D1 UA VA ART COR WSX WSY ADX2D ADY2D DRX2D DRY2D WUBOT WVBOT H1 EL1 UARK VARK CC_SPONGE
+    elem_ints = 9     # NV(3), edge list(3), IUCP, ISBCE, river offset
+    reads = (N * (elem_reals * real + elem_ints * I4)
+             + NE * (4 * real + real + I4 + 2 * real)  # EdgeFlux4, ELIJ, ISBC, DLTXC, DLTYC
+             + M * real)                               # new EL
+    writes = N * 4 * real                              # UA, VA, EL1, D1
+    return reads + writes
 
 
-def external_stage(s, real=R8):
-    return ext_elevation(s, real) + ext_momentum(s, real)
+# ----------------------------------------------------------------- 3-D mode
+def _cvrec(s, layers, nfields, real):
+    M, RSN = s["M"], s["RSN"]
+    reads = RSN * (I4 + 2 * real) + M * (2 * I4 + 2 * real) + nfields * M * layers * real
+    writes = M * layers * 8 * real
+    return reads + writes
 
 
-FAMILIES = {
-    "ext_elevation": ext_elevation,
-    "ext_momentum": ext_momentum,
-}
+def cvrec_ts(s, real=R8):
+    """Ring gradients/bounds + viscosity per node-layer (T or S, KBM1 layers)."""
+    return _cvrec(s, s["KBM1"], 3, real)       # f, mean, VISCOFH
+
+
+def cvrec_q(s, real=R8):
+    """The same for Q2 / Q2L on interfaces 2..KBM1."""
+    return _cvrec(s, s["KBM1"] - 1, 2, real)   # q, VISCOFH
+
+
+def _cvflux(s, rec_layers, uv_layers, real):
+    M, N, NCV = s["M"], s["N"], s["NCV"]
+    reads = (NCV * (3 * I4 + 6 * real)          # NIEC(2), NTRG; DLTXE, DLTYE, DLTXNCVE(2), DLTYNCVE(2)
+             + M * real + N * real              # NN_HVC, DT1
+             + M * rec_layers * 8 * real        # node-layer records
+             + N * uv_layers * 2 * real)        # U, V
+    writes = NCV * rec_layers * real            # face flux
+    return reads + writes
+
+
+def cvflux_ts(s, real=R8):
+    """ADV_T/ADV_S face flux per sub-edge and layer."""
+    return _cvflux(s, s["KBM1"], s["KBM1"], real)
+
+
+def cvflux_q(s, real=R8):
+    """ADV_Q face flux per sub-edge and interface."""
+    return _cvflux(s, s["KBM1"] - 1, s["KBM1"], real)
+
+
+def adv_ts(s, real=R8):
+    """Ordered sub-edge sums + vertical flux + update (T or S)."""
+    M, NCV, KB, KBM1 = s["M"], s["NCV"], s["KB"], s["KBM1"]
+    reads = (M * (3 * I4 + 4 * real) + 2 * NCV * I4   # counts, flags; ART1, DT, DTFA, NN_HVC; slot lists
+             + NCV * KBM1 * real                       # face fluxes
+             + M * KBM1 * real + M * KB * real         # T1, WTS
+             + M * (real + 8 * real))                  # bottom VISCOFH and record
+    writes = M * KBM1 * real + 3 * M * real            # TF1; PFPXB, PFPYB, AH_BOTTOM
+    return reads + writes
+
+
+def adv_q(s, real=R8):
+    """Ordered sub-edge sums + vertical flux + update (Q2 or Q2L)."""
+    M, NCV, KB, KBM1 = s["M"], s["NCV"], s["KB"], s["KBM1"]
+    reads = M * (I4 + 3 * real) + 2 * NCV * I4 + NCV * (KBM1 - 1) * real + 2 * M * KB * real
+    writes = M * KB * real
+    return reads + writes
+
+
+def _uvedge(s, real, visc):
+    M, N, NE, KBM1 = s["M"], s["N"], s["NE"], s["KBM1"]
+    reads = (NE * (5 * I4 + 4 * real)                  # IEC, IENODE, ISBC; XIJC, YIJC, DLTXC, DLTYC
+             + N * (3 * I4 + 12 * real)                # NBE; A1U, A2U, XC, YC, ART, CC_HVC
+             + M * real                                # DT
+             + N * KBM1 * 2 * real)                    # U, V
+    writes = NE * KBM1 * (4 * real + (real if visc else 0))
+    return reads + writes
+
+
+def uvedge_adv(s, real=R8):
+    """ADVECTION_EDGE_GCN edge terms per edge and layer."""
+    return _uvedge(s, real, False)
+
+
+def uvedge_uv(s, real=R8):
+    """ADV_UV_EDGE_GCN edge terms (and edge viscosity) per edge and layer."""
+    return _uvedge(s, real, True)
+
+
+def adv3d(s, real=R8):
+    """Edge-term gather summed over the column into ADX2D/ADY2D."""
+    N, NE, KBM1 = s["N"], s["NE"], s["KBM1"]
+    return N * (5 * I4 + real) + NE * I4 + NE * KBM1 * 4 * real + 2 * N * real
+
+
+def advuv(s, real=R8):
+    """Edge-term gather + vertical advection + PG + Coriolis -> UF, VF, VISCOFM."""
+    M, N, NE, KB, KBM1 = s["M"], s["N"], s["NE"], s["KB"], s["KBM1"]
+    reads = (N * (5 * I4 + 7 * real) + NE * (3 * I4 + 2 * real) + M * real   # lists, constants, edge geometry, EGF
+             + NE * KBM1 * 5 * real                                          # EdgeFlux4 + viscosity
+             + N * KBM1 * 4 * real + N * KB * real)                          # U, V, DRHOX, DRHOY; W
+    writes = N * KBM1 * 3 * real + N * real
+    return reads + writes
+
+
+def vertvl(s, real=R8):
+    """VERTVL_EDGE: sub-edge divergence, column integral, correction."""
+    M, N, NCV, KB, KBM1 = s["M"], s["N"], s["NCV"], s["KB"], s["KBM1"]
+    reads = (2 * NCV * (I4 + 2 * real) + M * (3 * I4 + 6 * real)
+             + N * real + N * KBM1 * 2 * real)
+    writes = M * KB * real + M * real
+    return reads + writes
+
+
+def viscofh(s, real=R8):
+    """VISCOF_H: nodal strain rate from the element ring."""
+    M, N, KBM1, RVE = s["M"], s["N"], s["KBM1"], s["RVE"]
+    reads = RVE * (I4 + 4 * real) + M * (3 * I4 + real) + N * KBM1 * 2 * real
+    writes = M * KBM1 * real
+    return reads + writes
+
+
+def vdif_q(s, real=R8):
+    """VDIF_Q (MY2.5): column solves for Q2, Q2L and the mixing coefficients."""
+    M, N, KB, KBM1, RVE = s["M"], s["N"], s["KB"], s["KBM1"], s["RVE"]
+    reads = M * KB * 9 * real + N * KBM1 * 2 * real + RVE * I4 + M * 4 * real + N * 4 * real
+    writes = M * KB * 6 * real + M * KB * 2 * real    # Q2F Q2LF L KM KH KQ; PROD scratch (write + read)
+    return reads + writes
+
+
+def vdif_uv(s, real=R8):
+    """VDIF_UV: implicit vertical diffusion of UF and VF."""
+    N, KB, KBM1 = s["N"], s["KB"], s["KBM1"]
+    return N * KB * 3 * real + N * 6 * real + N * KBM1 * 2 * real + N * 3 * real
+
+
+def vdif_ts(s, real=R8):
+    """VDIF_TS: implicit vertical diffusion of one tracer."""
+    M, KB, KBM1 = s["M"], s["KB"], s["KBM1"]
+    return M * KB * 2 * real + M * 8 * real + M * KBM1 * real
+
+
+FAMILIES = {f.__name__: f for f in (
+    ext_elevation, advave_edge, ext_momentum, cvrec_ts, cvrec_q, cvflux_ts, cvflux_q, adv_ts, adv_q,
+    uvedge_adv, uvedge_uv, adv3d, advuv, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts)}
 
 
 def bytes_per_launch(name, model, real=R8):

@@ -1,0 +1,86 @@
+"""Batched instances as one mesh (fv_model_union): every instance of the
+disjoint union advances exactly as it does alone.  The CPU test runs the
+oracle on the union and on each part; the GPU test runs the HIP path on the
+union and compares it with the oracle bit for bit."""
+import numpy as np
+import pytest
+
+import myfvcomcode_amd as F
+
+FIELDS = ("el", "d", "ua", "va", "u", "v", "w", "t1", "s1", "q2", "q2l", "km", "kh", "wts")
+
+
+def parts():
+    # three instances of the ensemble-member generator at different sizes and
+    # seeds (few seamounts, so that the small domains stay shallow and stable)
+    return [F.Model.build(3, nx=21, ny=11, kb=6, seed=11, n_seamounts=1),
+            F.Model.build(3, nx=31, ny=13, kb=6, seed=12, n_seamounts=1),
+            F.Model.build(3, nx=17, ny=9, kb=6, seed=13, n_seamounts=0)]
+
+
+def instance_slice(a, union, off, kb):
+    om, on, m, n = off
+    if a.size == union.m + 1:
+        return a[om + 1: om + m + 1]
+    if a.size == union.n + 1:
+        return a[on + 1: on + n + 1]
+    if a.size == (union.m + 1) * kb:
+        return a.reshape(kb, -1)[:, om + 1: om + m + 1]
+    return a.reshape(kb, -1)[:, on + 1: on + n + 1]
+
+
+def own(a, part, kb):
+    if a.size in (part.m + 1, part.n + 1):
+        return a[1:]
+    return a.reshape(kb, -1)[:, 1:]
+
+
+def test_union_sizes(built):
+    ps = parts()
+    u, offs = F.Model.union(ps)
+    assert u.m == sum(p.m for p in ps) and u.n == sum(p.n for p in ps)
+    assert u.hdr.ne == sum(p.hdr.ne for p in ps) and u.hdr.ncv == sum(p.hdr.ncv for p in ps)
+    assert u.hdr.iobcn == sum(p.hdr.iobcn for p in ps)
+    # renumbered topology: every triangle of instance p uses instance p's nodes
+    nv = u.field("nv").reshape(3, -1)
+    for (om, on, m, n) in offs:
+        t = nv[:, on + 1: on + n + 1]
+        assert t.min() == om + 1 and t.max() == om + m
+
+
+def test_union_rejects_mismatched_parts(built):
+    a = F.Model.build(3, nx=17, ny=9, kb=6, n_seamounts=0)
+    b = F.Model.build(3, nx=17, ny=9, kb=7, n_seamounts=0)
+    with pytest.raises(RuntimeError):
+        F.Model.union([a, b])
+
+
+def test_union_instances_evolve_alone_oracle(built):
+    from oracle import Oracle
+    ps = parts()
+    u, offs = F.Model.union(ps)
+    ou = Oracle(u)
+    ou.run(3)
+    for p, off in zip(ps, offs):
+        op = Oracle(p)
+        op.run(3)
+        for name in FIELDS:
+            a = instance_slice(ou.get(name), u, off, u.kb)
+            b = own(op.get(name), p, u.kb)
+            assert np.isfinite(b).all(), name
+            assert np.array_equal(a, b), (name, off)
+
+
+@pytest.mark.gpu
+def test_union_gpu_parity(built):
+    from oracle import Oracle
+    ps = parts()
+    u, offs = F.Model.union(ps)
+    print("case: union of 3 ensemble-generator instances, 4 steps (graph)", flush=True)
+    dev = F.DeviceModel(u)
+    dev.run(4, graph=True)
+    orc = Oracle(u)
+    orc.run(4)
+    for name in FIELDS:
+        a, b = dev.get(name), orc.get(name)
+        assert np.array_equal(a, b), (name, float(np.abs(a - b).max()))
