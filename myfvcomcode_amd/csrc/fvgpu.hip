@@ -171,7 +171,8 @@ ExtArgs make_ext_args(fvgpu_handle* h, int K, const Ext2DState& cur, const Ext2D
   return a;
 }
 
-void enqueue_external_step(fvgpu_handle* h, int iext) {
+template <typename FT>
+void enqueue_external_step_t(fvgpu_handle* h, int iext) {
   const DevGrid& g = h->g;
   LAUNCH("ext_clock", k_ext_clock, 1, 64, h->clk, iext);
   // stage buffers: 1: A->B, 2: B->C, 3: C->B, 4: B->A (in place on the RK base)
@@ -180,34 +181,35 @@ void enqueue_external_step(fvgpu_handle* h, int iext) {
   for (int K = 1; K <= 4; ++K) {
     ExtArgs a = make_ext_args(h, K, *cur[K], *nxt[K]);
     LAUNCH("ext_elevation", k_ext_elevation, nb(g.M + 1), 256, g, a);
-    LAUNCH("advave_edge", k_advave_edge, nb(g.NE + 1), 256, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
-    LAUNCH("ext_momentum", k_ext_momentum, nb(g.N + 1), 256, g, a);
+    LAUNCH("advave_edge", k_advave_edge<FT>, nb(g.NE + 1), 256, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
+    LAUNCH("ext_momentum", k_ext_momentum<FT>, nb(g.N + 1), 256, g, a);
   }
 }
 
-void enqueue_internal_step(fvgpu_handle* h) {
+template <typename FT>
+void enqueue_internal_step_t(fvgpu_handle* h) {
   const DevGrid& g = h->g;
   IntArgs& a = h->ia;
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
   LAUNCH("adjust1_brough", k_adjust1_brough, nb(g.N + 1), 256, g, a);
-  LAUNCH("uvedge_adv", k_uvedge<0>, nb(g.NE + 1), 256, g, a, a.u, a.v);
-  LAUNCH("adv3d", k_adv3d_gather, nb(g.N + 1), 256, g, a);
+  LAUNCH("uvedge_adv", (k_uvedge<0, FT>), nb(g.NE + 1), 256, g, a, a.u, a.v);
+  LAUNCH("adv3d", k_adv3d_gather<FT>, nb(g.N + 1), 256, g, a);
   LAUNCH("baropg", k_baropg, nb(g.N + 1), 256, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) LAUNCH("rho1_roundtrip", k_rho1_roundtrip, nb(g.M + 1), 256, g, a);
-  LAUNCH("advave_edge", k_advave_edge, nb(g.NE + 1), 256, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
-  LAUNCH("advave_int", k_advave_int, nb(g.N + 1), 256, g, a);
+  LAUNCH("advave_edge", k_advave_edge<FT>, nb(g.NE + 1), 256, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
+  LAUNCH("advave_int", k_advave_int<FT>, nb(g.N + 1), 256, g, a);
   LAUNCH("zero", k_zero, nb(g.m1), 256, a.egf, g.m1);
   if (g.IOBCN > 0) LAUNCH("zero", k_zero, nb(g.IOBCN), 256, a.uard_obcn, (long)g.IOBCN);
-  for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step(h, iext);
+  for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext);
   LAUNCH("adjust2", k_adjust2, nb(g.N + 1), 256, g, a);
 #define VERTVL_CALL(KM) LAUNCH("vertvl", k_vertvl_r<KM>, nb(g.M + 1), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VERTVL_CALL);
   LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) LAUNCH("viscofh", k_viscofh_r<RM>, nb(g.M + 1), 256, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
-  LAUNCH("uvedge_uv", k_uvedge<1>, nb(g.NE + 1), 256, g, a, a.u, a.v);
-  LAUNCH("advuv", k_advuv_gather, nb(g.N + 1), 256, g, a);
+  LAUNCH("uvedge_uv", (k_uvedge<1, FT>), nb(g.NE + 1), 256, g, a, a.u, a.v);
+  LAUNCH("advuv", k_advuv_gather<FT>, nb(g.N + 1), 256, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   LAUNCH("uv_finish", k_uv_finish, nb((long)g.N * g.KBM1), 256, g, a);
@@ -218,10 +220,10 @@ void enqueue_internal_step(fvgpu_handle* h) {
     for (int which = 0; which < 2; ++which) {
       double* q = which ? a.q2l : a.q2;
       double* qf = which ? a.q2lf : a.q2f;
-#define CVREC_Q_CALL(RM) LAUNCH("cvrec_q", (k_cvrec<1, RM>), nb(g.M + 1), 256, g, a, q, nullptr, qmean1, 2, g.KBM1)
+#define CVREC_Q_CALL(RM) LAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), nb(g.M + 1), 256, g, a, q, nullptr, qmean1, 2, g.KBM1)
       FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
-      LAUNCH("cvflux_q", k_cvflux_q, nb(g.NCV + 1), 256, g, a);
-#define ADV_Q_CALL(SM) LAUNCH("adv_q", k_adv_q_gather<SM>, nb(g.M + 1), 256, g, a, q, qf, fct)
+      LAUNCH("cvflux_q", k_cvflux_q<FT>, nb(g.NCV + 1), 256, g, a);
+#define ADV_Q_CALL(SM) LAUNCH("adv_q", (k_adv_q_gather<SM, FT>), nb(g.M + 1), 256, g, a, q, qf, fct)
       FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
     }
 #define VDIF_Q_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx)
@@ -239,12 +241,12 @@ void enqueue_internal_step(fvgpu_handle* h) {
     double* ff = which == 1 ? a.tf1 : a.sf1;
     const double* fm = which == 1 ? a.tmean1 : a.smean1;
     const double* fdis = which == 1 ? g.tdis : g.sdis;
-#define CVREC_TS_CALL(RM) LAUNCH("cvrec_ts", (k_cvrec<0, RM>), nb(g.M + 1), 256, g, a, f1, fm, 0.0, 1, g.KBM1)
+#define CVREC_TS_CALL(RM) LAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), nb(g.M + 1), 256, g, a, f1, fm, 0.0, 1, g.KBM1)
     FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
-    LAUNCH("cvflux_ts", k_cvflux_ts, nb(g.NCV + 1), 256, g, a);
-#define ADV_TS_CALL(SM) LAUNCH("adv_ts", k_adv_ts_gather<SM>, nb(g.M + 1), 256, g, a, f1, ff, fdis, which, fct)
+    LAUNCH("cvflux_ts", k_cvflux_ts<FT>, nb(g.NCV + 1), 256, g, a);
+#define ADV_TS_CALL(SM) LAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), nb(g.M + 1), 256, g, a, f1, ff, fdis, which, fct)
     FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
-    if (g.IOBCN > 0) LAUNCH("adv_ts_obc", k_adv_ts_obc, nb((long)g.IOBCN * g.KBM1), 256, g, a);
+    if (g.IOBCN > 0) LAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, nb((long)g.IOBCN * g.KBM1), 256, g, a);
 #define VDIF_TS_CALL(KM) LAUNCH("vdif_ts", k_vdif_ts_r<KM>, nb(g.M + 1), 256, g, a, ff, which)
     FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
     LAUNCH("bcond_ts", k_bcond_ts, nb(g.IOBCN + 1), 256, g, a, which);
@@ -258,6 +260,16 @@ void enqueue_internal_step(fvgpu_handle* h) {
   LAUNCH("copy", k_copy, nb(nk), 256, a.u, a.uf, nk);
   LAUNCH("copy", k_copy, nb(nk), 256, a.v, a.vf, nk);
   LAUNCH("end_2d", k_end_2d, nb(std::max(g.M, g.N) + 1), 256, g, a);
+}
+
+void enqueue_external_step(fvgpu_handle* h, int iext) {
+  if (h->precision == FV_MIXED) enqueue_external_step_t<float>(h, iext);
+  else enqueue_external_step_t<double>(h, iext);
+}
+
+void enqueue_internal_step(fvgpu_handle* h) {
+  if (h->precision == FV_MIXED) enqueue_internal_step_t<float>(h);
+  else enqueue_internal_step_t<double>(h);
 }
 
 // ------------------------------------------------------ derived structures
@@ -484,13 +496,19 @@ int upload_state(fvgpu_handle* h) {
   RW(l, md->l, mk); RW(km, md->km, mk); RW(kh, md->kh, mk); RW(kq, md->kq, mk); RW(wts, md->wts, mk);
   RW(viscofh, nullptr, mk);
   RW(gx, nullptr, mk);
-  if (dalloc(h, &a.ef2, (size_t)g.NE + 1)) return 1;
+  // flux buffers sized for FP64 (the mixed form uses the first half)
+  {
+    EdgeFlux4T<double>* p2 = nullptr; EdgeFlux4T<double>* p3 = nullptr; CvRecT<double>* pr = nullptr;
+    double *pv = nullptr, *pc = nullptr;
+    if (dalloc(h, &p2, (size_t)g.NE + 1)) return 1;
+    if (dalloc(h, &p3, ((size_t)g.NE + 1) * g.KB)) return 1;
+    if (dalloc(h, &pv, ((size_t)g.NE + 1) * g.KB)) return 1;
+    if (dalloc(h, &pr, mk)) return 1;
+    if (dalloc(h, &pc, ((size_t)g.NCV + 1) * g.KB)) return 1;
+    a.ef2 = p2; a.ef3 = p3; a.efvisc = pv; a.rec = pr; a.cvflux = pc;
+  }
   RW(elij2, nullptr, (size_t)g.NE + 1);
-  if (dalloc(h, &a.ef3, ((size_t)g.NE + 1) * g.KB)) return 1;
-  RW(efvisc, nullptr, ((size_t)g.NE + 1) * g.KB);
-  if (dalloc(h, &a.rec, mk)) return 1;
   if (dalloc(h, &a.rlohi, mk)) return 1;
-  RW(cvflux, nullptr, ((size_t)g.NCV + 1) * g.KB);
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
   RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
   RW(uard, nullptr, n1); RW(vard, nullptr, n1); RW(fluxobn, nullptr, n1);
@@ -555,7 +573,7 @@ int fvgpu_device_count(int32_t* n) {
 
 int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   if (!md || !out) return fail("null argument");
-  if (precision != FV_FP64) return fail("only the FP64 form is available in this build");
+  if (precision != FV_FP64 && precision != FV_MIXED) return fail("unknown precision mode");
   if (md->kb > FV_MAXKB) return fail("KB too large");
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));

@@ -99,7 +99,9 @@ __device__ __forceinline__ double SIGE_(const DevGrid& g, const double* a, int i
 #define DLTYNCVE_(c, j) g.dltyncve[(long)((j)-1) * (g.NCV + 1) + (c)]
 
 __device__ __forceinline__ double fsign1(double b) { return b >= 0.0 ? 1.0 : -1.0; }
+__device__ __forceinline__ float fsign1(float b) { return b >= 0.0f ? 1.0f : -1.0f; }
 __device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ float sq(float x) { return x * x; }
 
 // Device clock: the time-dependent scalars of the current step, written by
 // tiny kernels so that a captured graph replays correctly.

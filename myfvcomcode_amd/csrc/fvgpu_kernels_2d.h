@@ -29,8 +29,9 @@ struct Ext2DState {      // one buffer set of the 2-D state
 // Per edge (and layer) momentum-flux terms of the edge loops of
 // ADVAVE_EDGE_GCN / ADVECTION_EDGE_GCN / ADV_UV_EDGE_GCN, written once per
 // edge by an edge kernel and gathered by the two adjacent elements.
-struct __align__(16) EdgeFlux4 {
-  double xadv, yadv, fx, fy;
+template <typename T>
+struct __align__(16) EdgeFlux4T {
+  T xadv, yadv, fx, fy;
 };
 
 struct ExtArgs {
@@ -42,7 +43,7 @@ struct ExtArgs {
   double *xflux_obcn, *fobn, *uard_obcn, *egf, *fluxobn, *uard, *vard;
   const double *adx2d, *ady2d, *drx2d, *dry2d, *wubot, *wvbot;
   const double *qdis;    // QDIS (internal-mode river discharge)
-  EdgeFlux4* ef;         // per-edge ADVAVE terms of the stage input
+  void* ef;              // per-edge ADVAVE terms of the stage input, EdgeFlux4T<FT>
   double* elij;          // per-edge mean elevation of the stage input
   DevClock* clk;
 };
@@ -130,75 +131,87 @@ __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
 // the state (ua, va, d, el), one thread per edge: least-squares velocity
 // gradients on both sides, face velocities, upwind advective flux, the
 // Smagorinsky stresses; the side-dependent factors (EPOR, IUCP, ISBC, the
-// pressure term) are applied by the gathering element.
-__global__ __launch_bounds__(256) void k_advave_edge(DevGrid g, const double* ua, const double* va,
+// pressure term) are applied by the gathering element.  Arithmetic in the
+// flux type FT (FP64: the reference's expressions bit for bit).
+template <typename FT>
+__global__ __launch_bounds__(256) void k_advave_edge(DevGrid g, const double* ua_, const double* va_,
                                                      const double* d, const double* el,
-                                                     EdgeFlux4* ef, double* elij_out) {
+                                                     void* ef_, double* elij_out) {
   const int e = (int)xcd_thread();
   if (e < 1 || e > g.NE) return;
+  EdgeFlux4T<FT>* ef = (EdgeFlux4T<FT>*)ef_;
+  const FT half = (FT)0.5, one = (FT)1.0;
   const int ia = IEC_(e, 1), ib = IEC_(e, 2), j1 = IENODE_(e, 1), j2 = IENODE_(e, 2);
-  const double dij = 0.5 * (d[j1] + d[j2]);
+  const FT dij = half * ((FT)d[j1] + (FT)d[j2]);
   const double elij = 0.5 * (el[j1] + el[j2]);
+  auto U = [&](int i) { return (FT)ua_[i]; };
+  auto V = [&](int i) { return (FT)va_[i]; };
+  auto A1 = [&](int i, int j) { return (FT)A1U_(i, j); };
+  auto A2 = [&](int i, int j) { return (FT)A2U_(i, j); };
   int k1 = NBE_(ia, 1), k2 = NBE_(ia, 2), k3 = NBE_(ia, 3);
-  const double cofa1 = A1U_(ia, 1) * ua[ia] + A1U_(ia, 2) * ua[k1] + A1U_(ia, 3) * ua[k2] + A1U_(ia, 4) * ua[k3];
-  const double cofa2 = A2U_(ia, 1) * ua[ia] + A2U_(ia, 2) * ua[k1] + A2U_(ia, 3) * ua[k2] + A2U_(ia, 4) * ua[k3];
-  const double cofa5 = A1U_(ia, 1) * va[ia] + A1U_(ia, 2) * va[k1] + A1U_(ia, 3) * va[k2] + A1U_(ia, 4) * va[k3];
-  const double cofa6 = A2U_(ia, 1) * va[ia] + A2U_(ia, 2) * va[k1] + A2U_(ia, 3) * va[k2] + A2U_(ia, 4) * va[k3];
-  double xij = g.xijc[e] - g.xc[ia], yij = g.yijc[e] - g.yc[ia];
-  const double uij1 = ua[ia] + cofa1 * xij + cofa2 * yij;
-  const double vij1 = va[ia] + cofa5 * xij + cofa6 * yij;
+  const FT cofa1 = A1(ia, 1) * U(ia) + A1(ia, 2) * U(k1) + A1(ia, 3) * U(k2) + A1(ia, 4) * U(k3);
+  const FT cofa2 = A2(ia, 1) * U(ia) + A2(ia, 2) * U(k1) + A2(ia, 3) * U(k2) + A2(ia, 4) * U(k3);
+  const FT cofa5 = A1(ia, 1) * V(ia) + A1(ia, 2) * V(k1) + A1(ia, 3) * V(k2) + A1(ia, 4) * V(k3);
+  const FT cofa6 = A2(ia, 1) * V(ia) + A2(ia, 2) * V(k1) + A2(ia, 3) * V(k2) + A2(ia, 4) * V(k3);
+  FT xij = (FT)(g.xijc[e] - g.xc[ia]), yij = (FT)(g.yijc[e] - g.yc[ia]);
+  const FT uij1 = U(ia) + cofa1 * xij + cofa2 * yij;
+  const FT vij1 = V(ia) + cofa5 * xij + cofa6 * yij;
   k1 = NBE_(ib, 1); k2 = NBE_(ib, 2); k3 = NBE_(ib, 3);
-  const double cofa3 = A1U_(ib, 1) * ua[ib] + A1U_(ib, 2) * ua[k1] + A1U_(ib, 3) * ua[k2] + A1U_(ib, 4) * ua[k3];
-  const double cofa4 = A2U_(ib, 1) * ua[ib] + A2U_(ib, 2) * ua[k1] + A2U_(ib, 3) * ua[k2] + A2U_(ib, 4) * ua[k3];
-  const double cofa7 = A1U_(ib, 1) * va[ib] + A1U_(ib, 2) * va[k1] + A1U_(ib, 3) * va[k2] + A1U_(ib, 4) * va[k3];
-  const double cofa8 = A2U_(ib, 1) * va[ib] + A2U_(ib, 2) * va[k1] + A2U_(ib, 3) * va[k2] + A2U_(ib, 4) * va[k3];
-  xij = g.xijc[e] - g.xc[ib]; yij = g.yijc[e] - g.yc[ib];
-  const double uij2 = ua[ib] + cofa3 * xij + cofa4 * yij;
-  const double vij2 = va[ib] + cofa7 * xij + cofa8 * yij;
-  const double uij = 0.5 * (uij1 + uij2), vij = 0.5 * (vij1 + vij2);
-  const double dlx = g.dltxc[e], dly = g.dltyc[e];
-  const double un_tmp = -uij * dly + vij * dlx;
-  const double viscof1 = g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + 0.5 * sq(cofa2 + cofa5));
-  const double viscof2 = g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + 0.5 * sq(cofa4 + cofa7));
-  const double viscof = (g.fact * 0.5 * (viscof1 * g.cc_hvc[ia] + viscof2 * g.cc_hvc[ib]) +
-                         g.fm1 * 0.5 * (g.cc_hvc[ia] + g.cc_hvc[ib])) / g.hprnu;
-  const double txxij = (cofa1 + cofa3) * viscof;
-  const double tyyij = (cofa6 + cofa8) * viscof;
-  const double txyij = 0.5 * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
-  EdgeFlux4 r;
+  const FT cofa3 = A1(ib, 1) * U(ib) + A1(ib, 2) * U(k1) + A1(ib, 3) * U(k2) + A1(ib, 4) * U(k3);
+  const FT cofa4 = A2(ib, 1) * U(ib) + A2(ib, 2) * U(k1) + A2(ib, 3) * U(k2) + A2(ib, 4) * U(k3);
+  const FT cofa7 = A1(ib, 1) * V(ib) + A1(ib, 2) * V(k1) + A1(ib, 3) * V(k2) + A1(ib, 4) * V(k3);
+  const FT cofa8 = A2(ib, 1) * V(ib) + A2(ib, 2) * V(k1) + A2(ib, 3) * V(k2) + A2(ib, 4) * V(k3);
+  xij = (FT)(g.xijc[e] - g.xc[ib]); yij = (FT)(g.yijc[e] - g.yc[ib]);
+  const FT uij2 = U(ib) + cofa3 * xij + cofa4 * yij;
+  const FT vij2 = V(ib) + cofa7 * xij + cofa8 * yij;
+  const FT uij = half * (uij1 + uij2), vij = half * (vij1 + vij2);
+  const FT dlx = (FT)g.dltxc[e], dly = (FT)g.dltyc[e];
+  const FT un_tmp = -uij * dly + vij * dlx;
+  const FT viscof1 = (FT)g.art[ia] * sqrt(sq(cofa1) + sq(cofa6) + half * sq(cofa2 + cofa5));
+  const FT viscof2 = (FT)g.art[ib] * sqrt(sq(cofa3) + sq(cofa8) + half * sq(cofa4 + cofa7));
+  const FT cca = (FT)g.cc_hvc[ia], ccb = (FT)g.cc_hvc[ib];
+  const FT viscof = ((FT)g.fact * half * (viscof1 * cca + viscof2 * ccb) +
+                     (FT)g.fm1 * half * (cca + ccb)) / (FT)g.hprnu;
+  const FT txxij = (cofa1 + cofa3) * viscof;
+  const FT tyyij = (cofa6 + cofa8) * viscof;
+  const FT txyij = half * (cofa2 + cofa4 + cofa5 + cofa7) * viscof;
+  EdgeFlux4T<FT> r;
   r.fx = dij * (txxij * dly - txyij * dlx);
   r.fy = dij * (txyij * dly - tyyij * dlx);
-  const double sgn = fsign1(un_tmp);
-  r.xadv = dij * un_tmp * ((1.0 - sgn) * uij2 + (1.0 + sgn) * uij1) * 0.5;
-  r.yadv = dij * un_tmp * ((1.0 - sgn) * vij2 + (1.0 + sgn) * vij1) * 0.5;
+  const FT sgn = fsign1(un_tmp);
+  r.xadv = dij * un_tmp * ((one - sgn) * uij2 + (one + sgn) * uij1) * half;
+  r.yadv = dij * un_tmp * ((one - sgn) * vij2 + (one + sgn) * vij1) * half;
   ef[e] = r;
   elij_out[e] = elij;
 }
 
 // ADVAVE_EDGE_GCN for element i: its three edges' terms gathered in edge
 // order (the reference's scatter order), OBC flux and rivers
-// (advave_edge_gcn.F:746-992).
-__device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const EdgeFlux4* ef, const double* elij,
+// (advave_edge_gcn.F:746-992).  The element sums are FP64.
+template <typename FT>
+__device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const void* ef_, const double* elij,
                                               const double* ua, const double* va, const double* d1,
                                               const double* fluxobn, const double* qdis,
                                               double& xf, double& yf, double& px, double& py) {
+  const EdgeFlux4T<FT>* ef = (const EdgeFlux4T<FT>*)ef_;
   xf = 0.0; yf = 0.0; px = 0.0; py = 0.0;
   const double ep = g.epor[i], iu = (double)g.iucp[i], gdi = g.grav_e[i] * d1[i];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int ent = g.ed_ent[3 * i + q], e = ent >> 1;
-    const EdgeFlux4 r = ef[e];
+    const EdgeFlux4T<FT> r = ef[e];
+    const double xadv = (double)r.xadv, yadv = (double)r.yadv, fx = (double)r.fx, fy = (double)r.fy;
     const double isbc_tmp = (double)g.isbc[e];
     const double gd = gdi * elij[e];
     const double dlx = g.dltxc[e], dly = g.dltyc[e];
     if ((ent & 1) == 0) {
-      xf = xf + (r.xadv + r.fx * ep) * (1.0 - isbc_tmp) * iu;
-      yf = yf + (r.yadv + r.fy * ep) * (1.0 - isbc_tmp) * iu;
+      xf = xf + (xadv + fx * ep) * (1.0 - isbc_tmp) * iu;
+      yf = yf + (yadv + fy * ep) * (1.0 - isbc_tmp) * iu;
       px = px - gd * dly;
       py = py + gd * dlx;
     } else {
-      xf = xf - (r.xadv + r.fx * ep) * (1.0 - isbc_tmp) * iu;
-      yf = yf - (r.yadv + r.fy * ep) * (1.0 - isbc_tmp) * iu;
+      xf = xf - (xadv + fx * ep) * (1.0 - isbc_tmp) * iu;
+      yf = yf - (yadv + fy * ep) * (1.0 - isbc_tmp) * iu;
       px = px + gd * dly;
       py = py - gd * dlx;
     }
@@ -215,6 +228,7 @@ __device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const Edg
   }
 }
 
+template <typename FT>
 __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
   const int i = (int)xcd_thread();
   if (i < 1 || i > g.N) return;
@@ -230,7 +244,7 @@ __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
     a.fluxobn[i] = f;
   }
   double xf, yf, px, py;
-  advave_gather(g, i, a.ef, a.elij, a.cur.ua, a.cur.va, a.cur.d1, a.fluxobn, a.qdis, xf, yf, px, py);
+  advave_gather<FT>(g, i, a.ef, a.elij, a.cur.ua, a.cur.va, a.cur.d1, a.fluxobn, a.qdis, xf, yf, px, py);
   // EXTUV_EDGE
   const double ua = a.cur.ua[i], va = a.cur.va[i], d1 = a.cur.d1[i], art = g.art[i], cor = g.cor[i];
   const double ramp = a.clk->ramp_ext;

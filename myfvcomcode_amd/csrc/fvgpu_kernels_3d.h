@@ -8,11 +8,13 @@
 #include "fvgpu_kernels_2d.h"
 
 // Per node-layer inputs of a control-volume face flux, packed so that one
-// gather of a neighbour node reads one 64-byte record: the field, its ring
-// gradient (adv_t.F:219-376), ring bounds (adv_t.F:469-480), the gradient of
-// the anomaly, and the node's horizontal viscosity times NN_HVC.
-struct __align__(16) CvRec {
-  double f, gx, gy, fmin, fmax, gxd, gyd, vhn;
+// gather of a neighbour node reads one record (64 bytes in FP64, 32 in the
+// mixed form's FP32): the field, its ring gradient (adv_t.F:219-376), ring
+// bounds (adv_t.F:469-480), the gradient of the anomaly, and the node's
+// horizontal viscosity times NN_HVC.
+template <typename T>
+struct __align__(16) CvRecT {
+  T f, gx, gy, fmin, fmax, gxd, gyd, vhn;
 };
 
 struct IntArgs {
@@ -23,13 +25,14 @@ struct IntArgs {
   double *l, *km, *kh, *kq, *wts, *viscofh;
   // node-layer scratch: ring gradients and bounds
   double* gx;            // node-layer scratch (PROD of VDIF_Q)
-  EdgeFlux4* ef2;        // per-edge ADVAVE terms (2-D)
+  // flux buffers, of the flux type FT (double, or float in the mixed form)
+  void* ef2;             // per-edge ADVAVE terms (2-D), EdgeFlux4T<FT>
   double* elij2;         // per-edge mean elevation (2-D)
-  EdgeFlux4* ef3;        // per edge-layer momentum terms (3-D), (NE+1) x KB
-  double* efvisc;        // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB
-  CvRec* rec;            // node-layer records of the field being advected
+  void* ef3;             // per edge-layer momentum terms (3-D), (NE+1) x KB EdgeFlux4T<FT>
+  void* efvisc;          // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB FT
+  void* rec;             // node-layer records of the field being advected, CvRecT<FT>
   double2* rlohi;        // node-layer min/max of the field over the NBSN ring (FCT bounds)
-  double* cvflux;        // per sub-edge, per layer face flux (NCV+1) x KB
+  void* cvflux;          // per sub-edge, per layer face flux (NCV+1) x KB FT
   // 2-D
   double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;
   double *el, *d, *et, *dt, *egf, *dtfa, *elf;          // nodes (elf = EL after the ext. loop)
@@ -171,6 +174,7 @@ __global__ void k_rho1_roundtrip(DevGrid g, IntArgs a) {
 
 // Ext-mode ADVAVE of INTERNAL_STEP and GX/GY (internal_step.F:387-392),
 // plus the zeroing of UARD/VARD (internal_step.F:395-397).
+template <typename FT>
 __global__ void k_advave_int(DevGrid g, IntArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > g.N) return;
@@ -178,7 +182,7 @@ __global__ void k_advave_int(DevGrid g, IntArgs a) {
   a.vard[i] = 0.0;
   if (i < 1) return;
   double xf, yf, px, py;
-  advave_gather(g, i, a.ef2, a.elij2, a.ua, a.va, a.d1, a.fluxobn, a.qdis, xf, yf, px, py);
+  advave_gather<FT>(g, i, a.ef2, a.elij2, a.ua, a.va, a.d1, a.fluxobn, a.qdis, xf, yf, px, py);
   a.adx2d[i] = a.adx2d[i] - xf;
   a.ady2d[i] = a.ady2d[i] - yf;
 }

@@ -22,14 +22,16 @@
 // Ring gradient / bounds of field f at node i for the layers [k0, k1]
 // (adv_t.F:219-376 and :469-480; adv_q.F the same on layer interfaces),
 // and the ring-only min/max that FCT_T/FCT_S/FCT_Q2 clip to
-// (fct_t.F:74-80).  RMAX >= NTSN bounds the ring held in registers.
+// (fct_t.F:74-80).  RMAX >= NTSN bounds the ring held in registers.  The
+// sums are FP64; the record is stored in the flux type FT.
 // kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
 // DZ-weighted interface viscosity (adv_q.F) times NN_HVC.
-template <int KIND, int RMAX>
+template <int KIND, int RMAX, typename FT>
 __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const double* f, const double* mean,
                                                double cmean, int k0, int k1) {
   const int i = (int)xcd_thread();
   if (i < 1 || i > g.M) return;
+  CvRecT<FT>* rec = (CvRecT<FT>*)a.rec;
   const int o = g.nbsn_off[i], nt = g.ntsn[i];
   int rn[RMAX];
   double rdx[RMAX], rdy[RMAX];
@@ -71,101 +73,104 @@ __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const doubl
                          (DZ(i, k) + DZ(i, k - 1));
       vhn = vfa * nn;
     }
-    CvRec r;
-    r.f = c;
-    r.gx = s1 / art2;
-    r.gy = s2 / art2;
-    r.fmin = fmin(lo, c);
-    r.fmax = fmax(hi, c);
-    r.gxd = s3 / art2;
-    r.gyd = s4 / art2;
-    r.vhn = vhn;
-    a.rec[ko + i] = r;
-    // the ring alone (entry NTSN repeats entry 1): the FCT bounds
+    CvRecT<FT> r;
+    r.f = (FT)c;
+    r.gx = (FT)(s1 / art2);
+    r.gy = (FT)(s2 / art2);
+    r.fmin = (FT)fmin(lo, c);
+    r.fmax = (FT)fmax(hi, c);
+    r.gxd = (FT)(s3 / art2);
+    r.gyd = (FT)(s4 / art2);
+    r.vhn = (FT)vhn;
+    rec[ko + i] = r;
+    // the ring alone (entry NTSN repeats entry 1): the FCT bounds, FP64
     a.rlohi[ko + i] = make_double2(lo, hi);
   }
 }
 
-// Face flux of ADV_T/ADV_S through sub-edge c at layer k (adv_t.F:440-500).
-struct TsFace { double exflux, fxx, fyy; };
-__device__ __forceinline__ TsFace ts_face(const DevGrid& g, const IntArgs& a, int c, int k) {
-  const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
-  const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
-  const double dltx = g.dltxe[c], dlty = g.dltye[c];
-  const double dtij = a.dt1[i1] * DZ1(i1, k);
-  const double un = a.v[eo + i1] * dltx - a.u[eo + i1] * dlty;
-  const CvRec ra = a.rec[ko + ia], rb = a.rec[ko + ib];
-  double fij1 = ra.f + DLTXNCVE_(c, 1) * ra.gx + DLTYNCVE_(c, 1) * ra.gy;
-  double fij2 = rb.f + DLTXNCVE_(c, 2) * rb.gx + DLTYNCVE_(c, 2) * rb.gy;
+// Face flux of ADV_T/ADV_S through sub-edge c at layer k (adv_t.F:440-500),
+// in the flux type FT (FP64: bit-identical to the reference's expression).
+template <typename FT>
+struct CvFace { FT exflux, fxx, fyy; };
+
+template <typename FT>
+__device__ __forceinline__ CvFace<FT> cv_face(FT un, FT dtij, FT dltx, FT dlty, FT x1, FT y1, FT x2, FT y2,
+                                              FT fact, FT visc0, const CvRecT<FT>& ra, const CvRecT<FT>& rb) {
+  const FT half = (FT)0.5, one = (FT)1.0;
+  FT fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
+  FT fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
   if (fij1 < ra.fmin) fij1 = ra.fmin;
   if (fij1 > ra.fmax) fij1 = ra.fmax;
   if (fij2 < rb.fmin) fij2 = rb.fmin;
   if (fij2 > rb.fmax) fij2 = rb.fmax;
-  const double viscof = g.fact * 0.5 * (ra.vhn + rb.vhn) + g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]);
-  const double txx = 0.5 * (ra.gxd + rb.gxd) * viscof;
-  const double tyy = 0.5 * (ra.gyd + rb.gyd) * viscof;
-  TsFace r;
+  const FT viscof = fact * half * (ra.vhn + rb.vhn) + visc0;
+  const FT txx = half * (ra.gxd + rb.gxd) * viscof;
+  const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
+  CvFace<FT> r;
   r.fxx = -dtij * txx * dlty;
   r.fyy = dtij * tyy * dltx;
-  const double sgn = fsign1(un);
-  r.exflux = -un * dtij * ((1.0 + sgn) * fij2 + (1.0 - sgn) * fij1) * 0.5 + r.fxx + r.fyy;
+  const FT sgn = fsign1(un);
+  r.exflux = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + r.fxx + r.fyy;
   return r;
 }
 
-// Phase 2 of ADV_T/ADV_S, per sub-edge and layer: the same operations as
-// ts_face with the geometry held in registers across layers.
+// Phase 2 of ADV_T/ADV_S, per sub-edge and layer, with the geometry held in
+// registers across layers.
+template <typename FT>
 __global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
   const int c = (int)xcd_thread();
   if (c < 1 || c > g.NCV) return;
+  const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
+  FT* cvflux = (FT*)a.cvflux;
   const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
-  const double dltx = g.dltxe[c], dlty = g.dltye[c];
-  const double x1 = DLTXNCVE_(c, 1), y1 = DLTYNCVE_(c, 1), x2 = DLTXNCVE_(c, 2), y2 = DLTYNCVE_(c, 2);
-  const double visc0 = g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]);
+  const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
+  const FT x1 = (FT)DLTXNCVE_(c, 1), y1 = (FT)DLTYNCVE_(c, 1), x2 = (FT)DLTXNCVE_(c, 2), y2 = (FT)DLTYNCVE_(c, 2);
+  const FT visc0 = (FT)(g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]));
+  const FT fact = (FT)g.fact;
   const double dt1 = a.dt1[i1];
   const long cs = (long)g.NCV + 1;
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
-    const double dtij = dt1 * DZ1(i1, k);
-    const double un = a.v[eo + i1] * dltx - a.u[eo + i1] * dlty;
-    const CvRec ra = a.rec[ko + ia], rb = a.rec[ko + ib];
-    double fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
-    double fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
-    if (fij1 < ra.fmin) fij1 = ra.fmin;
-    if (fij1 > ra.fmax) fij1 = ra.fmax;
-    if (fij2 < rb.fmin) fij2 = rb.fmin;
-    if (fij2 > rb.fmax) fij2 = rb.fmax;
-    const double viscof = g.fact * 0.5 * (ra.vhn + rb.vhn) + visc0;
-    const double txx = 0.5 * (ra.gxd + rb.gxd) * viscof;
-    const double tyy = 0.5 * (ra.gyd + rb.gyd) * viscof;
-    const double fxx = -dtij * txx * dlty;
-    const double fyy = dtij * tyy * dltx;
-    const double sgn = fsign1(un);
-    a.cvflux[(long)(k - 1) * cs + c] = -un * dtij * ((1.0 + sgn) * fij2 + (1.0 - sgn) * fij1) * 0.5 + fxx + fyy;
+    const FT dtij = (FT)(dt1 * DZ1(i1, k));
+    const FT un = (FT)a.v[eo + i1] * dltx - (FT)a.u[eo + i1] * dlty;
+    const CvFace<FT> r = cv_face<FT>(un, dtij, dltx, dlty, x1, y1, x2, y2, fact, visc0, rec[ko + ia], rec[ko + ib]);
+    cvflux[(long)(k - 1) * cs + c] = r.exflux;
   }
 }
 
 // Advective part of the horizontal flux at the open-boundary nodes
 // (XFLUX_OBC of adv_t.F:507-520), one thread per OBC node and layer.
+template <typename FT>
 __global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.IOBCN * g.KBM1) return;
+  const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
   const int obc = (int)(t % g.IOBCN), k = (int)(t / g.IOBCN) + 1;
   const int i = g.i_obc_n[obc];
   const int n = g.cv_cnt[i];
+  const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
   double xadv = 0.0;
   for (int j = 0; j < n; ++j) {
-    const int e = g.cvl_ent[(long)j * g.m1 + i];
-    const TsFace r = ts_face(g, a, e >> 1, k);
-    if (e & 1) xadv = xadv - (r.exflux - r.fxx - r.fyy);
-    else xadv = xadv + (r.exflux - r.fxx - r.fyy);
+    const int e = g.cvl_ent[(long)j * g.m1 + i], c = e >> 1;
+    const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
+    const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
+    const FT dtij = (FT)(a.dt1[i1] * DZ1(i1, k));
+    const FT un = (FT)a.v[eo + i1] * dltx - (FT)a.u[eo + i1] * dlty;
+    const FT visc0 = (FT)(g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]));
+    const CvFace<FT> r = cv_face<FT>(un, dtij, dltx, dlty, (FT)DLTXNCVE_(c, 1), (FT)DLTYNCVE_(c, 1),
+                                     (FT)DLTXNCVE_(c, 2), (FT)DLTYNCVE_(c, 2), (FT)g.fact, visc0,
+                                     rec[ko + ia], rec[ko + ib]);
+    const double adv = (double)(r.exflux - r.fxx - r.fyy);
+    if (e & 1) xadv = xadv - adv;
+    else xadv = xadv + adv;
   }
   a.xflux_obc[(long)(k - 1) * g.IOBCN + obc] = xadv;
 }
 
-// Phase 3 of ADV_T/ADV_S (adv_t.F:507-1052): ordered sub-edge sums, bottom
-// gradients for VDIF_TS, vertical flux, rivers and the update
-// TF1 = (T1 - XFLUX/ART1*DTI/(DT*DZ))*DT/DTFA.
-template <int SMAX>
+// Phase 3 of ADV_T/ADV_S (adv_t.F:507-1052): ordered sub-edge sums (in FP64
+// whatever the flux type), bottom gradients for VDIF_TS, vertical flux,
+// rivers, the update TF1 = (T1 - XFLUX/ART1*DTI/(DT*DZ))*DT/DTFA and FCT.
+template <int SMAX, typename FT>
 __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, const double* f1, double* ff,
                                                         const double* fdis, int which, int fct) {
   const int i = (int)xcd_thread();
@@ -182,20 +187,20 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
   const long cs = (long)g.NCV + 1;
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1;
-    const double* fl = a.cvflux + (long)(k - 1) * cs;
+    const FT* fl = (const FT*)a.cvflux + (long)(k - 1) * cs;
     double xflux = 0.0;
     static_for<SMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j < n) {
-        const double x = fl[ce[j] >> 1];
+        const double x = (double)fl[ce[j] >> 1];
         if (ce[j] & 1) xflux = xflux - x;
         else xflux = xflux + x;
       }
     });
     if (k == g.KBM1) {  // ADV_T and ADV_S both set the bottom-gradient terms VDIF_TS reads
-      const CvRec r = a.rec[ko + i];
-      a.pfpxb[i] = r.gx;
-      a.pfpyb[i] = r.gy;
+      const CvRecT<FT> r = ((const CvRecT<FT>*)a.rec)[ko + i];
+      a.pfpxb[i] = (double)r.gx;
+      a.pfpyb[i] = (double)r.gy;
       a.ah_bottom[i] = (g.fact * a.viscofh[ko + i] + g.fm1) * g.nn_hvc[i];
     }
     const double fk = f1[ko + i], dzk = DZ(i, k);
@@ -241,13 +246,17 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
 
 // Phase 2 of ADV_Q (adv_q.F:200-330): face flux on layer interfaces
 // k = 2..KBM1, velocities interpolated to the interface with DZ1 weights.
+template <typename FT>
 __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
   const int c = (int)xcd_thread();
   if (c < 1 || c > g.NCV) return;
+  const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
+  FT* cvflux = (FT*)a.cvflux;
   const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
-  const double dltx = g.dltxe[c], dlty = g.dltye[c];
-  const double x1 = DLTXNCVE_(c, 1), y1 = DLTYNCVE_(c, 1), x2 = DLTXNCVE_(c, 2), y2 = DLTYNCVE_(c, 2);
-  const double visc0 = g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]);
+  const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
+  const FT x1 = (FT)DLTXNCVE_(c, 1), y1 = (FT)DLTYNCVE_(c, 1), x2 = (FT)DLTXNCVE_(c, 2), y2 = (FT)DLTYNCVE_(c, 2);
+  const FT visc0 = (FT)(g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]));
+  const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu;
   const double dt1 = a.dt1[i1];
   const long cs = (long)g.NCV + 1;
   double um = a.u[i1], vm = a.v[i1];   // layer 1
@@ -255,31 +264,32 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
     const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
     const double uk = a.u[eo + i1], vk = a.v[eo + i1];
     const double dz1k = DZ1(i1, k), dz1m = DZ1(i1, k - 1);
-    const double uq = (uk * dz1m + um * dz1k) / (dz1k + dz1m);
-    const double vq = (vk * dz1m + vm * dz1k) / (dz1k + dz1m);
+    const FT uq = (FT)((uk * dz1m + um * dz1k) / (dz1k + dz1m));
+    const FT vq = (FT)((vk * dz1m + vm * dz1k) / (dz1k + dz1m));
     um = uk;
     vm = vk;
-    const double dtij = dt1 * DZZ1(i1, k - 1);
-    const double un = vq * dltx - uq * dlty;
-    const CvRec ra = a.rec[ko + ia], rb = a.rec[ko + ib];
-    double fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
-    double fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
+    const FT dtij = (FT)(dt1 * DZZ1(i1, k - 1));
+    const FT un = vq * dltx - uq * dlty;
+    const CvRecT<FT> ra = rec[ko + ia], rb = rec[ko + ib];
+    const FT half = (FT)0.5, one = (FT)1.0;
+    FT fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
+    FT fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
     if (fij1 < ra.fmin) fij1 = ra.fmin;
     if (fij1 > ra.fmax) fij1 = ra.fmax;
     if (fij2 < rb.fmin) fij2 = rb.fmin;
     if (fij2 > rb.fmax) fij2 = rb.fmax;
-    const double viscof = (g.fact * 0.5 * (ra.vhn + rb.vhn) + visc0) / g.hprnu;
-    const double txx = 0.5 * (ra.gxd + rb.gxd) * viscof;
-    const double tyy = 0.5 * (ra.gyd + rb.gyd) * viscof;
-    const double fxx = -dtij * txx * dlty;
-    const double fyy = dtij * tyy * dltx;
-    const double sgn = fsign1(un);
-    a.cvflux[(long)(k - 1) * cs + c] = -un * dtij * ((1.0 + sgn) * fij2 + (1.0 - sgn) * fij1) * 0.5 + fxx + fyy;
+    const FT viscof = (fact * half * (ra.vhn + rb.vhn) + visc0) / rhprnu;
+    const FT txx = half * (ra.gxd + rb.gxd) * viscof;
+    const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
+    const FT fxx = -dtij * txx * dlty;
+    const FT fyy = dtij * tyy * dltx;
+    const FT sgn = fsign1(un);
+    cvflux[(long)(k - 1) * cs + c] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
   }
 }
 
 // Phase 3 of ADV_Q (adv_q.F:340-390): ordered sums, vertical flux, update.
-template <int SMAX>
+template <int SMAX, typename FT>
 __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, const double* q, double* qf, int fct) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
@@ -300,12 +310,12 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
   qf[NI(g, i, g.KB)] = 0.0;
   for (int k = 2; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1;
-    const double* fl = a.cvflux + (long)(k - 1) * cs;
+    const FT* fl = (const FT*)a.cvflux + (long)(k - 1) * cs;
     double xflux = 0.0;
     static_for<SMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j < n) {
-        const double x = fl[ce[j] >> 1];
+        const double x = (double)fl[ce[j] >> 1];
         if (ce[j] & 1) xflux = xflux - x;
         else xflux = xflux + x;
       }
