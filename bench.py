@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--kb", type=int, default=41)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only time a plain device copy of 2^25 doubles (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; the time step would shard by domain "
@@ -46,6 +48,14 @@ def main():
 
     import myfvcomcode_amd as F
     from myfvcomcode_amd import roofline
+
+    if args.copy_calibration:
+        dev = F.DeviceModel(F.Model.build(0))
+        n = 1 << 25
+        us = dev.stream_copy(n, 5)
+        print(json.dumps({"copy_doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n, "avg_us": us,
+                          "GBps": 16.0 * n / us / 1e3}), flush=True)
+        return
 
     t0 = time.time()
     model = F.Model.build(1, nx=args.nx, ny=args.ny, kb=args.kb)
