@@ -298,6 +298,39 @@ int  fvgpu_synchronize(fvgpu_handle* h);
  * calibrates HBM byte counters and gives the achievable copy bandwidth.   */
 int  fvgpu_stream_copy(fvgpu_handle* h, int64_t n, int32_t reps, double* avg_us);
 void fvgpu_destroy(fvgpu_handle* h);
+
+/* ---------------------------- learned closure ---------------------------
+ * A 5-64-64-2 MLP (SiLU hidden layers, softplus outputs) standing in for
+ * the MY2.5 stability functions and length scale that set KM/KH at the end
+ * of VDIF_Q (FVCOM_source/vdif_q.F:470-491).  No counterpart exists in the
+ * reference; the closure is an option of this library.  Parameters are an
+ * FP32 vector of fvmlp_nparams() values: W1[64][5] b1[64] W2[64][64] b2[64]
+ * W3[2][64] b3[2].  Samples are 8 floats of normalised inputs (N^2, S^2,
+ * z/h, u*, D, 1, 0, 0) and 2 floats of targets (KM, KH).                  */
+/* mode 0: MY2.5 (default); mode 1: the MLP with the per-column MY2.5
+ * fallback.  The parameters are copied (as BF16 for the MFMA inference). */
+int  fvgpu_set_closure(fvgpu_handle* h, int32_t mode, const float* params);
+/* Columns that fell back to MY2.5 in the last internal step.            */
+int  fvgpu_closure_fallbacks(fvgpu_handle* h, int64_t* fallback_columns, int64_t* columns);
+/* Features and the current KM/KH of every node and interface 2..KBM1 of
+ * the current state (x: n x 8, y: n x 2); with x or y NULL only *n is set. */
+int  fvgpu_closure_samples(fvgpu_handle* h, float* x, float* y, int64_t max_samples, int64_t* n);
+
+/* The trainer: device-resident samples, FP32 master weights, AdamW.       */
+typedef struct fvmlp fvmlp;
+int32_t fvmlp_nparams(void);
+int  fvmlp_create(int64_t n, const float* x, const float* y, int32_t bmax, fvmlp** out);
+int  fvmlp_set_params(fvmlp* t, const float* params);   /* also resets AdamW */
+int  fvmlp_get_params(fvmlp* t, float* params);
+/* Mean loss and its gradient over the samples idx[0..B) (NULL: 0..B-1).  */
+int  fvmlp_grad(fvmlp* t, const int32_t* idx, int32_t B, float* grad, float* loss);
+/* `steps` AdamW steps on seeded minibatches drawn from samples [0, ntrain);
+ * *last_loss is the last minibatch's mean loss.                          */
+int  fvmlp_train(fvmlp* t, int32_t steps, int32_t batch, int64_t ntrain, float lr, float wd, uint64_t seed,
+                 float* last_loss);
+/* Predicted (KM, KH) of samples [lo, lo+n): FP32 SIMT or BF16 MFMA.       */
+int  fvmlp_infer(fvmlp* t, int64_t lo, int64_t n, int32_t use_mfma, float* y);
+void fvmlp_destroy(fvmlp* t);
 int  fvgpu_device_count(int32_t* n);
 
 #ifdef __cplusplus

@@ -117,6 +117,18 @@ def gpu_lib():
         L.fvgpu_kernel_stat.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
+        fp = C.POINTER(C.c_float)
+        L.fvgpu_set_closure.argtypes = [vp, C.c_int32, fp]
+        L.fvgpu_closure_fallbacks.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.fvgpu_closure_samples.argtypes = [vp, fp, fp, C.c_int64, C.POINTER(C.c_int64)]
+        L.fvmlp_nparams.restype = C.c_int32
+        L.fvmlp_create.argtypes = [C.c_int64, fp, fp, C.c_int32, C.POINTER(vp)]
+        L.fvmlp_set_params.argtypes = [vp, fp]
+        L.fvmlp_get_params.argtypes = [vp, fp]
+        L.fvmlp_grad.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, fp, fp]
+        L.fvmlp_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, C.c_float, C.c_float, C.c_uint64, fp]
+        L.fvmlp_infer.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32, fp]
+        L.fvmlp_destroy.argtypes = [vp]
         L.fvgpu_profile_only.argtypes = [vp, C.c_char_p]
         L.fvgpu_synchronize.argtypes = [vp]
         L.fvgpu_stream_copy.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
@@ -332,6 +344,33 @@ class DeviceModel:
                "fvgpu_kernel_time")
         return us.value, cnt.value
 
+    def set_closure(self, mode, params=None):
+        """0: MY2.5; 1: the learned closure with per-column MY2.5 fallback."""
+        L = gpu_lib()
+        p = None
+        if params is not None:
+            params = np.ascontiguousarray(params, np.float32)
+            p = params.ctypes.data_as(C.POINTER(C.c_float))
+        _check(L, L.fvgpu_set_closure(self._h, int(mode), p), "fvgpu_set_closure")
+
+    def closure_fallbacks(self):
+        L = gpu_lib()
+        nf, nc = C.c_int64(), C.c_int64()
+        _check(L, L.fvgpu_closure_fallbacks(self._h, C.byref(nf), C.byref(nc)), "fvgpu_closure_fallbacks")
+        return nf.value, nc.value
+
+    def closure_samples(self):
+        """(x[n, 8], y[n, 2]) features and MY2.5 KM/KH of the current state."""
+        L = gpu_lib()
+        n = C.c_int64()
+        _check(L, L.fvgpu_closure_samples(self._h, None, None, 0, C.byref(n)), "fvgpu_closure_samples")
+        x = np.zeros((n.value, 8), np.float32)
+        y = np.zeros((n.value, 2), np.float32)
+        fp = C.POINTER(C.c_float)
+        _check(L, L.fvgpu_closure_samples(self._h, x.ctypes.data_as(fp), y.ctypes.data_as(fp), n.value,
+                                          C.byref(n)), "fvgpu_closure_samples")
+        return x, y
+
     def kernel_stats(self):
         """{family: (average us, launches)} of the events recorded so far."""
         L = gpu_lib()
@@ -374,3 +413,70 @@ class DeviceModel:
             self.close()
         except Exception:
             pass
+
+
+class Trainer:
+    """The closure MLP's device trainer (fvmlp): samples on the device,
+    FP32 master weights, hand-written forward/backward/AdamW kernels."""
+
+    def __init__(self, x, y, bmax=8192):
+        L = gpu_lib()
+        self.x = np.ascontiguousarray(x, np.float32)
+        self.y = np.ascontiguousarray(y, np.float32)
+        assert self.x.shape[1] == 8 and self.y.shape[1] == 2 and len(self.x) == len(self.y)
+        fp = C.POINTER(C.c_float)
+        h = C.c_void_p()
+        _check(L, L.fvmlp_create(len(self.x), self.x.ctypes.data_as(fp), self.y.ctypes.data_as(fp), int(bmax),
+                                 C.byref(h)), "fvmlp_create")
+        self._h = h
+        self.n = len(self.x)
+        self.nparams = L.fvmlp_nparams()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                gpu_lib().fvmlp_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        _check(gpu_lib(), gpu_lib().fvmlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))),
+               "fvmlp_set_params")
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        _check(gpu_lib(), gpu_lib().fvmlp_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))),
+               "fvmlp_get_params")
+        return p
+
+    def grad(self, idx=None, batch=None):
+        L = gpu_lib()
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        fp = C.POINTER(C.c_float)
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, np.int32)
+            rc = L.fvmlp_grad(self._h, idx.ctypes.data_as(C.POINTER(C.c_int32)), len(idx), g.ctypes.data_as(fp),
+                              C.byref(loss))
+        else:
+            rc = L.fvmlp_grad(self._h, None, int(batch), g.ctypes.data_as(fp), C.byref(loss))
+        _check(L, rc, "fvmlp_grad")
+        return loss.value, g
+
+    def train(self, steps, batch=8192, ntrain=None, lr=2e-3, wd=1e-4, seed=0):
+        L = gpu_lib()
+        loss = C.c_float()
+        _check(L, L.fvmlp_train(self._h, int(steps), int(batch), int(ntrain or self.n), float(lr), float(wd),
+                                int(seed), C.byref(loss)), "fvmlp_train")
+        return loss.value
+
+    def infer(self, lo=0, n=None, mfma=True):
+        L = gpu_lib()
+        n = int(n if n is not None else self.n - lo)
+        y = np.zeros((n, 2), np.float32)
+        _check(L, L.fvmlp_infer(self._h, int(lo), n, 1 if mfma else 0, y.ctypes.data_as(C.POINTER(C.c_float))),
+               "fvmlp_infer")
+        return y

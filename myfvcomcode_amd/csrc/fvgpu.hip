@@ -16,7 +16,7 @@
 //    and one internal step can be captured as a replayable hipGraph;
 //  * time-dependent scalars (RAMP, ExtTime) live in a device clock updated
 //    by single-thread kernels, never by host writes between launches.
-#include "fvgpu_kernels_edge.h"
+#include "fvgpu_closure.h"
 #include "../../include/fvgpu.h"
 
 #include <cmath>
@@ -65,6 +65,13 @@ struct fvgpu_handle {
   std::vector<hipEvent_t> event_pool;
   size_t event_next = 0;
   void* diag_part = nullptr;   // 256 partial sums of the diagnostics reduction
+  // learned closure (mode 1), allocated by fvgpu_set_closure
+  int closure_mode = 0;
+  float* cl_x = nullptr;       // features, M*(KBM1-1) x 8
+  float* cl_yp = nullptr;      // predicted (KM, KH)
+  MlpPacked* cl_w = nullptr;   // BF16 fragments
+  int* cl_flag = nullptr;      // per node: 1 = MY2.5 fallback
+  int* cl_nfall = nullptr;     // fallback columns of the last step
 };
 
 namespace {
@@ -216,7 +223,26 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   LAUNCH("wreal", k_wreal, nb(g.N + 1), 256, g, a);
   const double qmean1 = (double)1.E-8f;
   const int fct = h->md->p.scalar_positivity ? 1 : 0;
-  if (h->md->p.vertical_closure) {
+  if (h->md->p.vertical_closure && h->closure_mode == 1) {
+    // learned closure: features, MFMA inference, realizability, then the
+    // MY2.5 column solve (without horizontal advection of Q2/Q2L) for the
+    // rejected columns only
+    const long ns = (long)g.M * (g.KBM1 - 1);
+#define CLF_CALL(RM) LAUNCH("closure_features", k_closure_features<RM>, nb(g.M), 256, g, a, h->cl_x, (float*)nullptr)
+    FV_RMAX_DISPATCH(g.ve_max, CLF_CALL);
+    LAUNCH("closure_infer", k_mlp_infer_mfma, (unsigned)std::min<long>((ns + 31) / 32, 8192), 64, h->cl_w, h->cl_x, ns,
+           h->cl_yp);
+    LAUNCH("zero", k_zero_int, 1, 64, h->cl_nfall, 1);
+    LAUNCH("closure_apply", k_closure_apply, nb(g.M), 256, g, a, h->cl_yp, a.gx, h->cl_flag, h->cl_nfall);
+    LAUNCH("copy", k_copy, nb(mk), 256, a.q2f, a.q2, mk);
+    LAUNCH("copy", k_copy, nb(mk), 256, a.q2lf, a.q2l, mk);
+#define VDIF_QM_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx, h->cl_flag)
+#define VDIF_QM_CALL16(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), nb(g.M + 1), 256, g, a, a.gx, h->cl_flag)
+    if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL);
+    else FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL16);
+    LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
+    LAUNCH("copy", k_copy, nb(mk), 256, a.q2l, a.q2lf, mk);
+  } else if (h->md->p.vertical_closure) {
     for (int which = 0; which < 2; ++which) {
       double* q = which ? a.q2l : a.q2;
       double* qf = which ? a.q2lf : a.q2f;
@@ -226,8 +252,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
 #define ADV_Q_CALL(SM) LAUNCH("adv_q", (k_adv_q_gather<SM, FT>), nb(g.M + 1), 256, g, a, q, qf, fct)
       FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
     }
-#define VDIF_Q_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx)
-#define VDIF_Q_CALL16(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), nb(g.M + 1), 256, g, a, a.gx)
+#define VDIF_Q_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx, (const int*)nullptr)
+#define VDIF_Q_CALL16(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), nb(g.M + 1), 256, g, a, a.gx, (const int*)nullptr)
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
     LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
@@ -811,3 +837,219 @@ extern "C" int fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out) {
   out->iint = c.iint;
   return 0;
 }
+
+// ------------------------------------------------------------ learned closure
+extern "C" {
+
+int fvgpu_set_closure(fvgpu_handle* h, int32_t mode, const float* params) {
+  if (mode != 0 && mode != 1) return fail("closure mode must be 0 (MY2.5) or 1 (learned)");
+  if (mode == 1) {
+    if (!params) return fail("learned closure needs parameters");
+    if (!h->md->p.vertical_closure) return fail("the learned closure replaces VERTICAL_MIXING_TYPE='closure'");
+    const DevGrid& g = h->g;
+    const size_t ns = (size_t)g.M * (g.KBM1 - 1);
+    if (!h->cl_x) {
+      if (dalloc(h, &h->cl_x, ns * MLP_IN) || dalloc(h, &h->cl_yp, ns * MLP_OUT) ||
+          dalloc(h, &h->cl_w, 1) || dalloc(h, &h->cl_flag, (size_t)g.m1) || dalloc(h, &h->cl_nfall, 1))
+        return 1;
+    }
+    float* dp = nullptr;
+    HIPCHK(hipMalloc((void**)&dp, MLP_NPARAM * sizeof(float)));
+    HIPCHK(hipMemcpy(dp, params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, h->stream, dp, h->cl_w);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(dp));
+  }
+  h->closure_mode = mode;
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }   // the step changed
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  return 0;
+}
+
+int fvgpu_closure_fallbacks(fvgpu_handle* h, int64_t* fallback_columns, int64_t* columns) {
+  int n = 0;
+  if (h->cl_nfall) {
+    HIPCHK(hipMemcpyAsync(&n, h->cl_nfall, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  *fallback_columns = n;
+  *columns = h->g.M;
+  return 0;
+}
+
+int fvgpu_closure_samples(fvgpu_handle* h, float* x, float* y, int64_t max_samples, int64_t* n) {
+  const DevGrid& g = h->g;
+  const size_t ns = (size_t)g.M * (g.KBM1 - 1);
+  *n = (int64_t)ns;
+  if (!x || !y) return 0;   // size query
+  if (max_samples < (int64_t)ns) return fail("sample buffer too small");
+  float *dx = nullptr, *dy = nullptr;
+  HIPCHK(hipMalloc((void**)&dx, ns * MLP_IN * sizeof(float)));
+  HIPCHK(hipMalloc((void**)&dy, ns * MLP_OUT * sizeof(float)));
+  IntArgs a = h->ia;
+#define CLD_CALL(RM) hipLaunchKernelGGL(k_closure_features<RM>, dim3(nb(g.M)), dim3(256), 0, h->stream, g, a, dx, dy)
+  FV_RMAX_DISPATCH(g.ve_max, CLD_CALL);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(x, dx, ns * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(y, dy, ns * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipFree(dx));
+  HIPCHK(hipFree(dy));
+  return 0;
+}
+
+}  // extern "C"
+
+// The trainer: device-resident samples, FP32 master weights and AdamW state.
+struct fvmlp {
+  int64_t n = 0;
+  int bmax = 0;
+  int step = 0;
+  hipStream_t stream = nullptr;
+  float *X = nullptr, *Y = nullptr, *P = nullptr, *G = nullptr, *Mo = nullptr, *Ve = nullptr;
+  float *H1 = nullptr, *H2 = nullptr, *D1 = nullptr, *D2 = nullptr, *DO = nullptr, *loss = nullptr, *lsum = nullptr;
+  float* Yp = nullptr;
+  int* idx = nullptr;
+  MlpPacked* packed = nullptr;
+  std::vector<void*> allocs;
+};
+
+namespace {
+template <typename T>
+int malloc_mlp(fvmlp* t, T** p, size_t n) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
+  HIPCHK(hipMemset(q, 0, (n ? n : 1) * sizeof(T)));
+  t->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+// Minibatch of one training step: fwd/bwd, weight gradients, loss sum.
+int mlp_batch(fvmlp* t, const int* didx, int B, float* loss_out) {
+  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((B + 255) / 256), dim3(256), 0, t->stream, t->P, t->X, t->Y, didx, B, t->H1,
+                     t->H2, t->D1, t->D2, t->DO, t->loss);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, t->stream, t->X, didx, B, t->H1, t->H2,
+                     t->D1, t->D2, t->DO, t->G);
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, t->stream, t->loss, (long)B, t->lsum);
+  HIPCHK(hipGetLastError());
+  if (loss_out) {
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(&l, t->lsum, sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    *loss_out = l / (float)B;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int fvmlp_create(int64_t n, const float* x, const float* y, int32_t bmax, fvmlp** out) {
+  if (n < 1 || bmax < 1 || !x || !y || !out) return fail("fvmlp_create: bad arguments");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail("no HIP device");
+  fvmlp* t = new fvmlp();
+  t->n = n;
+  t->bmax = bmax;
+  const size_t bb = (size_t)bmax;
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
+      malloc_mlp(t, &t->X, (size_t)n * MLP_IN) || malloc_mlp(t, &t->Y, (size_t)n * MLP_OUT) ||
+      malloc_mlp(t, &t->P, MLP_NPARAM) || malloc_mlp(t, &t->G, MLP_NPARAM) || malloc_mlp(t, &t->Mo, MLP_NPARAM) ||
+      malloc_mlp(t, &t->Ve, MLP_NPARAM) || malloc_mlp(t, &t->H1, bb * MLP_H) || malloc_mlp(t, &t->H2, bb * MLP_H) ||
+      malloc_mlp(t, &t->D1, bb * MLP_H) || malloc_mlp(t, &t->D2, bb * MLP_H) || malloc_mlp(t, &t->DO, bb * MLP_OUT) ||
+      malloc_mlp(t, &t->loss, bb) || malloc_mlp(t, &t->lsum, 1) || malloc_mlp(t, &t->idx, bb) ||
+      malloc_mlp(t, &t->Yp, (size_t)n * MLP_OUT) || malloc_mlp(t, &t->packed, 1)) {
+    fvmlp_destroy(t);
+    return 1;
+  }
+  HIPCHK(hipMemcpy(t->X, x, (size_t)n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(t->Y, y, (size_t)n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice));
+  *out = t;
+  return 0;
+}
+
+int fvmlp_set_params(fvmlp* t, const float* p) {
+  HIPCHK(hipMemcpy(t->P, p, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(t->Mo, 0, MLP_NPARAM * sizeof(float)));
+  HIPCHK(hipMemset(t->Ve, 0, MLP_NPARAM * sizeof(float)));
+  t->step = 0;
+  return 0;
+}
+
+int fvmlp_get_params(fvmlp* t, float* p) {
+  HIPCHK(hipStreamSynchronize(t->stream));
+  HIPCHK(hipMemcpy(p, t->P, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int32_t fvmlp_nparams(void) { return MLP_NPARAM; }
+
+int fvmlp_grad(fvmlp* t, const int32_t* idx, int32_t B, float* grad, float* loss) {
+  if (B < 1 || B > t->bmax) return fail("fvmlp_grad: batch size outside [1, bmax]");
+  const int* didx = nullptr;
+  if (idx) {
+    for (int b = 0; b < B; ++b)
+      if (idx[b] < 0 || idx[b] >= t->n) return fail("fvmlp_grad: sample index out of range");
+    HIPCHK(hipMemcpy(t->idx, idx, (size_t)B * sizeof(int), hipMemcpyHostToDevice));
+    didx = t->idx;
+  } else if (B > t->n) {
+    return fail("fvmlp_grad: batch larger than the sample set");
+  }
+  if (mlp_batch(t, didx, B, loss)) return 1;
+  HIPCHK(hipMemcpy(grad, t->G, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fvmlp_train(fvmlp* t, int32_t steps, int32_t batch, int64_t ntrain, float lr, float wd, uint64_t seed,
+                float* last_loss) {
+  if (batch < 1 || batch > t->bmax) return fail("fvmlp_train: batch size outside [1, bmax]");
+  if (ntrain < 1 || ntrain > t->n) return fail("fvmlp_train: ntrain outside [1, n]");
+  std::vector<int> hidx((size_t)batch);
+  uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+  float l = 0.0f;
+  for (int it = 0; it < steps; ++it) {
+    for (int b = 0; b < batch; ++b) {   // seeded minibatch (PCG-style LCG, high bits)
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      hidx[b] = (int)((st >> 33) % (uint64_t)ntrain);
+    }
+    HIPCHK(hipMemcpyAsync(t->idx, hidx.data(), (size_t)batch * sizeof(int), hipMemcpyHostToDevice, t->stream));
+    if (mlp_batch(t, t->idx, batch, it + 1 == steps ? &l : nullptr)) return 1;
+    t->step += 1;
+    hipLaunchKernelGGL(k_adamw, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, t->stream, t->P, t->G, t->Mo, t->Ve,
+                       t->step, lr, wd, 0.9f, 0.999f, 1e-8f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(t->stream));   // the host index buffer is reused
+  }
+  if (last_loss) *last_loss = l;
+  return 0;
+}
+
+int fvmlp_infer(fvmlp* t, int64_t lo, int64_t n, int32_t use_mfma, float* y) {
+  if (lo < 0 || n < 1 || lo + n > t->n) return fail("fvmlp_infer: range outside the sample set");
+  if (use_mfma) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, t->stream, t->P, t->packed);
+    const long groups = (n + 31) / 32;
+    hipLaunchKernelGGL(k_mlp_infer_mfma, dim3((unsigned)std::min<long>(groups, 8192)), dim3(64), 0, t->stream,
+                       t->packed, t->X + lo * MLP_IN, (long)n, t->Yp);
+  } else {
+    hipLaunchKernelGGL(k_mlp_infer_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, t->stream, t->P,
+                       t->X + lo * MLP_IN, (long)n, t->Yp);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, t->Yp, (size_t)n * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return 0;
+}
+
+void fvmlp_destroy(fvmlp* t) {
+  if (!t) return;
+  if (t->stream) hipStreamSynchronize(t->stream);
+  for (void* p : t->allocs) hipFree(p);
+  if (t->stream) hipStreamDestroy(t->stream);
+  delete t;
+}
+
+}  // extern "C"

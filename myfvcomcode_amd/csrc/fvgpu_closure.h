@@ -1,0 +1,413 @@
+// fvgpu_closure.h -- the learned vertical closure: a 5-64-64-2 MLP standing
+// in for MY2.5's KM/KH (vdif_q.F:470-491).
+//
+//   features  per node and interface k = 2..KBM1, the quantities VDIF_Q
+//             itself forms: N^2 = -BOYGR (vdif_q.F:256-268), S^2 from the
+//             node-averaged shear (vdif_q.F:274-303), z/h = Z(k), the
+//             surface friction velocity u* = sqrt(|tau_s|) (vdif_q.F:172-190)
+//             and the depth D.  Normalised inputs:
+//               x0 = asinh(N^2 / 1e-5), x1 = log10(S^2 + 1e-10) / 4 + 1,
+//               x2 = z/h, x3 = u* / 0.01, x4 = log10(D) - 1, x5 = 1 (bias),
+//               x6 = x7 = 0 (padding to the MFMA K of 8)
+//   network   h1 = silu(W1 x + b1), h2 = silu(W2 h1 + b2),
+//             (KM, KH) = softplus(W3 h2 + b3) + 1e-7
+//   training  FP32 master weights, hand-written forward/backward/AdamW
+//             kernels, loss = mean over samples of
+//             (ln KM - ln KM_MY25)^2 + (ln KH - ln KH_MY25)^2
+//   inference BF16 weights, FP32 accumulation on MFMA
+//             (v_mfma_f32_32x32x8_bf16): one wavefront evaluates 32 samples
+//             per pass in the transposed form H^T = W X^T, so each layer's
+//             accumulators are already the next layer's B operand.
+//   fallback  a column whose predictions leave [1e-6, 1] m^2/s, or that
+//             predicts KH > KM where the gradient Richardson number
+//             N^2/S^2 exceeds 1 (stable stratification mixes heat no faster
+//             than momentum), takes the MY2.5 column solve instead.
+#pragma once
+#include "fvgpu_kernels_edge.h"
+
+#define MLP_IN 8       // 5 inputs + bias + 2 padding
+#define MLP_H 64
+#define MLP_OUT 2
+// FP32 parameter vector: W1[64][5] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]
+#define MLP_OFF_W1 0
+#define MLP_OFF_B1 (MLP_OFF_W1 + MLP_H * 5)
+#define MLP_OFF_W2 (MLP_OFF_B1 + MLP_H)
+#define MLP_OFF_B2 (MLP_OFF_W2 + MLP_H * MLP_H)
+#define MLP_OFF_W3 (MLP_OFF_B2 + MLP_H)
+#define MLP_OFF_B3 (MLP_OFF_W3 + MLP_OUT * MLP_H)
+#define MLP_NPARAM (MLP_OFF_B3 + MLP_OUT)
+#define MLP_KEPS 1e-7f
+
+__device__ __forceinline__ float mlp_sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+__device__ __forceinline__ float mlp_silu(float z) { return z * mlp_sigmoid(z); }
+__device__ __forceinline__ float mlp_dsilu(float z) {
+  const float s = mlp_sigmoid(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ float mlp_softplus(float z) { return z > 20.0f ? z : log1pf(expf(z)); }
+
+// ------------------------------------------------------------ features
+// Node-layer features of the current state, and optionally the MY2.5
+// targets (KM, KH) when `y` is given.  Sample s = (k-2)*M + (i-1).
+template <int RMAX>
+__global__ __launch_bounds__(256) void k_closure_features(DevGrid g, IntArgs a, float* x, float* y) {
+  const int i = (int)xcd_thread() + 1;
+  if (i > g.M) return;
+  const int o = g.nbve_off[i], nt = g.ntve[i];
+  const double ntd = (double)nt, d = a.d[i], ramp = a.clk->ramp_int;
+  int re[RMAX];
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    re[j] = j < nt ? g.nbve[o + j] : 0;
+  });
+  double wus = 0.0, wvs = 0.0;
+  for (int j = 0; j < nt; ++j) {
+    const int e = g.nbve[o + j];
+    wus = wus + (g.wind_on ? -g.wsx[e] * ramp * 0.001 : 0.0);
+    wvs = wvs + (g.wind_on ? -g.wsy[e] * ramp * 0.001 : 0.0);
+  }
+  wus = wus / ntd;
+  wvs = wvs / ntd;
+  const float ustar = (float)sqrt(sqrt(wus * wus + wvs * wvs));
+  auto ring_avg = [&](const double* f, long eo) {
+    double s = 0.0;
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j < nt) s = s + f[eo + re[j]];
+    });
+    return s / ntd;
+  };
+  double um = ring_avg(a.u, 0), vm = ring_avg(a.v, 0);
+  const float ld = (float)log10(d > 1e-3 ? d : 1e-3) - 1.0f;
+  for (int k = 2; k <= g.KBM1; ++k) {
+    const long q = NI(g, i, k), eo = (long)(k - 1) * g.n1;
+    const double dzd = DZZ(i, k - 1) * d;
+    const double n2 = -g.grav_n[i] * (a.rho1[q - g.m1] - a.rho1[q]) / dzd;
+    const double uk = ring_avg(a.u, eo), vk = ring_avg(a.v, eo);
+    const double s2 = ((uk - um) * (uk - um) + (vk - vm) * (vk - vm)) / (dzd * dzd);
+    um = uk;
+    vm = vk;
+    const long s = (long)(k - 2) * g.M + (i - 1);
+    float4* xs = (float4*)(x + s * MLP_IN);
+    xs[0] = make_float4((float)asinh(n2 / 1e-5), (float)(log10(s2 + 1e-10) * 0.25 + 1.0), (float)ZN(i, k),
+                        ustar / 0.01f);
+    xs[1] = make_float4(ld, 1.0f, 0.0f, 0.0f);
+    if (y) {
+      y[s * 2 + 0] = (float)a.km[q];
+      y[s * 2 + 1] = (float)a.kh[q];
+    }
+    // gradient Richardson number of the layer, kept for the fallback test
+    if (!y) a.gx[q] = n2 / (s2 > 1e-12 ? s2 : 1e-12);
+  }
+}
+
+// ------------------------------------------------------------ training (FP32)
+// Forward pass and the per-sample deltas of one minibatch; thread per sample.
+// Stores H1, H2 (activations), D1, D2 (deltas) and DO (output deltas) for
+// the weight-gradient kernel, and the sample's loss.  D1 holds the
+// first-layer pre-activations until the deltas overwrite them.
+__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ P, const float* __restrict__ X,
+                                                      const float* __restrict__ Y, const int* __restrict__ idx,
+                                                      int B, float* H1, float* H2, float* D1, float* D2, float* DO,
+                                                      float* loss) {
+  __shared__ float sP[MLP_NPARAM];
+  for (int t = threadIdx.x; t < MLP_NPARAM; t += blockDim.x) sP[t] = P[t];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const long s = idx ? idx[b] : b;
+  float* h1g = H1 + (long)b * MLP_H;
+  float* d1g = D1 + (long)b * MLP_H;
+  float xin[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) xin[c] = X[s * MLP_IN + c];
+  float h1[MLP_H];
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    float acc = sP[MLP_OFF_B1 + j];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += sP[MLP_OFF_W1 + j * 5 + c] * xin[c];
+    d1g[j] = acc;
+    h1[j] = mlp_silu(acc);
+    h1g[j] = h1[j];
+  }
+  float z2[MLP_H];
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    float acc = sP[MLP_OFF_B2 + j];
+#pragma unroll
+    for (int c = 0; c < MLP_H; ++c) acc += sP[MLP_OFF_W2 + j * MLP_H + c] * h1[c];
+    z2[j] = acc;
+  }
+  float o0 = sP[MLP_OFF_B3], o1 = sP[MLP_OFF_B3 + 1];
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    const float h2 = mlp_silu(z2[j]);
+    H2[(long)b * MLP_H + j] = h2;
+    o0 += sP[MLP_OFF_W3 + j] * h2;
+    o1 += sP[MLP_OFF_W3 + MLP_H + j] * h2;
+  }
+  float dout[MLP_OUT], l = 0.0f;
+  const float oo[MLP_OUT] = {o0, o1};
+#pragma unroll
+  for (int c = 0; c < MLP_OUT; ++c) {
+    const float k = mlp_softplus(oo[c]) + MLP_KEPS;
+    const float r = logf(k) - logf(fmaxf(Y[s * 2 + c], MLP_KEPS));
+    l += r * r;
+    dout[c] = 2.0f * r / k * mlp_sigmoid(oo[c]) / (float)B;   // d(mean loss)/do
+    DO[(long)b * MLP_OUT + c] = dout[c];
+  }
+  loss[b] = l;
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    const float acc = sP[MLP_OFF_W3 + j] * dout[0] + sP[MLP_OFF_W3 + MLP_H + j] * dout[1];
+    z2[j] = acc * mlp_dsilu(z2[j]);          // z2 now holds delta2
+    D2[(long)b * MLP_H + j] = z2[j];
+  }
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < MLP_H; ++c) acc += sP[MLP_OFF_W2 + c * MLP_H + j] * z2[c];
+    d1g[j] = acc * mlp_dsilu(d1g[j]);
+  }
+}
+
+// Gradient of every parameter over the minibatch, one thread per parameter
+// (a fixed-order sum over the samples, so the result is deterministic).
+__global__ void k_mlp_wgrad(const float* __restrict__ X, const int* __restrict__ idx, int B,
+                            const float* __restrict__ H1, const float* __restrict__ H2,
+                            const float* __restrict__ D1, const float* __restrict__ D2,
+                            const float* __restrict__ DO, float* G) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MLP_NPARAM) return;
+  float acc = 0.0f;
+  if (p < MLP_OFF_B1) {
+    const int j = (p - MLP_OFF_W1) / 5, c = (p - MLP_OFF_W1) % 5;
+    for (int b = 0; b < B; ++b) acc += D1[(long)b * MLP_H + j] * X[(long)(idx ? idx[b] : b) * MLP_IN + c];
+  } else if (p < MLP_OFF_W2) {
+    const int j = p - MLP_OFF_B1;
+    for (int b = 0; b < B; ++b) acc += D1[(long)b * MLP_H + j];
+  } else if (p < MLP_OFF_B2) {
+    const int j = (p - MLP_OFF_W2) / MLP_H, c = (p - MLP_OFF_W2) % MLP_H;
+    for (int b = 0; b < B; ++b) acc += D2[(long)b * MLP_H + j] * H1[(long)b * MLP_H + c];
+  } else if (p < MLP_OFF_W3) {
+    const int j = p - MLP_OFF_B2;
+    for (int b = 0; b < B; ++b) acc += D2[(long)b * MLP_H + j];
+  } else if (p < MLP_OFF_B3) {
+    const int c = (p - MLP_OFF_W3) / MLP_H, j = (p - MLP_OFF_W3) % MLP_H;
+    for (int b = 0; b < B; ++b) acc += DO[(long)b * MLP_OUT + c] * H2[(long)b * MLP_H + j];
+  } else {
+    const int c = p - MLP_OFF_B3;
+    for (int b = 0; b < B; ++b) acc += DO[(long)b * MLP_OUT + c];
+  }
+  G[p] = acc;
+}
+
+// Sum of n values in a fixed order by one workgroup of 256 threads.
+__global__ void k_sum_fixed(const float* v, long n, float* out) {
+  __shared__ float sh[256];
+  float acc = 0.0f;
+  for (long t = threadIdx.x; t < n; t += 256) acc += v[t];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// AdamW step on the FP32 master weights (decoupled weight decay on the
+// weight matrices, not the biases).
+__global__ void k_adamw(float* P, const float* G, float* Mo, float* Ve, int step, float lr, float wd,
+                        float b1, float b2, float eps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MLP_NPARAM) return;
+  const bool is_bias = (p >= MLP_OFF_B1 && p < MLP_OFF_W2) || (p >= MLP_OFF_B2 && p < MLP_OFF_W3) || p >= MLP_OFF_B3;
+  const float g = G[p];
+  const float m = b1 * Mo[p] + (1.0f - b1) * g;
+  const float v = b2 * Ve[p] + (1.0f - b2) * g * g;
+  Mo[p] = m;
+  Ve[p] = v;
+  const float mh = m / (1.0f - powf(b1, (float)step)), vh = v / (1.0f - powf(b2, (float)step));
+  float w = P[p];
+  if (!is_bias) w -= lr * wd * w;
+  P[p] = w - lr * mh / (sqrtf(vh) + eps);
+}
+
+// FP32 forward of n samples (the reference the MFMA path is tested against).
+__global__ __launch_bounds__(256) void k_mlp_infer_f32(const float* __restrict__ P, const float* __restrict__ X, long n, float* Yp) {
+  __shared__ float sP[MLP_NPARAM];
+  for (int t = threadIdx.x; t < MLP_NPARAM; t += blockDim.x) sP[t] = P[t];
+  __syncthreads();
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float h1[MLP_H];
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    float acc = sP[MLP_OFF_B1 + j];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc += sP[MLP_OFF_W1 + j * 5 + c] * X[s * MLP_IN + c];
+    h1[j] = mlp_silu(acc);
+  }
+  float o0 = sP[MLP_OFF_B3], o1 = sP[MLP_OFF_B3 + 1];
+#pragma unroll
+  for (int j = 0; j < MLP_H; ++j) {
+    float acc = sP[MLP_OFF_B2 + j];
+#pragma unroll
+    for (int c = 0; c < MLP_H; ++c) acc += sP[MLP_OFF_W2 + j * MLP_H + c] * h1[c];
+    const float h2 = mlp_silu(acc);
+    o0 += sP[MLP_OFF_W3 + j] * h2;
+    o1 += sP[MLP_OFF_W3 + MLP_H + j] * h2;
+  }
+  Yp[s * 2 + 0] = mlp_softplus(o0) + MLP_KEPS;
+  Yp[s * 2 + 1] = mlp_softplus(o1) + MLP_KEPS;
+}
+
+// ------------------------------------------------------------ inference (MFMA)
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline unsigned short mlp_to_bf16(float f) {   // round to nearest even
+  union { float f; unsigned int u; } c;
+  c.f = f;
+  const unsigned int u = c.u + 0x7fffu + ((c.u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// BF16 A-operand fragments of the three layers, laid out per lane, built
+// from the FP32 parameters once (k_mlp_pack).  For a 32x32x8 MFMA, lane l
+// holds A[row = l % 32][k = 4*(l/32) .. 4*(l/32)+3].
+//   W1aug: 2 row tiles (hidden 0-31, 32-63), K = 8 (5 inputs, bias, 0, 0)
+//   W2:    2 row tiles x 8 k-steps
+//   W3:    1 row tile (rows 0,1 used) x 8 k-steps
+struct MlpPacked {
+  mlp_bf16x4 w1[2][64];
+  mlp_bf16x4 w2[2][8][64];
+  mlp_bf16x4 w3[8][64];
+  float b2[MLP_H];
+  float b3[MLP_OUT];
+};
+
+__global__ void k_mlp_pack(const float* P, MlpPacked* out) {
+  const int l = threadIdx.x;   // one wavefront
+  if (l >= 64) return;
+  const int r = l % 32, kb = 4 * (l / 32);
+  for (int t = 0; t < 2; ++t) {
+    mlp_bf16x4 v;
+    for (int q = 0; q < 4; ++q) {
+      const int k = kb + q, j = 32 * t + r;
+      float w = 0.0f;
+      if (k < 5) w = P[MLP_OFF_W1 + j * 5 + k];
+      else if (k == 5) w = P[MLP_OFF_B1 + j];
+      v[q] = (short)mlp_to_bf16(w);
+    }
+    out->w1[t][l] = v;
+    for (int s = 0; s < 8; ++s) {
+      mlp_bf16x4 u;
+      for (int q = 0; q < 4; ++q) u[q] = (short)mlp_to_bf16(P[MLP_OFF_W2 + (32 * t + r) * MLP_H + 8 * s + kb + q]);
+      out->w2[t][s][l] = u;
+    }
+  }
+  for (int s = 0; s < 8; ++s) {
+    mlp_bf16x4 u;
+    for (int q = 0; q < 4; ++q) u[q] = (short)(r < MLP_OUT ? mlp_to_bf16(P[MLP_OFF_W3 + r * MLP_H + 8 * s + kb + q]) : 0);
+    out->w3[s][l] = u;
+  }
+  if (l < MLP_H) out->b2[l] = P[MLP_OFF_B2 + l];
+  if (l < MLP_OUT) out->b3[l] = P[MLP_OFF_B3 + l];
+}
+
+// Inference of n samples; each wavefront loops over groups of 32 samples.
+// Accumulator layout of the 32x32 MFMA: lane l, register v holds
+// D[row = 8*(v/4) + 4*(l/32) + v%4][col = l % 32]; with samples as columns,
+// registers 4q..4q+3 of row tile t are the B fragment (k-step 4t+q) of the
+// next layer for the same lane.
+__global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restrict__ W, const float* __restrict__ X,
+                                                       long n, float* Yp) {
+  const int l = threadIdx.x;
+  const int col = l % 32, half = l / 32;
+  mlp_bf16x4 w1[2], w2[2][8], w3[8];
+  float b2r[2][16];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    w1[t] = W->w1[t][l];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) w2[t][s] = W->w2[t][s][l];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) b2r[t][v] = W->b2[32 * t + 8 * (v / 4) + 4 * half + (v % 4)];
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) w3[s] = W->w3[s][l];
+  const float b30 = W->b3[0], b31 = W->b3[1];
+  const long groups = (n + 31) / 32;
+  for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+    const long smp = grp * 32 + col;
+    // B operand of layer 1: X^T, lane holds inputs 4*half..4*half+3 of its sample
+    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (smp < n) xv = ((const float4*)(X + smp * MLP_IN))[half];
+    mlp_bf16x4 xb;
+    xb[0] = (short)mlp_to_bf16(xv.x); xb[1] = (short)mlp_to_bf16(xv.y);
+    xb[2] = (short)mlp_to_bf16(xv.z); xb[3] = (short)mlp_to_bf16(xv.w);
+    mlp_bf16x4 h1b[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      mlp_f32x16 acc = {};
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1[t], xb, acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h1b[t][q][e] = (short)mlp_to_bf16(mlp_silu(acc[4 * q + e]));
+    }
+    mlp_bf16x4 h2b[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      mlp_f32x16 acc;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[v] = b2r[t][v];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[t][s], h1b[s / 4][s % 4], acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h2b[t][q][e] = (short)mlp_to_bf16(mlp_silu(acc[4 * q + e]));
+    }
+    mlp_f32x16 o = {};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) o = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3[s], h2b[s / 4][s % 4], o, 0, 0, 0);
+    if (half == 0 && smp < n) {   // rows 0 and 1 of the output tile live in lanes 0..31, registers 0 and 1
+      Yp[smp * 2 + 0] = mlp_softplus(o[0] + b30) + MLP_KEPS;
+      Yp[smp * 2 + 1] = mlp_softplus(o[1] + b31) + MLP_KEPS;
+    }
+  }
+}
+
+// ------------------------------------------------------------ closure in the step
+// Realizability test per node column; accepted columns take the predicted
+// KM/KH (surface and bottom levels copy the nearest interface), the others
+// are flagged for the MY2.5 column solve.  Ri (the gradient Richardson
+// number per interface) was left in the node-layer scratch `ri`.
+__global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const double* ri, int* flag, int* nfall) {
+  const int i = (int)xcd_thread() + 1;
+  if (i > g.M) return;
+  bool ok = true;
+  for (int k = 2; k <= g.KBM1 && ok; ++k) {
+    const long s = (long)(k - 2) * g.M + (i - 1);
+    const float km = Yp[2 * s], kh = Yp[2 * s + 1];
+    if (!(km >= 1e-6f && km <= 1.0f && kh >= 1e-6f && kh <= 1.0f)) ok = false;
+    if (ri[NI(g, i, k)] > 1.0 && kh > km) ok = false;
+  }
+  flag[i] = ok ? 0 : 1;
+  if (!ok) {
+    atomicAdd(nfall, 1);
+    return;
+  }
+  for (int k = 1; k <= g.KB; ++k) {
+    const int kk = k < 2 ? 2 : (k > g.KBM1 ? g.KBM1 : k);
+    const long s = (long)(kk - 2) * g.M + (i - 1);
+    const long q = NI(g, i, k);
+    a.km[q] = (double)Yp[2 * s];
+    a.kh[q] = (double)Yp[2 * s + 1];
+  }
+}

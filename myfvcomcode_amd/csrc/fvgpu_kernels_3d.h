@@ -58,6 +58,11 @@ __global__ void k_zero(double* p, long n) {
   if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_zero_int(int* p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 __global__ void k_copy(double* dst, const double* src, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];

@@ -168,9 +168,10 @@ __device__ __forceinline__ Q25Level q25_level(const DevGrid& g, const IntArgs& a
 // VDIF_Q, MY2.5 (vdif_q.F:34-551).  PROD (which needs ring averages of U/V)
 // is formed once per level into the node-layer scratch array `prod`.
 template <int KMAX, int RMAX>
-__global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* prod) {
+__global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* prod, const int* only) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
+  if (only && !only[i]) return;   // column subset (the learned closure's fallback)
   const int KB = g.KB, KBM1 = g.KBM1;
   const double A1 = 0.92, B1 = 16.6, A2 = 0.74, B2 = 10.1, C1 = 0.08, E1 = 1.80, E2 = 1.33, SEF = 1.00;
   const double KAPPA = 0.40, SMALL = 1.E-8, SURFL = 2.E+5;
