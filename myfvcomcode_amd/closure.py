@@ -49,7 +49,7 @@ def load_params(path=DEFAULT_PARAMS):
     return p
 
 
-def collect(case, nsteps, every, precision="fp64", **overrides):
+def collect(case, nsteps, every, precision="fp64", log=None, **overrides):
     """Run the plain model and dump samples every `every` internal steps."""
     m = Model.build(case, **overrides)
     dev = DeviceModel(m, precision=precision)
@@ -61,6 +61,8 @@ def collect(case, nsteps, every, precision="fp64", **overrides):
         x, y = dev.closure_samples()
         xs.append(x)
         ys.append(y)
+        if log:
+            log("case %d: %d steps, %d samples" % (case, done, sum(len(a) for a in xs)))
     return np.concatenate(xs), np.concatenate(ys)
 
 
@@ -102,7 +104,14 @@ def train(x, y, steps=3000, batch=8192, lr=2e-3, wd=1e-4, seed=0, log=None):
     return t.params(), rep
 
 
-def judge(params, case=1, hours=6.0, **overrides):
+def speed(st, kb):
+    """Current speed at the element-layers 1..KBM1 (dummy entry 0 dropped)."""
+    u = st["u"].reshape(kb, -1)[:-1, 1:]
+    v = st["v"].reshape(kb, -1)[:-1, 1:]
+    return np.hypot(u, v)
+
+
+def judge(params, case=1, hours=6.0, log=None, **overrides):
     """A/B run: MY2.5 vs learned vs constant mixing, RMS differences vs MY2.5."""
     m = Model.build(case, **overrides)
     dti = m.hdr.p.dte * m.hdr.p.isplit
@@ -116,8 +125,14 @@ def judge(params, case=1, hours=6.0, **overrides):
         if name == "learned":
             dev.set_closure(1, params)
         t0 = time.time()
-        dev.run(nsteps, graph=True)
-        dev.synchronize()
+        done = 0
+        while done < nsteps:
+            n = min(500, nsteps - done)
+            dev.run(n, graph=True)
+            dev.synchronize()
+            done += n
+            if log:
+                log("%s: %d / %d steps, %.1f s" % (name, done, nsteps, time.time() - t0))
         wall = time.time() - t0
         kb = mm.kb
         st = {f: dev.get(f) for f in ("el", "ua", "va", "u", "v", "t1", "s1")}
@@ -129,10 +144,11 @@ def judge(params, case=1, hours=6.0, **overrides):
     out = {"steps": nsteps, "hours": hours, "elements": m.n, "layers": m.kb - 1}
     for name in ("learned", "constant"):
         st, wall, fb = runs[name]
+        kb = m.kb
         rms = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))
         out[name] = {"el_rms_m": rms(st["el"][1:], ref["el"][1:]),
                      "surface_t_rms": rms(st["ts"], ref["ts"]), "surface_s_rms": rms(st["ss"], ref["ss"]),
-                     "current_rms": rms(np.hypot(st["u"], st["v"]), np.hypot(ref["u"], ref["v"])),
+                     "current_rms": rms(speed(st, kb), speed(ref, kb)),
                      "ms_per_step": 1000.0 * wall / nsteps, "fallback_columns_last_step": fb[0]}
     out["my25_ms_per_step"] = 1000.0 * runs["my25"][1] / nsteps
     return out
@@ -142,8 +158,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train", "judge"])
     ap.add_argument("--samples", type=int, default=50_000_000)
-    ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_PARAMS)
     ap.add_argument("--params", default=DEFAULT_PARAMS)
@@ -156,13 +172,13 @@ def main(argv=None):
     if args.cmd == "train":
         # the estuary channel (every 18 steps of 1 day) and the seamount mesh
         xs, ys = [], []
-        x, y = collect(0, 1728, 18)
+        x, y = collect(0, 1728, 18, log=log)
         xs.append(x); ys.append(y)
         log("channel samples %d" % len(x))
         need = max(0, args.samples - len(x))
         per = 501501 * 39
         snaps = max(1, int(np.ceil(need / per)))
-        x, y = collect(1, 60 * snaps, 60)
+        x, y = collect(1, 60 * snaps, 60, log=log)
         xs.append(x); ys.append(y)
         log("seamount-mesh samples %d" % len(x))
         x, y = np.concatenate(xs), np.concatenate(ys)
@@ -176,7 +192,7 @@ def main(argv=None):
             kw["nx"] = args.nx
         if args.ny:
             kw["ny"] = args.ny
-        print(json.dumps(judge(load_params(args.params), hours=args.hours, **kw)), flush=True)
+        print(json.dumps(judge(load_params(args.params), hours=args.hours, log=log, **kw)), flush=True)
     return 0
 
 
