@@ -196,6 +196,8 @@ typedef struct fv_case {
                                 1: T=20-10*(z/h)... S=33+2*(z/h)          */
   double  el_pert_std, el_pert_corr;  /* initial elevation perturbation   */
   double  coriolis;          /* f-plane Coriolis parameter (s^-1)         */
+  double  sponge_radius;     /* sponge around every OBC node (m), 0: none  */
+  double  sponge_coef;       /* its damping coefficient at the node        */
   uint64_t seed;
   fv_params p;
 } fv_case;
@@ -240,6 +242,11 @@ int  fv_model_set_river(fv_model* mdl, int32_t node, double q, double t, double 
  * files, FVCOM_source/fvcom.F:2006-2128).  Every instance of the union
  * advances exactly as it would alone.                                    */
 int  fv_model_union(const fv_model* const* parts, int32_t nparts, fv_model** out);
+/* Sponge layer (SETUP_SPONGE, FVCOM_source/mod_setup.F:458-499): for each
+ * of the n sponge nodes (1-based) with radius r and coefficient c, every
+ * element within r gets CC_SPONGE = max(CC_SPONGE, c*(1 - dist/r)).      */
+int  fv_model_set_sponge(fv_model* mdl, int32_t n, const int32_t* nodes, const double* radius,
+                         const double* coef);
 /* Cold start: EL=0, rest, DENS2 density, RMEAN/TMEAN/SMEAN, default
  * turbulence (mod_startup.F:1062-1181, 1332-1408).                       */
 int  fv_model_cold_start(fv_model* mdl);

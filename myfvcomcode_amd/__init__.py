@@ -40,7 +40,8 @@ class FvCase(C.Structure):
                 ("river_q", C.c_double), ("river_t", C.c_double), ("river_s", C.c_double),
                 ("wind", C.c_int32), ("wind_mean", C.c_double), ("wind_std", C.c_double),
                 ("wind_corr", C.c_double), ("ts_kind", C.c_int32), ("el_pert_std", C.c_double),
-                ("el_pert_corr", C.c_double), ("coriolis", C.c_double), ("seed", C.c_uint64),
+                ("el_pert_corr", C.c_double), ("coriolis", C.c_double), ("sponge_radius", C.c_double),
+                ("sponge_coef", C.c_double), ("seed", C.c_uint64),
                 ("p", FvParams)]
 
 
@@ -88,6 +89,7 @@ def grid_lib():
         L.fv_model_cold_start.argtypes = [C.c_void_p]
         L.fv_model_set_tide.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_set_river.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
+        L.fv_model_set_sponge.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_union.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
         L.fv_extent_len.restype = C.c_int64
         L.fv_extent_len.argtypes = [C.c_void_p, C.c_int32]
@@ -256,6 +258,15 @@ class Model:
         L = grid_lib()
         _check(L, L.fv_model_set_river(self._ptr, int(node), float(q), float(t), float(s)),
                "fv_model_set_river")
+
+    def set_sponge(self, nodes, radius, coef):
+        """Sponge layer around nodes (1-based): CC_SPONGE = max(c*(1-dist/r))."""
+        L = grid_lib()
+        nd = np.ascontiguousarray(nodes, np.int32)
+        r = np.ascontiguousarray(np.broadcast_to(radius, nd.shape), np.float64)
+        c = np.ascontiguousarray(np.broadcast_to(coef, nd.shape), np.float64)
+        _check(L, L.fv_model_set_sponge(self._ptr, len(nd), nd.ctypes.data, r.ctypes.data, c.ctypes.data),
+               "fv_model_set_sponge")
 
     def cold_start(self):
         """Recompute the cold-start state from the current T1/S1/EL fields."""

@@ -678,6 +678,10 @@ extern "C" int fv_case_defaults(int32_t config, fv_case* c) {
   p.iint0 = 0; p.time0 = 0.0;
   c->coriolis = 1.0e-4;
   c->tide_amp[0] = 1.0;
+  // a sponge along the open boundary, as the reference's Estuary example
+  // carries one (Examples/Estuary/tstinp/tst_spg.dat)
+  c->sponge_radius = 4000.0;
+  c->sponge_coef = 0.01;
   c->seed = 20240601;
   switch (config) {
     case 0:
@@ -782,6 +786,10 @@ extern "C" int fv_model_build(const fv_case* c, fv_model** out) {
     md->cc_sponge[i] = 0.0;
   }
   for (int i = 0; i <= M; ++i) md->nn_hvc[i] = i ? 0.2 : 0.0;
+  if (c->sponge_radius > 0.0 && c->sponge_coef > 0.0 && md->iobcn > 0) {
+    std::vector<double> r((size_t)md->iobcn, c->sponge_radius), cf((size_t)md->iobcn, c->sponge_coef);
+    if (fv_model_set_sponge(md, md->iobcn, md->i_obc_n, r.data(), cf.data())) { fv_model_free(md); return 1; }
+  }
   // river at the west-end node (middle of the west wall)
   if (c->river) {
     const int jn = (ny / 2) * nx + 1;
@@ -818,6 +826,22 @@ extern "C" int fv_model_build(const fv_case* c, fv_model** out) {
   rc = fv_model_cold_start(md);
   if (rc) { fv_model_free(md); return rc; }
   *out = md;
+  return 0;
+}
+
+extern "C" int fv_model_set_sponge(fv_model* md, int32_t n, const int32_t* nodes, const double* radius,
+                                   const double* coef) {
+  if (n < 0 || (n > 0 && (!nodes || !radius || !coef))) return fail("fv_model_set_sponge: bad arguments");
+  for (int j = 0; j < n; ++j) {
+    if (nodes[j] < 1 || nodes[j] > md->m) return fail("fv_model_set_sponge: node out of range");
+    if (!(radius[j] > 0.0)) return fail("fv_model_set_sponge: radius must be positive");
+  }
+  for (int i = 1; i <= md->n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double dx = md->xc[i] - md->vx[nodes[j]], dy = md->yc[i] - md->vy[nodes[j]];
+      const double dtmp = std::sqrt(dx * dx + dy * dy) / radius[j];
+      if (dtmp <= 1.0) md->cc_sponge[i] = std::max(coef[j] * (1.0 - dtmp), md->cc_sponge[i]);
+    }
   return 0;
 }
 

@@ -33,7 +33,7 @@ module fvgpu
     integer(c_int32_t) :: wind
     real(c_double)     :: wind_mean, wind_std, wind_corr
     integer(c_int32_t) :: ts_kind
-    real(c_double)     :: el_pert_std, el_pert_corr, coriolis
+    real(c_double)     :: el_pert_std, el_pert_corr, coriolis, sponge_radius, sponge_coef
     integer(c_int64_t) :: seed
     type(fv_params)    :: p
   end type fv_case
