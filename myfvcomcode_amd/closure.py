@@ -2,7 +2,7 @@
 comparison with MY2.5.
 
     python -m myfvcomcode_amd.closure train  [--samples N] [--steps S] [--out FILE]
-    python -m myfvcomcode_amd.closure judge  [--params FILE] [--hours H] [--nx NX --ny NY]
+    python -m myfvcomcode_amd.closure judge  [--params FILE] [--case C] [--hours H] [--nx NX --ny NY]
 
 `train` runs the plain MY2.5 model (the 81x21 estuary channel and the
 jittered seamount mesh), dumps (features, KM, KH) samples from its own
@@ -135,6 +135,8 @@ def judge(params, case=1, hours=6.0, log=None, **overrides):
                 log("%s: %d / %d steps, %.1f s" % (name, done, nsteps, time.time() - t0))
         wall = time.time() - t0
         kb = mm.kb
+        if dev.diagnostics().nonfinite:
+            raise RuntimeError("the %s run became non-finite within %.1f simulated hours" % (name, hours))
         st = {f: dev.get(f) for f in ("el", "ua", "va", "u", "v", "t1", "s1")}
         st["ts"] = st["t1"].reshape(kb, -1)[0, 1:]
         st["ss"] = st["s1"].reshape(kb, -1)[0, 1:]
@@ -164,6 +166,7 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_PARAMS)
     ap.add_argument("--params", default=DEFAULT_PARAMS)
     ap.add_argument("--hours", type=float, default=24.0)
+    ap.add_argument("--case", type=int, default=1)
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     args = ap.parse_args(argv)
@@ -192,7 +195,8 @@ def main(argv=None):
             kw["nx"] = args.nx
         if args.ny:
             kw["ny"] = args.ny
-        print(json.dumps(judge(load_params(args.params), hours=args.hours, log=log, **kw)), flush=True)
+        print(json.dumps(judge(load_params(args.params), case=args.case, hours=args.hours, log=log, **kw)),
+              flush=True)
     return 0
 
 

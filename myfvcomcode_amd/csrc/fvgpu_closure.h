@@ -18,10 +18,11 @@
 //             (v_mfma_f32_32x32x8_bf16): one wavefront evaluates 32 samples
 //             per pass in the transposed form H^T = W X^T, so each layer's
 //             accumulators are already the next layer's B operand.
-//   fallback  a column whose predictions leave [1e-6, 1] m^2/s, or that
-//             predicts KH > KM where the gradient Richardson number
-//             N^2/S^2 exceeds 1 (stable stratification mixes heat no faster
-//             than momentum), takes the MY2.5 column solve instead.
+//   fallback  a column whose predictions leave [1e-6, 1] m^2/s, whose
+//             Prandtl number KM/KH leaves [0.1, 10] (the range of the MY2.5
+//             stability functions), or that predicts KH < KM in a statically
+//             unstable layer (N^2 < 0, where MY2.5 mixes heat at least as
+//             fast as momentum) takes the MY2.5 column solve instead.
 #pragma once
 #include "fvgpu_kernels_edge.h"
 
@@ -96,8 +97,8 @@ __global__ __launch_bounds__(256) void k_closure_features(DevGrid g, IntArgs a, 
       y[s * 2 + 0] = (float)a.km[q];
       y[s * 2 + 1] = (float)a.kh[q];
     }
-    // gradient Richardson number of the layer, kept for the fallback test
-    if (!y) a.gx[q] = n2 / (s2 > 1e-12 ? s2 : 1e-12);
+    // N^2 of the interface, kept for the fallback test
+    if (!y) a.gx[q] = n2;
   }
 }
 
@@ -386,9 +387,9 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restri
 // ------------------------------------------------------------ closure in the step
 // Realizability test per node column; accepted columns take the predicted
 // KM/KH (surface and bottom levels copy the nearest interface), the others
-// are flagged for the MY2.5 column solve.  Ri (the gradient Richardson
-// number per interface) was left in the node-layer scratch `ri`.
-__global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const double* ri, int* flag, int* nfall) {
+// are flagged for the MY2.5 column solve.  N^2 per interface was left in
+// the node-layer scratch `n2`.
+__global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const double* n2, int* flag, int* nfall) {
   const int i = (int)xcd_thread() + 1;
   if (i > g.M) return;
   bool ok = true;
@@ -396,7 +397,9 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
     const long s = (long)(k - 2) * g.M + (i - 1);
     const float km = Yp[2 * s], kh = Yp[2 * s + 1];
     if (!(km >= 1e-6f && km <= 1.0f && kh >= 1e-6f && kh <= 1.0f)) ok = false;
-    if (ri[NI(g, i, k)] > 1.0 && kh > km) ok = false;
+    const float pr = km / kh;
+    if (!(pr >= 0.1f && pr <= 10.0f)) ok = false;
+    if (n2[NI(g, i, k)] < 0.0 && kh < km) ok = false;
   }
   flag[i] = ok ? 0 : 1;
   if (!ok) {
