@@ -18,11 +18,15 @@
 //             (v_mfma_f32_32x32x8_bf16): one wavefront evaluates 32 samples
 //             per pass in the transposed form H^T = W X^T, so each layer's
 //             accumulators are already the next layer's B operand.
-//   fallback  a column whose predictions leave [1e-6, 1] m^2/s, whose
-//             Prandtl number KM/KH leaves [0.1, 10] (the range of the MY2.5
-//             stability functions), or that predicts KH < KM in a statically
-//             unstable layer (N^2 < 0, where MY2.5 mixes heat at least as
-//             fast as momentum) takes the MY2.5 column solve instead.
+//   fallback  predictions are realizable in [1e-6, 1] m^2/s: values below
+//             1e-6 are raised to it (they sit under the background UMOL = 1e-4
+//             the solves add, and the MY2.5 closure itself yields KM ~ 1e-8 in
+//             strongly stratified spin-up), while a column with a
+//             non-finite prediction, one above 1 m^2/s, a Prandtl number
+//             KM/KH outside [0.1, 10] (the range of the MY2.5 stability
+//             functions) or KH < KM in a statically unstable layer (N^2 < 0,
+//             where MY2.5 mixes heat at least as fast as momentum) takes the
+//             MY2.5 column solve instead.
 #pragma once
 #include "fvgpu_kernels_edge.h"
 
@@ -395,8 +399,8 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
   bool ok = true;
   for (int k = 2; k <= g.KBM1 && ok; ++k) {
     const long s = (long)(k - 2) * g.M + (i - 1);
-    const float km = Yp[2 * s], kh = Yp[2 * s + 1];
-    if (!(km >= 1e-6f && km <= 1.0f && kh >= 1e-6f && kh <= 1.0f)) ok = false;
+    const float km = fmaxf(Yp[2 * s], 1e-6f), kh = fmaxf(Yp[2 * s + 1], 1e-6f);
+    if (!(km <= 1.0f && kh <= 1.0f)) ok = false;   // also rejects NaN
     const float pr = km / kh;
     if (!(pr >= 0.1f && pr <= 10.0f)) ok = false;
     if (n2[NI(g, i, k)] < 0.0 && kh < km) ok = false;
@@ -410,7 +414,7 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
     const int kk = k < 2 ? 2 : (k > g.KBM1 ? g.KBM1 : k);
     const long s = (long)(kk - 2) * g.M + (i - 1);
     const long q = NI(g, i, k);
-    a.km[q] = (double)Yp[2 * s];
-    a.kh[q] = (double)Yp[2 * s + 1];
+    a.km[q] = (double)fmaxf(Yp[2 * s], 1e-6f);
+    a.kh[q] = (double)fmaxf(Yp[2 * s + 1], 1e-6f);
   }
 }
