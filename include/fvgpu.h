@@ -306,6 +306,14 @@ int  fvgpu_synchronize(fvgpu_handle* h);
 int  fvgpu_stream_copy(fvgpu_handle* h, int64_t n, int32_t reps, double* avg_us);
 void fvgpu_destroy(fvgpu_handle* h);
 
+/* Workgroup size (64, 128 or 256) of one kernel family of the step.      */
+int  fvgpu_set_block(fvgpu_handle* h, const char* family, int32_t block);
+/* Autotuner: times every kernel family of the step with workgroups of 64,
+ * 128 and 256 (`steps` internal steps per candidate), keeps the fastest,
+ * and restores the model state afterwards.  `report` receives
+ * "family=size:time;..." (may be NULL).                                  */
+int  fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len);
+
 /* ---------------------------- learned closure ---------------------------
  * A 5-64-64-2 MLP (SiLU hidden layers, softplus outputs) standing in for
  * the MY2.5 stability functions and length scale that set KM/KH at the end

@@ -119,6 +119,8 @@ def gpu_lib():
         L.fvgpu_kernel_stat.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
+        L.fvgpu_set_block.argtypes = [vp, C.c_char_p, C.c_int32]
+        L.fvgpu_autotune.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int64]
         fp = C.POINTER(C.c_float)
         L.fvgpu_set_closure.argtypes = [vp, C.c_int32, fp]
         L.fvgpu_closure_fallbacks.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -363,6 +365,23 @@ class DeviceModel:
             params = np.ascontiguousarray(params, np.float32)
             p = params.ctypes.data_as(C.POINTER(C.c_float))
         _check(L, L.fvgpu_set_closure(self._h, int(mode), p), "fvgpu_set_closure")
+
+    def autotune(self, steps=1):
+        """Pick the workgroup size of each kernel family by timing; the
+        state is restored.  Returns {family: (size, microseconds)}."""
+        L = gpu_lib()
+        buf = C.create_string_buffer(8192)
+        _check(L, L.fvgpu_autotune(self._h, int(steps), buf, len(buf)), "fvgpu_autotune")
+        out = {}
+        for item in buf.value.decode().split(";"):
+            if item:
+                f, rest = item.split("=")
+                size, us = rest.split(":")
+                out[f] = (int(size), float(us[:-2]))
+        return out
+
+    def set_block(self, family, size):
+        _check(gpu_lib(), gpu_lib().fvgpu_set_block(self._h, family.encode(), int(size)), "fvgpu_set_block")
 
     def closure_fallbacks(self):
         L = gpu_lib()

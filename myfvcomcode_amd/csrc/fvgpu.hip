@@ -72,6 +72,8 @@ struct fvgpu_handle {
   MlpPacked* cl_w = nullptr;   // BF16 fragments
   int* cl_flag = nullptr;      // per node: 1 = MY2.5 fallback
   int* cl_nfall = nullptr;     // fallback columns of the last step
+  std::map<std::string, int> tune;   // workgroup size per kernel family (autotuner)
+  std::vector<std::pair<void*, size_t>> rw;   // state buffers (saved around autotuning)
 };
 
 namespace {
@@ -97,6 +99,7 @@ int dupload(fvgpu_handle* h, const T** p, const T* src, size_t n) {
 
 int dupload_rw(fvgpu_handle* h, double** p, const double* src, size_t n) {
   if (dalloc(h, p, n)) return 1;
+  h->rw.push_back({(void*)*p, (n ? n : 1) * sizeof(double)});
   if (n && src) HIPCHK(hipMemcpy(*p, src, n * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
@@ -152,6 +155,20 @@ int flush_profile(fvgpu_handle* h) {
 
 inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 
+// Launch of a 1-D kernel family over n work items with the family's
+// workgroup size (256 unless the autotuner chose 64 or 128).
+int block_of(const fvgpu_handle* h, const char* name) {
+  auto it = h->tune.find(name);
+  return it == h->tune.end() ? 256 : it->second;
+}
+#define TLAUNCH(NAME, KERNEL, NITEMS, ...)                                                         \
+  do {                                                                                            \
+    const int bs_ = block_of(h, NAME);                                                            \
+    Launch l_(h, NAME);                                                                           \
+    hipLaunchKernelGGL(KERNEL, dim3((unsigned)(((long)(NITEMS) + bs_ - 1) / bs_)), dim3(bs_), 0, \
+                       h->stream, __VA_ARGS__);                                                   \
+  } while (0)
+
 const double ALPHA_RK[5] = {0.0, 0.2500, 1.0 / 3.0, 0.5000, 1.0};
 
 // ------------------------------------------------------------- the steps
@@ -187,9 +204,9 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext) {
   const Ext2DState* nxt[5] = {nullptr, &h->ext[1], &h->ext[2], &h->ext[1], &h->ext[0]};
   for (int K = 1; K <= 4; ++K) {
     ExtArgs a = make_ext_args(h, K, *cur[K], *nxt[K]);
-    LAUNCH("ext_elevation", k_ext_elevation, nb(g.M + 1), 256, g, a);
-    LAUNCH("advave_edge", k_advave_edge<FT>, nb(g.NE + 1), 256, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
-    LAUNCH("ext_momentum", k_ext_momentum<FT>, nb(g.N + 1), 256, g, a);
+    TLAUNCH("ext_elevation", k_ext_elevation, g.M + 1, g, a);
+    TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
+    TLAUNCH("ext_momentum", k_ext_momentum<FT>, g.N + 1, g, a);
   }
 }
 
@@ -199,28 +216,28 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   IntArgs& a = h->ia;
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
-  LAUNCH("adjust1_brough", k_adjust1_brough, nb(g.N + 1), 256, g, a);
-  LAUNCH("uvedge_adv", (k_uvedge<0, FT>), nb(g.NE + 1), 256, g, a, a.u, a.v);
-  LAUNCH("adv3d", k_adv3d_gather<FT>, nb(g.N + 1), 256, g, a);
-  LAUNCH("baropg", k_baropg, nb(g.N + 1), 256, g, a, (int)h->md->p.barotropic);
-  if (!h->md->p.barotropic) LAUNCH("rho1_roundtrip", k_rho1_roundtrip, nb(g.M + 1), 256, g, a);
-  LAUNCH("advave_edge", k_advave_edge<FT>, nb(g.NE + 1), 256, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
-  LAUNCH("advave_int", k_advave_int<FT>, nb(g.N + 1), 256, g, a);
-  LAUNCH("zero", k_zero, nb(g.m1), 256, a.egf, g.m1);
-  if (g.IOBCN > 0) LAUNCH("zero", k_zero, nb(g.IOBCN), 256, a.uard_obcn, (long)g.IOBCN);
+  TLAUNCH("adjust1_brough", k_adjust1_brough, g.N + 1, g, a);
+  TLAUNCH("uvedge_adv", (k_uvedge<0, FT>), g.NE + 1, g, a, a.u, a.v);
+  TLAUNCH("adv3d", k_adv3d_gather<FT>, g.N + 1, g, a);
+  TLAUNCH("baropg", k_baropg, g.N + 1, g, a, (int)h->md->p.barotropic);
+  if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, g.M + 1, g, a);
+  TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
+  TLAUNCH("advave_int", k_advave_int<FT>, g.N + 1, g, a);
+  TLAUNCH("zero", k_zero, g.m1, a.egf, g.m1);
+  if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext);
-  LAUNCH("adjust2", k_adjust2, nb(g.N + 1), 256, g, a);
-#define VERTVL_CALL(KM) LAUNCH("vertvl", k_vertvl_r<KM>, nb(g.M + 1), 256, g, a)
+  TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
+#define VERTVL_CALL(KM) TLAUNCH("vertvl", k_vertvl_r<KM>, g.M + 1, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VERTVL_CALL);
-  LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.wts, a.w, g.KB);
-#define VISCOFH_CALL(RM) LAUNCH("viscofh", k_viscofh_r<RM>, nb(g.M + 1), 256, g, a)
+  TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.wts, a.w, g.KB);
+#define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
-  LAUNCH("uvedge_uv", (k_uvedge<1, FT>), nb(g.NE + 1), 256, g, a, a.u, a.v);
-  LAUNCH("advuv", k_advuv_gather<FT>, nb(g.N + 1), 256, g, a);
+  TLAUNCH("uvedge_uv", (k_uvedge<1, FT>), g.NE + 1, g, a, a.u, a.v);
+  TLAUNCH("advuv", k_advuv_gather<FT>, g.N + 1, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
-  LAUNCH("uv_finish", k_uv_finish, nb((long)g.N * g.KBM1), 256, g, a);
-  LAUNCH("wreal", k_wreal, nb(g.N + 1), 256, g, a);
+  TLAUNCH("uv_finish", k_uv_finish, (long)g.N * g.KBM1, g, a);
+  TLAUNCH("wreal", k_wreal, g.N + 1, g, a);
   const double qmean1 = (double)1.E-8f;
   const int fct = h->md->p.scalar_positivity ? 1 : 0;
   if (h->md->p.vertical_closure && h->closure_mode == 1) {
@@ -228,38 +245,38 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     // MY2.5 column solve (without horizontal advection of Q2/Q2L) for the
     // rejected columns only
     const long ns = (long)g.M * (g.KBM1 - 1);
-#define CLF_CALL(RM) LAUNCH("closure_features", k_closure_features<RM>, nb(g.M), 256, g, a, h->cl_x, (float*)nullptr)
+#define CLF_CALL(RM) TLAUNCH("closure_features", k_closure_features<RM>, g.M, g, a, h->cl_x, (float*)nullptr)
     FV_RMAX_DISPATCH(g.ve_max, CLF_CALL);
     LAUNCH("closure_infer", k_mlp_infer_mfma, (unsigned)std::min<long>((ns + 31) / 32, 8192), 64, h->cl_w, h->cl_x, ns,
            h->cl_yp);
     LAUNCH("zero", k_zero_int, 1, 64, h->cl_nfall, 1);
-    LAUNCH("closure_apply", k_closure_apply, nb(g.M), 256, g, a, h->cl_yp, a.gx, h->cl_flag, h->cl_nfall);
-    LAUNCH("copy", k_copy, nb(mk), 256, a.q2f, a.q2, mk);
-    LAUNCH("copy", k_copy, nb(mk), 256, a.q2lf, a.q2l, mk);
-#define VDIF_QM_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx, h->cl_flag)
-#define VDIF_QM_CALL16(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), nb(g.M + 1), 256, g, a, a.gx, h->cl_flag)
+    TLAUNCH("closure_apply", k_closure_apply, g.M, g, a, h->cl_yp, a.gx, h->cl_flag, h->cl_nfall);
+    TLAUNCH("copy", k_copy, mk, a.q2f, a.q2, mk);
+    TLAUNCH("copy", k_copy, mk, a.q2lf, a.q2l, mk);
+#define VDIF_QM_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), g.M + 1, g, a, a.gx, h->cl_flag)
+#define VDIF_QM_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), g.M + 1, g, a, a.gx, h->cl_flag)
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL16);
-    LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
-    LAUNCH("copy", k_copy, nb(mk), 256, a.q2l, a.q2lf, mk);
+    TLAUNCH("copy", k_copy, mk, a.q2, a.q2f, mk);
+    TLAUNCH("copy", k_copy, mk, a.q2l, a.q2lf, mk);
   } else if (h->md->p.vertical_closure) {
     for (int which = 0; which < 2; ++which) {
       double* q = which ? a.q2l : a.q2;
       double* qf = which ? a.q2lf : a.q2f;
-#define CVREC_Q_CALL(RM) LAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), nb(g.M + 1), 256, g, a, q, nullptr, qmean1, 2, g.KBM1)
+#define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
       FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
-      LAUNCH("cvflux_q", k_cvflux_q<FT>, nb(g.NCV + 1), 256, g, a);
-#define ADV_Q_CALL(SM) LAUNCH("adv_q", (k_adv_q_gather<SM, FT>), nb(g.M + 1), 256, g, a, q, qf, fct)
+      TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.NCV + 1, g, a);
+#define ADV_Q_CALL(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, FT>), g.M + 1, g, a, q, qf, fct)
       FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
     }
-#define VDIF_Q_CALL(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), nb(g.M + 1), 256, g, a, a.gx, (const int*)nullptr)
-#define VDIF_Q_CALL16(KM) LAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), nb(g.M + 1), 256, g, a, a.gx, (const int*)nullptr)
+#define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), g.M + 1, g, a, a.gx, (const int*)nullptr)
+#define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), g.M + 1, g, a, a.gx, (const int*)nullptr)
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
-    LAUNCH("copy", k_copy, nb(mk), 256, a.q2, a.q2f, mk);
-    LAUNCH("copy", k_copy, nb(mk), 256, a.q2l, a.q2lf, mk);
+    TLAUNCH("copy", k_copy, mk, a.q2, a.q2f, mk);
+    TLAUNCH("copy", k_copy, mk, a.q2l, a.q2lf, mk);
   }
-  LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.km, a.km1, g.KB);
+  TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.km, a.km1, g.KB);
   for (int which = 1; which <= 2; ++which) {
     if (which == 1 && !h->md->p.temperature_active) continue;
     if (which == 2 && !h->md->p.salinity_active) continue;
@@ -267,25 +284,25 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     double* ff = which == 1 ? a.tf1 : a.sf1;
     const double* fm = which == 1 ? a.tmean1 : a.smean1;
     const double* fdis = which == 1 ? g.tdis : g.sdis;
-#define CVREC_TS_CALL(RM) LAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), nb(g.M + 1), 256, g, a, f1, fm, 0.0, 1, g.KBM1)
+#define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
     FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
-    LAUNCH("cvflux_ts", k_cvflux_ts<FT>, nb(g.NCV + 1), 256, g, a);
-#define ADV_TS_CALL(SM) LAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), nb(g.M + 1), 256, g, a, f1, ff, fdis, which, fct)
+    TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.NCV + 1, g, a);
+#define ADV_TS_CALL(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
     FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
-    if (g.IOBCN > 0) LAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, nb((long)g.IOBCN * g.KBM1), 256, g, a);
-#define VDIF_TS_CALL(KM) LAUNCH("vdif_ts", k_vdif_ts_r<KM>, nb(g.M + 1), 256, g, a, ff, which)
+    if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
+#define VDIF_TS_CALL(KM) TLAUNCH("vdif_ts", k_vdif_ts_r<KM>, g.M + 1, g, a, ff, which)
     FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
-    LAUNCH("bcond_ts", k_bcond_ts, nb(g.IOBCN + 1), 256, g, a, which);
-    LAUNCH("copy", k_copy, nb(mk), 256, f1, ff, mk);
-    LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, f1, which == 1 ? a.t : a.s, g.KB);
+    TLAUNCH("bcond_ts", k_bcond_ts, g.IOBCN + 1, g, a, which);
+    TLAUNCH("copy", k_copy, mk, f1, ff, mk);
+    TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, f1, which == 1 ? a.t : a.s, g.KB);
   }
   if (!h->md->p.barotropic) {
-    LAUNCH("dens2", k_dens2, nb(mk), 256, g, a);
-    LAUNCH("n2e3d", k_n2e3d, nb((long)g.N * g.KB), 256, g, a.rho1, a.rho, g.KB);
+    TLAUNCH("dens2", k_dens2, mk, g, a);
+    TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.rho1, a.rho, g.KB);
   }
-  LAUNCH("copy", k_copy, nb(nk), 256, a.u, a.uf, nk);
-  LAUNCH("copy", k_copy, nb(nk), 256, a.v, a.vf, nk);
-  LAUNCH("end_2d", k_end_2d, nb(std::max(g.M, g.N) + 1), 256, g, a);
+  TLAUNCH("copy", k_copy, nk, a.u, a.uf, nk);
+  TLAUNCH("copy", k_copy, nk, a.v, a.vf, nk);
+  TLAUNCH("end_2d", k_end_2d, std::max(g.M, g.N) + 1, g, a);
 }
 
 void enqueue_external_step(fvgpu_handle* h, int iext) {
@@ -1050,6 +1067,88 @@ void fvmlp_destroy(fvmlp* t) {
   for (void* p : t->allocs) hipFree(p);
   if (t->stream) hipStreamDestroy(t->stream);
   delete t;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ autotuner
+extern "C" {
+
+int fvgpu_set_block(fvgpu_handle* h, const char* family, int32_t block) {
+  if (block != 64 && block != 128 && block != 256) return fail("workgroup size must be 64, 128 or 256");
+  h->tune[family] = block;
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  return 0;
+}
+
+int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
+  static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
+                               "cvflux_q", "adv_ts", "adv_q", "uvedge_adv", "uvedge_uv", "adv3d", "advuv", "vertvl",
+                               "viscofh", "vdif_q", "vdif_ts", "n2e3d", "adjust1_brough", "adjust2", "baropg",
+                               "wreal", "uv_finish", "copy", "closure_features", "closure_apply"};
+  static const int cands[] = {64, 128, 256};
+  if (steps < 1) return fail("autotune needs at least one step per candidate");
+  // save the state; every candidate runs real steps
+  size_t tot = 0;
+  for (auto& b : h->rw) tot += (b.second + 255) & ~(size_t)255;
+  char* arena = nullptr;
+  HIPCHK(hipMalloc((void**)&arena, tot + sizeof(DevClock) + 256));
+  size_t off = 0;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (auto& b : h->rw) {
+    HIPCHK(hipMemcpyAsync(arena + off, b.first, b.second, hipMemcpyDeviceToDevice, h->stream));
+    off += (b.second + 255) & ~(size_t)255;
+  }
+  HIPCHK(hipMemcpyAsync(arena + off, h->clk, sizeof(DevClock), hipMemcpyDeviceToDevice, h->stream));
+  const bool prof = h->profiling;
+  const std::string only = h->profile_only;
+  std::string rep;
+  int rc = 0;
+  for (const char* f : fams) {
+    double best = 1e30;
+    int bestc = 256;
+    bool seen = false;
+    for (int c : cands) {
+      h->tune[f] = c;
+      h->profiling = true;
+      h->profile_only = f;
+      h->stats.clear();
+      for (int st = 0; st < steps && rc == 0; ++st) enqueue_internal_step(h);
+      if (rc == 0) rc = flush_profile(h);
+      auto it = h->stats.find(f);
+      if (it == h->stats.end() || it->second.count == 0) break;   // family not in this step
+      seen = true;
+      const double t = it->second.total_ms / it->second.count;
+      if (t < best) { best = t; bestc = c; }
+    }
+    if (seen) {
+      h->tune[f] = bestc;
+      char buf[96];
+      std::snprintf(buf, sizeof(buf), "%s%s=%d:%.1fus", rep.empty() ? "" : ";", f, bestc, 1000.0 * best);
+      rep += buf;
+    } else {
+      h->tune.erase(f);
+    }
+  }
+  h->profiling = prof;
+  h->profile_only = only;
+  h->stats.clear();
+  off = 0;
+  for (auto& b : h->rw) {
+    HIPCHK(hipMemcpyAsync(b.first, arena + off, b.second, hipMemcpyDeviceToDevice, h->stream));
+    off += (b.second + 255) & ~(size_t)255;
+  }
+  HIPCHK(hipMemcpyAsync(h->clk, arena + off, sizeof(DevClock), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipFree(arena));
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  if (report && report_len > 0) {
+    std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return rc;
 }
 
 }  // extern "C"

@@ -1,0 +1,24 @@
+"""The autotuner (fvgpu_autotune) only changes workgroup sizes: the model
+state is restored after tuning and the tuned step produces bit-identical
+results to the untuned one."""
+import numpy as np
+import pytest
+
+import myfvcomcode_amd as F
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_keeps_results_bit_identical(built):
+    m = F.Model.build(1, nx=61, ny=31, kb=9, n_seamounts=2)
+    print("case: jittered mesh 61x31 KB=9, 3 + autotune + 3 steps vs 6 steps", flush=True)
+    a = F.DeviceModel(m)
+    a.run(3, graph=False)
+    rep = a.autotune(1)
+    assert rep and all(size in (64, 128, 256) for size, _ in rep.values())
+    a.run(3, graph=True)
+    b = F.DeviceModel(m)
+    b.run(6, graph=True)
+    for name in ("el", "ua", "va", "u", "v", "w", "t1", "s1", "q2", "km", "kh", "wts"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    assert a.diagnostics().iint == b.diagnostics().iint == 6
