@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--kb", type=int, default=41)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip timing the mixed-precision and learned-closure forms")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a plain device copy of 2^25 doubles (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -100,14 +102,25 @@ def main():
     # dominant kernel = the modelled family with the largest device time per step
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
     pk = per_kernel[dom]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "%s_traffic.json" % dom)
-    if os.path.exists(tfile):
+    # HBM bytes per launch from the calibrated FETCH_SIZE/WRITE_SIZE runs of
+    # this workload (rocprofv3 --pmc), when the default workload is timed
+    traffic_tab = {}
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_1001x501x41_fp64.json")
+    if os.path.exists(tfile) and (args.nx, args.ny, args.kb) == (1001, 501, 41):
         try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch")
+            traffic_tab = json.load(open(tfile)).get("families", {})
         except (OSError, ValueError):
-            traffic = None
+            traffic_tab = {}
+    for name, rec in per_kernel.items():
+        if name in traffic_tab:
+            rec["traffic_bytes"] = traffic_tab[name]["bytes_per_launch"]
+    traffic = traffic_tab.get(dom, {}).get("bytes_per_launch")
     step_kernel_ms = sum(v[0] * v[1] for v in stats.values()) / 1000.0 / args.steps
+
+    variants = None
+    if not args.no_variants:
+        del dev
+        variants = time_variants(F, model, args.steps, args.warmup, wall / args.steps, dti, n * kbm1)
 
     cpu = None
     if not args.no_cpu:
@@ -139,9 +152,36 @@ def main():
                           "frac": round(tot_b / tot_t / PEAK_HBM, 4) if tot_t else None,
                           "kernel_ms_per_step": round(step_kernel_ms, 3)},
         "roofline_by_kernel": per_kernel,
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def time_variants(F, model, steps, warmup, fp64_step_s, dti, units):
+    """The mixed-precision form, and mixed precision with the learned
+    closure, on the same workload (graph replays, same step count)."""
+    from myfvcomcode_amd.closure import load_params
+    out = {}
+    for name, closure in (("mixed", False), ("mixed_learned_closure", True)):
+        dev = F.DeviceModel(model, precision="mixed")
+        if closure:
+            dev.set_closure(1, load_params())
+        dev.run(warmup, graph=True)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        dev.run(steps, graph=True)
+        dev.synchronize()
+        w = (time.perf_counter() - t0) / steps
+        if dev.diagnostics().nonfinite:
+            raise RuntimeError("bench: the %s form became non-finite" % name)
+        out[name] = {"ms_per_step": 1000.0 * w, "value": units / w * dti / (24.0 * units),
+                     "speedup_vs_fp64": fp64_step_s / w}
+        if closure:
+            out[name]["fallback_columns_last_step"] = dev.closure_fallbacks()[0]
+        print("bench: %s %.2f ms/step" % (name, 1000.0 * w), file=sys.stderr, flush=True)
+        del dev
+    return out
 
 
 def cpu_baseline(steps, kb):
