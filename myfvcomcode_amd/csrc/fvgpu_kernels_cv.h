@@ -342,8 +342,8 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
   }
 }
 
-// VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer with the
-// node's sub-edges outermost and the layer sums in registers, then the
+// VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer (sums over
+// the node's sub-edges in sub-edge order, kept in registers), then the
 // column integration of omega, the linear mass-error correction and DTFA.
 template <int KMAX>
 __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
@@ -355,22 +355,25 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
     return;
   }
   const int KBM1 = g.KBM1;
+  // layers outermost: one layer of the elements around a wavefront's nodes
+  // is gathered at a time (the slot lists are re-read from cache), so each
+  // element-layer value crosses HBM once
   double acc[KMAX];
-  static_for<KMAX>([&](auto kc) { acc[decltype(kc)::value] = 0.0; });
   const int n = g.cv_cnt[i];
-  for (int j = 0; j < n; ++j) {
-    const long q = (long)j * g.m1 + i;
-    const int i1 = g.cvl_elem[q];
-    const double sdx = g.cvl_sdx[q], sdy = g.cvl_sdy[q], dt1 = a.dt1[i1];
-    static_for<KMAX>([&](auto kc) {
-      constexpr int k = decltype(kc)::value + 1;
-      if (k <= KBM1) {
-        const long ko = (long)(k - 1) * g.n1;
-        const double dij = dt1 * DZ1(i1, k);
-        acc[k - 1] = acc[k - 1] + dij * (-a.u[ko + i1] * sdy + a.v[ko + i1] * sdx);
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    double x = 0.0;
+    if (k <= KBM1) {
+      const long ko = (long)(k - 1) * g.n1;
+      for (int j = 0; j < n; ++j) {
+        const long q = (long)j * g.m1 + i;
+        const int i1 = g.cvl_elem[q];
+        const double dij = a.dt1[i1] * DZ1(i1, k);
+        x = x + dij * (-a.u[ko + i1] * g.cvl_sdy[q] + a.v[ko + i1] * g.cvl_sdx[q]);
       }
-    });
-  }
+    }
+    acc[k - 1] = x;
+  });
   const bool obc = g.isonb[i] == 2;
   const int rv = g.node_riv[i];
   const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
