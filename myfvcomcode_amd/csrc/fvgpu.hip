@@ -253,12 +253,12 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     TLAUNCH("closure_apply", k_closure_apply, g.M, g, a, h->cl_yp, a.gx, h->cl_flag, h->cl_nfall);
     TLAUNCH("copy", k_copy, mk, a.q2f, a.q2, mk);
     TLAUNCH("copy", k_copy, mk, a.q2lf, a.q2l, mk);
-#define VDIF_QM_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), g.M + 1, g, a, a.gx, h->cl_flag)
-#define VDIF_QM_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), g.M + 1, g, a, a.gx, h->cl_flag)
+#define VDIF_QM_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, h->cl_flag)
+#define VDIF_QM_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, h->cl_flag)
+    // (vdif_q also stores Q2 = Q2F, Q2L = Q2LF for its columns; the others
+    // hold Q2F = Q2 from the copies above)
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL16);
-    TLAUNCH("copy", k_copy, mk, a.q2, a.q2f, mk);
-    TLAUNCH("copy", k_copy, mk, a.q2l, a.q2lf, mk);
   } else if (h->md->p.vertical_closure) {
     for (int which = 0; which < 2; ++which) {
       double* q = which ? a.q2l : a.q2;
@@ -269,12 +269,13 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
 #define ADV_Q_CALL(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, FT>), g.M + 1, g, a, q, qf, fct)
       FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
     }
-#define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10>), g.M + 1, g, a, a.gx, (const int*)nullptr)
-#define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16>), g.M + 1, g, a, a.gx, (const int*)nullptr)
+#define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
+#define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
+    // VDIF_Q, then Q2 = Q2F and Q2L = Q2LF (internal_step.F:1029-1035) in
+    // the same kernel.  `rlohi` is free here (ADV_Q is done, ADV_T/ADV_S
+    // rebuild it) and holds the elimination coefficients.
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
-    TLAUNCH("copy", k_copy, mk, a.q2, a.q2f, mk);
-    TLAUNCH("copy", k_copy, mk, a.q2l, a.q2lf, mk);
   }
   TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.km, a.km1, g.KB);
   for (int which = 1; which <= 2; ++which) {

@@ -10,6 +10,12 @@
 #include <utility>
 
 #define FV_MAXKB 64
+// VDIF_Q keeps the elimination coefficients of its two solves in registers
+// (1) or in a node-layer scratch pair (0: unrolled level loops, 2: rolled),
+// trading VGPRs for occupancy.
+#ifndef FV_VDIFQ_VHREG
+#define FV_VDIFQ_VHREG 0
+#endif
 #define FV_MAX_CV 32   // control-volume sub-edges per node (twice the node degree)
 #define FV_MAX_RING 16 // NTSN / NTVE bound of the node rings held in registers
 

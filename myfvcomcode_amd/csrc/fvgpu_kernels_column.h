@@ -133,22 +133,22 @@ __global__ __launch_bounds__(256) void k_vdif_ts_r(DevGrid g, IntArgs a, double*
   });
 }
 
-// Per-level quantities of VDIF_Q that are recomputed instead of stored.
+// Per-level quantities of VDIF_Q (vdif_q.F:255-368): from Q2, Q2L and RHO1
+// at levels k-1 and k of node i (the values passed in).
 struct Q25Level { double q2, boy, ll, gh, stf, dtef; };
 
-__device__ __forceinline__ Q25Level q25_level(const DevGrid& g, const IntArgs& a, int i, int k, double d,
-                                              double l0) {
+__device__ __forceinline__ Q25Level q25_eval(const DevGrid& g, int i, int k, double d, double l0, double q2raw,
+                                             double q2l, double rho_km1, double rho_k) {
   const double SMALL = 1.E-8, KAPPA = 0.40, GH_MAX = .0233, GHC = -6.0, GH_MIN = -.281, B1 = 16.6;
   const double p9 = (double)0.9f, p1 = (double)0.1f;
   Q25Level r;
-  const long q = NI(g, i, k);
-  r.q2 = fabs(a.q2[q] + SMALL);
+  r.q2 = fabs(q2raw + SMALL);
   r.boy = 0.0;
-  if (k >= 2 && k <= g.KBM1) r.boy = g.grav_n[i] * (a.rho1[q - g.m1] - a.rho1[q]) / (DZZ(i, k - 1) * d);
+  if (k >= 2 && k <= g.KBM1) r.boy = g.grav_n[i] * (rho_km1 - rho_k) / (DZZ(i, k - 1) * d);
   if (k == 1) r.ll = KAPPA * l0;
   else if (k == g.KB) r.ll = 0.0;
   else {
-    double ll = fabs(a.q2l[q]) / r.q2;
+    double ll = fabs(q2l) / r.q2;
     if (ZN(i, k) > -0.5) ll = fmax(ll, KAPPA * l0);
     if (r.boy < 0.0) {
       const double lmax = sqrt(fabs(GH_MIN) * r.q2 / (-r.boy + SMALL));
@@ -165,10 +165,24 @@ __device__ __forceinline__ Q25Level q25_level(const DevGrid& g, const IntArgs& a
   return r;
 }
 
-// VDIF_Q, MY2.5 (vdif_q.F:34-551).  PROD (which needs ring averages of U/V)
-// is formed once per level into the node-layer scratch array `prod`.
-template <int KMAX, int RMAX>
-__global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* prod, const int* only) {
+// VDIF_Q, MY2.5 (vdif_q.F:34-551), one thread per node column in two
+// sweeps.  The downward sweep k = 1..KB reads every node-layer value once
+// (sliding windows over RHO1 and KQ, the ring averages of U/V): it forms the
+// level quantities (vdif_q.F:255-368) and PROD (vdif_q.F:259-302), runs the
+// forward elimination of both the Q2 solve (vdif_q.F:372-378) and the Q2L
+// solve (vdif_q.F:398-417), and updates KQ/KM/KH and L, which depend on the
+// old Q2/Q2L only (vdif_q.F:466-487; L(1)=KAPPA*L0, L(KB)=0).  The
+// elimination coefficients VH of both solves go to registers (VHMODE 1,
+// loops unrolled) or to the node-layer pair scratch `vhs` (VHMODE 0 with
+// unrolled loops, 2 with rolled ones); VHP(k) is parked in Q2F(k)/Q2LF(k),
+// which the elimination has just consumed.  The upward
+// sweep does both back-substitutions (vdif_q.F:380-391, 420-431) and the
+// threshold (vdif_q.F:440-448), and stores each final value both to
+// Q2F/Q2LF and to Q2/Q2L, the copy that follows VDIF_Q in INTERNAL_STEP
+// (internal_step.F:1034-1035); column 0 is copied by its thread.  Every value is formed by the reference's
+// operations in its order; only the loops are fused.
+template <int KMAX, int RMAX, int VHMODE>
+__global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double2* vhs, const int* only) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
   if (only && !only[i]) return;   // column subset (the learned closure's fallback)
@@ -186,14 +200,13 @@ __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* 
       KM[NI(g, 0, k)] = 0.5 * (kn0 * 0.0 + KM[NI(g, 0, k)]);
       KH[NI(g, 0, k)] = 0.5 * (kn0 * 0.0 * g.vprnu + KH[NI(g, 0, k)]);
     }
+    for (int k = 1; k <= KB; ++k) {
+      a.q2[NI(g, 0, k)] = Q2F[NI(g, 0, k)];
+      a.q2l[NI(g, 0, k)] = Q2LF[NI(g, 0, k)];
+    }
     return;
   }
   const double dti = g.dti, umol = g.umol, d = a.d[i];
-  auto clampq = [&](int k) {
-    const long q = (long)(k - 1) * m1 + i;
-    if (Q2F[q] <= SMALL || Q2LF[q] <= SMALL) { Q2F[q] = SMALL; Q2LF[q] = SMALL; }
-  };
-  for (int k = 2; k <= KBM1; ++k) clampq(k);
   const int o = g.nbve_off[i], nt = g.ntve[i];
   const double ramp = a.clk->ramp_int;
   double wus = 0.0, wvs = 0.0, wub = 0.0, wvb = 0.0;
@@ -212,9 +225,8 @@ __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* 
     wvb = wvb + a.wvbot[e];
   }
   wub = wub / ntd; wvb = wvb / ntd;
-  Q2F[NI(g, i, KB)] = sqrt(sq(wub) + sq(wvb)) * g.const1;
-  // PROD: buoyancy part times zero, shear and buoyancy production.  The
-  // ring averages of U/V at layer k-1 are those of the previous level.
+  const double q2f_kb = sqrt(sq(wub) + sq(wvb)) * g.const1;
+  Q2F[NI(g, i, KB)] = q2f_kb;
   int re[RMAX];
   static_for<RMAX>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
@@ -228,97 +240,133 @@ __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double* 
     });
     return x / ntd;
   };
-  double ukm1 = 0.0, vkm1 = 0.0;
-  if (d > 0.0) { ukm1 = ring_avg(a.u, 0); vkm1 = ring_avg(a.v, 0); }
-  for (int k = 2; k <= KBM1; ++k) {
-    const long q = NI(g, i, k);
-    const double boy = g.grav_n[i] * (a.rho1[q - m1] - a.rho1[q]) / (DZZ(i, k - 1) * d);
-    double pr = KM[q] * 0.0 * fv_pow15(.5 * (-boy + fabs(boy)));
-    if (d > 0.0) {
-      const long eo = (long)(k - 1) * g.n1;
-      const double uk = ring_avg(a.u, eo), vk = ring_avg(a.v, eo);
-      pr = pr + KM[q] * SEF * (sq(uk - ukm1) + sq(vk - vkm1)) / sq(DZZ(i, k - 1) * d);
-      pr = pr + KH[q] * boy;
-      ukm1 = uk;
-      vkm1 = vk;
-    }
-    prod[q] = pr;
-  }
-  // first solve: Q2 (vdif_q.F:372-391)
-  double VH[KMAX + 1], VHP[KMAX + 1];
-  VH[1] = 0.0;
-  VHP[1] = g.cvhp1 * utau2;
-  static_for<KMAX - 1>([&](auto kc) {
-    constexpr int k = decltype(kc)::value + 2;
-    if (k <= KBM1) {
-      const long q = (long)(k - 1) * m1 + i;
-      const double ak = -0.5 * dti * (KQ[q + m1] + KQ[q] + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k) * d * d);
-      const double ck = -0.5 * dti * (KQ[q - m1] + KQ[q] + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k - 1) * d * d);
-      const Q25Level lv = q25_level(g, a, i, k, d, l0);
-      const double t = 1. / (ak + ck * (1. - VH[k - 1]) - (2. * dti * lv.dtef + 1.));
-      VH[k] = ak * t;
-      VHP[k] = (-2. * dti * prod[q] + ck * VHP[k - 1] - Q2F[q]) * t;
-    }
-  });
-  {
-    double x = Q2F[NI(g, i, KB)];
-    static_for<KMAX>([&](auto kc) {
-      constexpr int k = KMAX - decltype(kc)::value;
-      if (k <= KBM1) {
-        x = VH[k] * x + VHP[k];
-        Q2F[(long)(k - 1) * m1 + i] = x;
-      }
-    });
-  }
-  // second solve: Q2L with the wall function (vdif_q.F:398-431)
-  VH[1] = 0.0;
-  VHP[1] = 0.0;
-  const double z1 = ZN(i, 1), zkb = ZN(i, KB);
-  static_for<KMAX - 1>([&](auto kc) {
-    constexpr int k = decltype(kc)::value + 2;
-    if (k <= KBM1) {
-      if (d > 0.0) {
-        const long q = (long)(k - 1) * m1 + i;
-        const double ak = -0.5 * dti * (KQ[q + m1] + KQ[q] + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k) * d * d);
-        const double ck = -0.5 * dti * (KQ[q - m1] + KQ[q] + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k - 1) * d * d);
-        const Q25Level lv = q25_level(g, a, i, k, d, l0);
-        const double zk = ZN(i, k);
-        const double dtef = lv.dtef * (1. + E2 * sq((1. / fabs(zk - z1) + 1. / fabs(zk - zkb)) * lv.ll / (d * KAPPA)));
-        const double t = 1. / (ak + ck * (1. - VH[k - 1]) - (dti * dtef + 1.));
-        VH[k] = ak * t;
-        VHP[k] = (dti * (-prod[q] * lv.ll * E1) + ck * VHP[k - 1] - Q2LF[q]) * t;
-      }  // D <= 0: VH/VHP keep the first solve's values, as in the reference
-    }
-  });
-  {
-    double x = Q2LF[NI(g, i, KB)];
-    static_for<KMAX>([&](auto kc) {
-      constexpr int k = KMAX - decltype(kc)::value;
-      if (k <= KBM1) {
-        x = VH[k] * x + VHP[k];
-        Q2LF[(long)(k - 1) * m1 + i] = x;
-      }
-    });
-  }
-  for (int k = 2; k <= KBM1; ++k) clampq(k);
-  // stability functions and the under-relaxed mixing coefficients
   const double coef4 = 18.0 * A1 * A1 + 9.0 * A1 * A2, coef5 = 9.0 * A1 * A2;
-  for (int k = 1; k <= KB; ++k) {
-    const Q25Level lv = q25_level(g, a, i, k, d, l0);
+  // stability functions and the under-relaxed mixing coefficients at level k
+  auto mix = [&](long q, const Q25Level& lv, double kq_old) {
     const double coef1 = A2 * (1.0 - 6.0 * A1 / B1 * lv.stf);
     const double coef2 = 3.0 * A2 * B2 / lv.stf + 18.0 * A1 * A2;
     const double coef3 = A1 * (1.0 - 3.0 * C1 - 6.0 * A1 / B1 * lv.stf);
     const double sh = coef1 / (1.0 - coef2 * lv.gh);
     const double sm = (coef3 + sh * coef4 * lv.gh) / (1. - coef5 * lv.gh);
     const double kn = lv.ll * sqrt(lv.q2);
-    const long q = NI(g, i, k);
-    KQ[q] = 0.5 * (kn * KAPPA * sm + KQ[q]);
+    KQ[q] = 0.5 * (kn * KAPPA * sm + kq_old);
     KM[q] = 0.5 * (kn * sm + KM[q]);
     KH[q] = 0.5 * (kn * sh * g.vprnu + KH[q]);
+    L[q] = lv.ll;
+  };
+  const double z1 = ZN(i, 1), zkb = ZN(i, KB);
+  constexpr int NVH = VHMODE == 1 ? KMAX + 1 : 1;
+  double VH1[NVH], VH2[NVH];
+  double vh1p = 0.0, vh2p = 0.0;             // VH(k-1) of the two solves
+  const double vhp1_1 = g.cvhp1 * utau2;
+  double vhp1 = vhp1_1, vhp2 = 0.0;          // VHP(k-1) of the two solves
+  // level 1
+  double rho_km1 = a.rho1[NI(g, i, 1)];
+  double kq_km1 = KQ[NI(g, i, 1)];           // old KQ(k-1)
+  double kq_k = KQ[NI(g, i, 2)];             // old KQ(k)
+  {
+    const long q = NI(g, i, 1);
+    const Q25Level lv = q25_eval(g, i, 1, d, l0, a.q2[q], a.q2l[q], 0.0, 0.0);
+    mix(q, lv, kq_km1);
   }
-  // L last: the level values above read the old L only at k = 1 (GH(1) is 0)
-  for (int k = 1; k <= KB; ++k) {
-    const Q25Level lv = q25_level(g, a, i, k, d, l0);
-    L[NI(g, i, k)] = lv.ll;
+  double ukm1 = 0.0, vkm1 = 0.0;
+  if (d > 0.0) { ukm1 = ring_avg(a.u, 0); vkm1 = ring_avg(a.v, 0); }
+  auto fwd = [&](const int k) __attribute__((always_inline)) {
+    {
+      const long q = (long)(k - 1) * m1 + i;
+      const double rho_k = a.rho1[q];
+      const double kq_kp1 = KQ[q + m1];
+      const Q25Level lv = q25_eval(g, i, k, d, l0, a.q2[q], a.q2l[q], rho_km1, rho_k);
+      // PROD (vdif_q.F:259-302)
+      const double boy = lv.boy;
+      const double kmq = KM[q];
+      double pr = kmq * 0.0 * fv_pow15(.5 * (-boy + fabs(boy)));
+      if (d > 0.0) {
+        const long eo = (long)(k - 1) * g.n1;
+        const double uk = ring_avg(a.u, eo), vk = ring_avg(a.v, eo);
+        pr = pr + kmq * SEF * (sq(uk - ukm1) + sq(vk - vkm1)) / sq(DZZ(i, k - 1) * d);
+        pr = pr + KH[q] * boy;
+        ukm1 = uk;
+        vkm1 = vk;
+      }
+      // the threshold before the solves (vdif_q.F:136-145): both to SMALL
+      // if either is at or below it
+      double q2f = Q2F[q], q2lf = Q2LF[q];
+      if (q2f <= SMALL || q2lf <= SMALL) { q2f = SMALL; q2lf = SMALL; }
+      const double ak = -0.5 * dti * (kq_kp1 + kq_k + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k) * d * d);
+      const double ck = -0.5 * dti * (kq_km1 + kq_k + 2.0 * umol) / (DZZ(i, k - 1) * DZ(i, k - 1) * d * d);
+      const double t1 = 1. / (ak + ck * (1. - vh1p) - (2. * dti * lv.dtef + 1.));
+      const double vh1 = ak * t1;
+      const double p1 = (-2. * dti * pr + ck * vhp1 - q2f) * t1;
+      double vh2, p2;
+      if (d > 0.0) {
+        const double zk = ZN(i, k);
+        const double dtef = lv.dtef * (1. + E2 * sq((1. / fabs(zk - z1) + 1. / fabs(zk - zkb)) * lv.ll / (d * KAPPA)));
+        const double t2 = 1. / (ak + ck * (1. - vh2p) - (dti * dtef + 1.));
+        vh2 = ak * t2;
+        p2 = (dti * (-pr * lv.ll * E1) + ck * vhp2 - q2lf) * t2;
+      } else {              // D <= 0: VH/VHP keep the first solve's values
+        vh2 = vh1;
+        p2 = p1;
+      }
+      if constexpr (VHMODE == 1) { VH1[k] = vh1; VH2[k] = vh2; }
+      else vhs[q] = make_double2(vh1, vh2);
+      Q2F[q] = p1;
+      Q2LF[q] = p2;
+      vh1p = vh1;
+      vh2p = vh2;
+      vhp1 = p1;
+      vhp2 = p2;
+      mix(q, lv, kq_k);
+      rho_km1 = rho_k;
+      kq_km1 = kq_k;
+      kq_k = kq_kp1;
+    }
+  };
+  if constexpr (VHMODE != 2) {
+    static_for<KMAX - 1>([&](auto kc) {
+      constexpr int k = decltype(kc)::value + 2;
+      if (k <= KBM1) fwd(k);
+    });
+  } else {
+    for (int k = 2; k <= KBM1; ++k) fwd(k);
+  }
+  {
+    // level KB (kq_k is the old KQ(KB) here)
+    const long q = NI(g, i, KB);
+    const Q25Level lv = q25_eval(g, i, KB, d, l0, a.q2[q], a.q2l[q], 0.0, 0.0);
+    mix(q, lv, kq_k);
+  }
+  // upward sweep: both back-substitutions, then the threshold on each level
+  double x1 = q2f_kb, x2 = Q2LF[NI(g, i, KB)];
+  a.q2[NI(g, i, KB)] = x1;
+  a.q2l[NI(g, i, KB)] = x2;
+  auto bwd = [&](const int k) __attribute__((always_inline)) {
+    {
+      const long q = (long)(k - 1) * m1 + i;
+      double v1 = 0.0, v2 = 0.0, p1 = vhp1_1, p2 = 0.0;   // VH(1) = 0, VHP(1) of the two solves
+      if (k >= 2) {
+        if constexpr (VHMODE == 1) { v1 = VH1[k]; v2 = VH2[k]; }
+        else { const double2 w = vhs[q]; v1 = w.x; v2 = w.y; }
+        p1 = Q2F[q];
+        p2 = Q2LF[q];
+      }
+      x1 = v1 * x1 + p1;
+      x2 = v2 * x2 + p2;
+      double y1 = x1, y2 = x2;
+      if (k >= 2 && (x1 <= SMALL || x2 <= SMALL)) { y1 = SMALL; y2 = SMALL; }
+      Q2F[q] = y1;
+      Q2LF[q] = y2;
+      a.q2[q] = y1;
+      a.q2l[q] = y2;
+    }
+  };
+  if constexpr (VHMODE != 2) {
+    static_for<KMAX>([&](auto kc) {
+      constexpr int k = KMAX - decltype(kc)::value;
+      if (k <= KBM1) bwd(k);
+    });
+  } else {
+    for (int k = KBM1; k >= 1; --k) bwd(k);
   }
 }

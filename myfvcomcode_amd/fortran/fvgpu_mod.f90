@@ -109,6 +109,60 @@ module fvgpu
       import :: c_ptr
       type(c_ptr), value :: h
     end subroutine
+    ! a mesh handed over by the host model (VX, VY, NV, H, I_OBC_N)
+    function fv_model_from_mesh(m, n, x, y, nv_rows, h, kb, iobcn, obc_nodes, p, mdl) &
+         bind(c, name="fv_model_from_mesh") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr, fv_params
+      integer(c_int32_t), value :: m, n, kb, iobcn
+      real(c_double) :: x(*), y(*), h(*)
+      integer(c_int32_t) :: nv_rows(*), obc_nodes(*)
+      type(fv_params) :: p
+      type(c_ptr) :: mdl
+      integer(c_int) :: rc
+    end function
+    function fv_model_set_tide(mdl, ntide, period, amp, phase_deg) bind(c, name="fv_model_set_tide") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: mdl
+      integer(c_int32_t), value :: ntide
+      real(c_double) :: period(*), amp(*), phase_deg(*)
+      integer(c_int) :: rc
+    end function
+    function fv_model_set_river(mdl, node, q, t, s) bind(c, name="fv_model_set_river") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: mdl
+      integer(c_int32_t), value :: node
+      real(c_double), value :: q, t, s
+      integer(c_int) :: rc
+    end function
+    function fv_model_set_sponge(mdl, n, nodes, radius, coef) bind(c, name="fv_model_set_sponge") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: mdl
+      integer(c_int32_t), value :: n
+      integer(c_int32_t) :: nodes(*)
+      real(c_double) :: radius(*), coef(*)
+      integer(c_int) :: rc
+    end function
+    function fv_model_cold_start(mdl) bind(c, name="fv_model_cold_start") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mdl
+      integer(c_int) :: rc
+    end function
+    function fvgpu_put_state(h, name, host, n) bind(c, name="fvgpu_put_state") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_char, c_double
+      type(c_ptr), value :: h
+      character(kind=c_char) :: name(*)
+      real(c_double) :: host(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+    function fvgpu_autotune(h, steps, report, report_len) bind(c, name="fvgpu_autotune") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: steps
+      type(c_ptr), value :: report
+      integer(c_int64_t), value :: report_len
+      integer(c_int) :: rc
+    end function
     function fv_last_error() bind(c, name="fv_last_error") result(p)
       import :: c_ptr
       type(c_ptr) :: p
@@ -156,6 +210,33 @@ contains
     if (present(kb)) c%kb = kb
     call fvgpu_check(fv_model_build(c, fvgpu_model), "fv_model_build")
     call fvgpu_check(fvgpu_create(fvgpu_model, 0_c_int32_t, fvgpu_h), "fvgpu_create")
+  end subroutine
+
+  ! The host model's own mesh (after GRID_METRICS/STARTUP, fvcom.F:629-753):
+  ! nodes VX/VY/H (1:M), element triples NV(1:N,1:3) clockwise, the OBC node
+  ! list; cold start on the device model, then upload.  precision: 0 FP64,
+  ! 1 mixed.  Forcing (fv_model_set_tide/_river/_sponge) goes between
+  ! fv_model_from_mesh and fvgpu_create, so this variant sets none.
+  subroutine fvgpu_setup_mesh(m, n, vx, vy, nv, h, kb, obc_nodes, p, precision)
+    integer, intent(in) :: m, n, kb, precision
+    real(c_double), intent(in) :: vx(m), vy(m), h(m)
+    integer, intent(in) :: nv(n, 3), obc_nodes(:)
+    type(fv_params), intent(inout) :: p
+    integer(c_int32_t), allocatable :: rows(:), obc(:)
+    real(c_double), allocatable :: xx(:), yy(:), hh(:)
+    integer :: i
+    allocate(rows(3 * n), obc(max(1, size(obc_nodes))), xx(m), yy(m), hh(m))
+    do i = 1, n
+      rows(3 * i - 2) = nv(i, 1)
+      rows(3 * i - 1) = nv(i, 2)
+      rows(3 * i)     = nv(i, 3)
+    end do
+    obc(1:size(obc_nodes)) = obc_nodes
+    xx = vx; yy = vy; hh = h
+    call fvgpu_check(fv_model_from_mesh(int(m, c_int32_t), int(n, c_int32_t), xx, yy, rows, hh, &
+         int(kb, c_int32_t), int(size(obc_nodes), c_int32_t), obc, p, fvgpu_model), "fv_model_from_mesh")
+    call fvgpu_check(fv_model_cold_start(fvgpu_model), "fv_model_cold_start")
+    call fvgpu_check(fvgpu_create(fvgpu_model, int(precision, c_int32_t), fvgpu_h), "fvgpu_create")
   end subroutine
 
   ! INTERNAL_STEP: one baroclinic step with ISPLIT barotropic steps inside.

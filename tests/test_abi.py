@@ -14,7 +14,7 @@ def declared_functions():
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s+\*?([a-z][a-z0-9_]+)\s*\(",
                        src, flags=re.M)
-    return sorted(set(n for n in names if n.startswith(("fv_", "fvgpu_"))))
+    return sorted(set(n for n in names if n.startswith(("fv_", "fvgpu_", "fvmlp_"))))
 
 
 def test_header_lists_entry_points():
@@ -33,7 +33,7 @@ def test_libraries_export_every_declared_symbol(built):
     lg, lgpu = C.CDLL(grid), C.CDLL(gpu)
     missing = []
     for n in declared_functions():
-        lib = lgpu if n.startswith("fvgpu_") else lg
+        lib = lgpu if n.startswith(("fvgpu_", "fvmlp_")) else lg
         if not hasattr(lib, n):
             missing.append(n)
     assert not missing, missing
@@ -52,3 +52,29 @@ def test_gpu_library_fails_loudly_without_device(built):
     m = F.Model.build(0, nx=9, ny=5, kb=4)
     with pytest.raises(RuntimeError):
         F.DeviceModel(m)
+
+
+def fortran_bindings():
+    src = open(os.path.join(ROOT, "myfvcomcode_amd", "fortran", "fvgpu_mod.f90")).read()
+    return sorted(set(re.findall(r'bind\(c,\s*name="([a-z0-9_]+)"\)', src)))
+
+
+def test_fortran_bindings_match_header():
+    """Every BIND(C) interface of the Fortran module names a declared entry
+    point (a misspelt name would only fail at link time of a host model)."""
+    decl = set(declared_functions())
+    names = fortran_bindings()
+    assert len(names) >= 20
+    assert not [n for n in names if n not in decl]
+
+
+def test_fortran_driver_usage(built):
+    """The Fortran driver links against both libraries and reports its usage
+    (exit status 2) before touching the device."""
+    import subprocess
+    exe = os.path.join(ROOT, "myfvcomcode_amd", "lib", "fvcom_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("the Fortran driver is not built (no amdflang)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "usage: fvcom_gpu" in r.stdout

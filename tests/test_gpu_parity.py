@@ -79,3 +79,33 @@ def test_jittered_wind_case_parity(built, graph):
     for k, v in sorted(rep.items(), key=lambda x: -x[1])[:8]:
         print("  %-8s %.3e" % (k, v))
     assert not bad, bad
+
+
+def test_fortran_driver_matches_library(built):
+    """The Fortran host (ISO_C_BINDING module + driver) advances the estuary
+    channel exactly as the library called directly: its REPORT lines of
+    volume, salt and minimum depth (printed to 17 significant digits) equal
+    fvgpu_diagnostics of the same run."""
+    import os
+    import subprocess
+    import myfvcomcode_amd as F
+    exe = os.path.join(os.path.dirname(F.__file__), "lib", "fvcom_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("the Fortran driver is not built (no amdflang)")
+    print("case: 81x21 estuary channel, 6 INTERNAL_STEPs through the Fortran driver", flush=True)
+    r = subprocess.run([exe, "0", "6", "3"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    rows = {}
+    for line in r.stdout.splitlines():
+        if "IINT=" in line:
+            t = line.replace("=", " ").split()
+            rows[int(t[1])] = (float(t[3]), float(t[5]), float(t[7]))
+    assert sorted(rows) == [3, 6]
+    dev = F.DeviceModel(F.Model.build(0))
+    for it in (3, 6):
+        dev.run(3, graph=False)
+        d = dev.diagnostics()
+        assert d.iint == it
+        for got, want in zip(rows[it], (d.volume, d.salt, d.min_depth)):
+            assert abs(got - want) <= 1e-16 * abs(want), (it, got, want)
