@@ -227,8 +227,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext);
   TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
-#define VERTVL_CALL(KM) TLAUNCH("vertvl", k_vertvl_r<KM>, g.M + 1, g, a)
-  FV_KMAX_DISPATCH(g.KBM1, VERTVL_CALL);
+#define VERTVL_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
+  FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
   TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);

@@ -343,9 +343,15 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
 }
 
 // VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer (sums over
-// the node's sub-edges in sub-edge order, kept in registers), then the
-// column integration of omega, the linear mass-error correction and DTFA.
-template <int KMAX>
+// the node's sub-edges in sub-edge order), the column integration of omega
+// stored to WTS as it goes, then the linear mass-error correction and DTFA
+// in a second pass over the column.  Layers are outermost, so one layer of
+// the elements around a wavefront's nodes is gathered at a time and each
+// element-layer value crosses HBM once; the sub-edge elements, their DT1
+// and the signed DLTXE/DLTYE stay in registers (SMAX >= the node's sub-edge
+// count), so the per-layer reads are the U/V gathers alone and issue
+// together.
+template <int SMAX>
 __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
@@ -355,40 +361,39 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
     return;
   }
   const int KBM1 = g.KBM1;
-  // layers outermost: one layer of the elements around a wavefront's nodes
-  // is gathered at a time (the slot lists are re-read from cache), so each
-  // element-layer value crosses HBM once
-  double acc[KMAX];
   const int n = g.cv_cnt[i];
-  static_for<KMAX>([&](auto kc) {
-    constexpr int k = decltype(kc)::value + 1;
-    double x = 0.0;
-    if (k <= KBM1) {
-      const long ko = (long)(k - 1) * g.n1;
-      for (int j = 0; j < n; ++j) {
-        const long q = (long)j * g.m1 + i;
-        const int i1 = g.cvl_elem[q];
-        const double dij = a.dt1[i1] * DZ1(i1, k);
-        x = x + dij * (-a.u[ko + i1] * g.cvl_sdy[q] + a.v[ko + i1] * g.cvl_sdx[q]);
-      }
-    }
-    acc[k - 1] = x;
+  int ce[SMAX];
+  double dte[SMAX], sdx[SMAX], sdy[SMAX];
+  static_for<SMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    const long q = (long)j * g.m1 + i;
+    ce[j] = j < n ? g.cvl_elem[q] : 0;
+    dte[j] = j < n ? a.dt1[ce[j]] : 0.0;
+    sdx[j] = j < n ? g.cvl_sdx[q] : 0.0;
+    sdy[j] = j < n ? g.cvl_sdy[q] : 0.0;
   });
   const bool obc = g.isonb[i] == 2;
   const int rv = g.node_riv[i];
   const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
   double w = 0.0;
-  static_for<KMAX>([&](auto kc) {
-    constexpr int k = decltype(kc)::value + 1;
-    if (k <= KBM1) {
-      double xflux = obc ? 0.0 : acc[k - 1];
-      if (rv >= 0)
-        for (int j = rv; j < g.NUMQBC; ++j)
-          if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
-      w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
-      acc[k - 1] = w;
-    }
-  });
+  a.wts[NI(g, i, 1)] = 0.0;
+  for (int k = 1; k <= KBM1; ++k) {
+    const long ko = (long)(k - 1) * g.n1;
+    double x = 0.0;
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (j < n) {
+        const double dij = dte[j] * DZ1(ce[j], k);
+        x = x + dij * (-a.u[ko + ce[j]] * sdy[j] + a.v[ko + ce[j]] * sdx[j]);
+      }
+    });
+    double xflux = obc ? 0.0 : x;
+    if (rv >= 0)
+      for (int j = rv; j < g.NUMQBC; ++j)
+        if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+    w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
+    a.wts[NI(g, i, k + 1)] = w;
+  }
   const double wbottom = 0.0;
   double dtfa = a.d[i];
   const bool corr = fabs(w - wbottom) > 1.0E-8 && !obc;
@@ -396,19 +401,12 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
     double tmp1 = a.elf[i] * (double)KBM1 - (w - wbottom) * g.dti / DZ(i, 1);
     tmp1 = tmp1 / (double)KBM1;
     dtfa = tmp1 + g.h[i];
-  }
-  a.wts[NI(g, i, 1)] = 0.0;
-  static_for<KMAX>([&](auto kc) {
-    constexpr int k = decltype(kc)::value + 2;
-    if (k <= KBM1 + 1) {
-      double wk = acc[k - 2];
-      if (corr) {
-        const double ratio = (double)((float)(k - 1) / (float)KBM1);
-        wk = wk - ratio * (w - wbottom);
-      }
-      a.wts[NI(g, i, k)] = wk;
+    for (int k = 2; k <= KBM1 + 1; ++k) {
+      const double ratio = (double)((float)(k - 1) / (float)KBM1);
+      const long q = NI(g, i, k);
+      a.wts[q] = a.wts[q] - ratio * (w - wbottom);
     }
-  });
+  }
   a.dtfa[i] = dtfa;
 }
 

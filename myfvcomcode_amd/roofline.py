@@ -167,10 +167,13 @@ def viscofh(s, real=R8):
 
 
 def vdif_q(s, real=R8):
-    """VDIF_Q (MY2.5): column solves for Q2, Q2L and the mixing coefficients."""
+    """VDIF_Q (MY2.5): column solves for Q2, Q2L and the mixing coefficients,
+    then Q2 = Q2F, Q2L = Q2LF.  Reads Q2 Q2L RHO1 KQ KM KH Q2F Q2LF and the
+    ring of U/V; writes Q2F Q2LF Q2 Q2L L KM KH KQ.  The in-kernel scratch
+    (elimination coefficients, VHP parked in Q2F/Q2LF) is not counted."""
     M, N, KB, KBM1, RVE = s["M"], s["N"], s["KB"], s["KBM1"], s["RVE"]
-    reads = M * KB * 9 * real + N * KBM1 * 2 * real + RVE * I4 + M * 4 * real + N * 4 * real
-    writes = M * KB * 6 * real + M * KB * 2 * real    # Q2F Q2LF L KM KH KQ; PROD scratch (write + read)
+    reads = M * KB * 8 * real + N * KBM1 * 2 * real + RVE * I4 + M * 4 * real + N * 4 * real
+    writes = M * KB * 8 * real
     return reads + writes
 
 
