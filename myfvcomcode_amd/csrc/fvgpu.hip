@@ -47,12 +47,19 @@ struct fvgpu_handle {
   int precision = 0;
   int device = 0;
   hipStream_t stream = nullptr;
+  // Independent chains of the step (the 2-D elevation and edge terms, the
+  // T and S passes, the Q2 and Q2L advection) fork onto stream2 and join
+  // back; both streams have the default priority.  `cur` is the stream the
+  // launch macros enqueue on.
+  hipStream_t stream2 = nullptr, cur = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   DevGrid g{};
   std::vector<void*> allocs;
   // device state
   DevClock* clk = nullptr;
   Ext2DState ext[3];      // ext[0] = state / RK base, ext[1], ext[2] stage buffers
   IntArgs ia{};
+  IntArgs ib{};           // ia with its own scratch, for the chain on stream2
   double *xflux_obcn = nullptr, *fobn = nullptr;
   // graph
   hipGraph_t graph = nullptr;
@@ -121,12 +128,12 @@ struct Launch {
   bool on;
   Launch(fvgpu_handle* h_, const char* n) : h(h_), name(n) {
     on = h->profiling && (h->profile_only.empty() || h->profile_only == n);
-    if (on) { e0 = next_event(h); hipEventRecord(e0, h->stream); }
+    if (on) { e0 = next_event(h); hipEventRecord(e0, h->cur); }
   }
   ~Launch() {
     if (on) {
       e1 = next_event(h);
-      hipEventRecord(e1, h->stream);
+      hipEventRecord(e1, h->cur);
       h->pending.push_back({name, {e0, e1}});
     }
   }
@@ -150,7 +157,7 @@ int flush_profile(fvgpu_handle* h) {
 #define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...)                                   \
   do {                                                                            \
     Launch l_(h, NAME);                                                           \
-    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), 0, h->stream, __VA_ARGS__); \
+    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), 0, h->cur, __VA_ARGS__);    \
   } while (0)
 
 inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
@@ -166,8 +173,20 @@ int block_of(const fvgpu_handle* h, const char* name) {
     const int bs_ = block_of(h, NAME);                                                            \
     Launch l_(h, NAME);                                                                           \
     hipLaunchKernelGGL(KERNEL, dim3((unsigned)(((long)(NITEMS) + bs_ - 1) / bs_)), dim3(bs_), 0, \
-                       h->stream, __VA_ARGS__);                                                   \
+                       h->cur, __VA_ARGS__);                                                      \
   } while (0)
+
+// Fork: work enqueued on stream2 after this waits for everything enqueued
+// on the main stream so far.  Join: the main stream waits for stream2.
+// Under graph capture the events become dependency edges of the graph.
+void fork_streams(fvgpu_handle* h) {
+  hipEventRecord(h->ev_fork, h->stream);
+  hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+}
+void join_streams(fvgpu_handle* h) {
+  hipEventRecord(h->ev_join, h->stream2);
+  hipStreamWaitEvent(h->stream, h->ev_join, 0);
+}
 
 const double ALPHA_RK[5] = {0.0, 0.2500, 1.0 / 3.0, 0.5000, 1.0};
 
@@ -204,10 +223,51 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext) {
   const Ext2DState* nxt[5] = {nullptr, &h->ext[1], &h->ext[2], &h->ext[1], &h->ext[0]};
   for (int K = 1; K <= 4; ++K) {
     ExtArgs a = make_ext_args(h, K, *cur[K], *nxt[K]);
-    TLAUNCH("ext_elevation", k_ext_elevation, g.M + 1, g, a);
+    // the edge terms and the elevation both read only the stage input
+    fork_streams(h);
+    h->cur = h->stream2;
     TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
+    h->cur = h->stream;
+    TLAUNCH("ext_elevation", k_ext_elevation, g.M + 1, g, a);
+    join_streams(h);
     TLAUNCH("ext_momentum", k_ext_momentum<FT>, g.N + 1, g, a);
   }
+}
+
+// ADV_T or ADV_S with FCT, VDIF_TS, BCOND_TS, F1 = FF and N2E3D of one
+// tracer (internal_step.F), on h->cur with the scratch of `a`.
+template <typename FT>
+void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
+  const DevGrid& g = h->g;
+  const long mk = g.m1 * g.KB;
+  double* f1 = which == 1 ? a.t1 : a.s1;
+  double* ff = which == 1 ? a.tf1 : a.sf1;
+  const double* fm = which == 1 ? a.tmean1 : a.smean1;
+  const double* fdis = which == 1 ? g.tdis : g.sdis;
+#define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
+  FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
+  TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.NCV + 1, g, a);
+#define ADV_TS_CALL(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
+  FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
+  if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
+#define VDIF_TS_CALL(KM) TLAUNCH("vdif_ts", k_vdif_ts_r<KM>, g.M + 1, g, a, ff, which)
+  FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
+  TLAUNCH("bcond_ts", k_bcond_ts, g.IOBCN + 1, g, a, which);
+  TLAUNCH("copy", k_copy, mk, f1, ff, mk);
+  TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, f1, which == 1 ? a.t : a.s, g.KB);
+}
+
+// ADV_Q(Q, QF) with FCT_Q2 (adv_q.F, fct_q2.F) on h->cur with the scratch
+// of `a`.
+template <typename FT>
+void enqueue_adv_q(fvgpu_handle* h, IntArgs& a, double* q, double* qf, int fct) {
+  const DevGrid& g = h->g;
+  const double qmean1 = (double)1.E-8f;
+#define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
+  FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
+  TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.NCV + 1, g, a);
+#define ADV_Q_CALL(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, FT>), g.M + 1, g, a, q, qf, fct)
+  FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
 }
 
 template <typename FT>
@@ -238,7 +298,6 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   TLAUNCH("uv_finish", k_uv_finish, (long)g.N * g.KBM1, g, a);
   TLAUNCH("wreal", k_wreal, g.N + 1, g, a);
-  const double qmean1 = (double)1.E-8f;
   const int fct = h->md->p.scalar_positivity ? 1 : 0;
   if (h->md->p.vertical_closure && h->closure_mode == 1) {
     // learned closure: features, MFMA inference, realizability, then the
@@ -260,15 +319,13 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL16);
   } else if (h->md->p.vertical_closure) {
-    for (int which = 0; which < 2; ++which) {
-      double* q = which ? a.q2l : a.q2;
-      double* qf = which ? a.q2lf : a.q2f;
-#define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
-      FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
-      TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.NCV + 1, g, a);
-#define ADV_Q_CALL(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, FT>), g.M + 1, g, a, q, qf, fct)
-      FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
-    }
+    // ADV_Q of Q2 on the main stream, of Q2L on stream2 (own scratch)
+    fork_streams(h);
+    h->cur = h->stream2;
+    enqueue_adv_q<FT>(h, h->ib, a.q2l, a.q2lf, fct);
+    h->cur = h->stream;
+    enqueue_adv_q<FT>(h, a, a.q2, a.q2f, fct);
+    join_streams(h);
 #define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
 #define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
     // VDIF_Q, then Q2 = Q2F and Q2L = Q2LF (internal_step.F:1029-1035) in
@@ -278,24 +335,20 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
   }
   TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.km, a.km1, g.KB);
-  for (int which = 1; which <= 2; ++which) {
-    if (which == 1 && !h->md->p.temperature_active) continue;
-    if (which == 2 && !h->md->p.salinity_active) continue;
-    double* f1 = which == 1 ? a.t1 : a.s1;
-    double* ff = which == 1 ? a.tf1 : a.sf1;
-    const double* fm = which == 1 ? a.tmean1 : a.smean1;
-    const double* fdis = which == 1 ? g.tdis : g.sdis;
-#define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
-    FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
-    TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.NCV + 1, g, a);
-#define ADV_TS_CALL(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
-    FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
-    if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
-#define VDIF_TS_CALL(KM) TLAUNCH("vdif_ts", k_vdif_ts_r<KM>, g.M + 1, g, a, ff, which)
-    FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
-    TLAUNCH("bcond_ts", k_bcond_ts, g.IOBCN + 1, g, a, which);
-    TLAUNCH("copy", k_copy, mk, f1, ff, mk);
-    TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, f1, which == 1 ? a.t : a.s, g.KB);
+  // ADV_T .. T1 = TF1 on the main stream, the same for S on stream2 (own
+  // scratch); the two passes share no written array
+  const bool t_on = h->md->p.temperature_active, s_on = h->md->p.salinity_active;
+  if (t_on && s_on) {
+    fork_streams(h);
+    h->cur = h->stream2;
+    enqueue_ts_pass<FT>(h, h->ib, 2, fct);
+    h->cur = h->stream;
+    enqueue_ts_pass<FT>(h, a, 1, fct);
+    join_streams(h);
+  } else if (t_on) {
+    enqueue_ts_pass<FT>(h, a, 1, fct);
+  } else if (s_on) {
+    enqueue_ts_pass<FT>(h, a, 2, fct);
   }
   if (!h->md->p.barotropic) {
     TLAUNCH("dens2", k_dens2, mk, g, a);
@@ -574,6 +627,20 @@ int upload_state(fvgpu_handle* h) {
   c.isplit = g.isplit; c.iramp = g.iramp; c.depth_fail = 0;
   HIPCHK(hipMemcpy(h->clk, &c, sizeof(c), hipMemcpyHostToDevice));
   a.clk = h->clk;
+  // the stream2 chains: the same arrays, their own scratch (node-layer
+  // records and FCT bounds, sub-edge fluxes, bottom gradients, OBC fluxes)
+  IntArgs& b = h->ib;
+  b = a;
+  {
+    CvRecT<double>* pr = nullptr;
+    double* pc = nullptr;
+    if (dalloc(h, &pr, mk)) return 1;
+    if (dalloc(h, &pc, ((size_t)g.NCV + 1) * g.KB)) return 1;
+    b.rec = pr; b.cvflux = pc;
+  }
+  if (dalloc(h, &b.rlohi, mk)) return 1;
+  if (dalloc(h, &b.pfpxb, m1) || dalloc(h, &b.pfpyb, m1) || dalloc(h, &b.ah_bottom, m1)) return 1;
+  if (dalloc(h, &b.xflux_obc, (size_t)g.IOBCN * g.KB)) return 1;
   return 0;
 }
 
@@ -627,6 +694,13 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   h->precision = precision;
   HIPCHK(hipGetDevice(&h->device));
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
+  h->cur = h->stream;
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+    fvgpu_destroy(h);
+    return fail("stream2 / events");
+  }
   if (upload_grid(h) || upload_state(h)) { fvgpu_destroy(h); return 1; }
   HIPCHK(hipStreamSynchronize(h->stream));
   *out = h;
@@ -762,10 +836,14 @@ int fvgpu_synchronize(fvgpu_handle* h) {
 void fvgpu_destroy(fvgpu_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream2) hipStreamSynchronize(h->stream2);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
   for (hipEvent_t e : h->event_pool) hipEventDestroy(e);
   for (void* p : h->allocs) hipFree(p);
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }

@@ -5,16 +5,20 @@
 #pragma once
 #include "fvgpu_kernels_3d.h"
 
-// BCOND_TS(NCON) (bcond_ts.F:28-268), one thread per OBC node; thread
-// IOBCN.. handles the specified river values first.
+// BCOND_TS(NCON) (bcond_ts.F:28-268), one thread per OBC node; thread 0
+// also sets the specified river values.  The reference sets TF1 and SF1 at
+// the river nodes in both calls; each call here sets its own tracer only,
+// which leaves T1 and S1 unchanged (each copy F1 = FF follows its own call)
+// and lets the T and S passes run concurrently.
 __global__ void k_bcond_ts(DevGrid g, IntArgs a, int ncon) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double* ff = ncon == 1 ? a.tf1 : a.sf1;
   const double* f1 = ncon == 1 ? a.t1 : a.s1;
   if (t == 0 && g.river_ts_specified) {
+    const double* fdis = ncon == 1 ? g.tdis : g.sdis;
     for (int r = 0; r < g.NUMQBC; ++r) {
       const int j11 = g.inodeq[r];
-      for (int k = 1; k <= g.KBM1; ++k) { a.tf1[NI(g, j11, k)] = g.tdis[r]; a.sf1[NI(g, j11, k)] = g.sdis[r]; }
+      for (int k = 1; k <= g.KBM1; ++k) ff[NI(g, j11, k)] = fdis[r];
     }
   }
   if (!g.obc_on || t >= g.IOBCN) return;
