@@ -246,7 +246,7 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
   const double* fdis = which == 1 ? g.tdis : g.sdis;
 #define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
   FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
-  TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.NCV + 1, g, a);
+  TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.N + 1, g, a);
 #define ADV_TS_CALL(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
   FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
   if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
@@ -265,7 +265,7 @@ void enqueue_adv_q(fvgpu_handle* h, IntArgs& a, double* q, double* qf, int fct) 
   const double qmean1 = (double)1.E-8f;
 #define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
   FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
-  TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.NCV + 1, g, a);
+  TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.N + 1, g, a);
 #define ADV_Q_CALL(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, FT>), g.M + 1, g, a, q, qf, fct)
   FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
 }
@@ -385,6 +385,32 @@ int build_lists(fvgpu_handle* h) {
     ent[off[ia] + fill[ia]++] = (c << 1) | 0;
     ent[off[ib] + fill[ib]++] = (c << 1) | 1;
   }
+  // element -> 3 sub-edges, and each sub-edge's element-major flux position
+  std::vector<int> pos(NCV + 1, 0);
+  {
+    const size_t n1 = (size_t)N + 1;
+    std::vector<int> ec(3 * n1, 0), eab(3 * n1, 0), ecnt(n1, 0);
+    for (int c = 1; c <= NCV; ++c) {
+      const int e = md->ntrg[c];
+      const int j = ecnt[e]++;
+      if (j >= 3) return fail("an element has more than 3 control-volume sub-edges");
+      const int ia = md->niec[c], ib = md->niec[(size_t)(NCV + 1) + c];
+      int la = -1, lb = -1;
+      for (int q = 0; q < 3; ++q) {
+        const int nq = md->nv[(size_t)q * n1 + e];
+        if (nq == ia) la = q;
+        if (nq == ib) lb = q;
+      }
+      if (la < 0 || lb < 0) return fail("a sub-edge's nodes are not nodes of its element");
+      ec[(size_t)j * n1 + e] = c;
+      eab[(size_t)j * n1 + e] = la | (lb << 2);
+      pos[c] = 3 * (e - 1) + j + 1;
+    }
+    for (int e = 1; e <= N; ++e)
+      if (ecnt[e] != 3) return fail("an element has fewer than 3 control-volume sub-edges");
+    if (dupload(h, &g.ecv_c, ec.data(), ec.size())) return 1;
+    if (dupload(h, &g.ecv_ab, eab.data(), eab.size())) return 1;
+  }
   g.ring_max = 1;
   g.ve_max = 1;
   for (int i = 1; i <= M; ++i) {
@@ -407,7 +433,7 @@ int build_lists(fvgpu_handle* h) {
       for (int j = 0; j < cnt[i]; ++j) {
         const int e = ent[off[i] + j], c = e >> 1;
         const size_t q = (size_t)j * m1 + i;
-        le[q] = e;
+        le[q] = (pos[c] << 1) | (e & 1);
         lel[q] = md->ntrg[c];
         sdx[q] = (e & 1) ? md->dltxe[c] : -md->dltxe[c];
         sdy[q] = (e & 1) ? md->dltye[c] : -md->dltye[c];

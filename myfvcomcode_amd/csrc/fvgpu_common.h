@@ -46,10 +46,17 @@ struct DevGrid {
   // the same lists slot-major (entry j of node i at j*m1+i, j < cv_cnt[i]):
   // consecutive nodes read consecutive addresses for each slot.  cvl_sdx/sdy
   // are DLTXE/DLTYE negated on side 0 (XFLUX(IA) -= f  ==  XFLUX(IA) += (-f)).
+  // cvl_ent holds (p<<1 | side), p the element-major position of the
+  // sub-edge's flux in the per-layer flux buffers (below).
   int cv_max;
   int ring_max, ve_max;            // max NTSN, max NTVE
   const int *cv_cnt, *cvl_ent, *cvl_elem;
   const double *cvl_sdx, *cvl_sdy;
+  // element -> its 3 sub-edges in increasing index order, slot-major
+  // (slot j of element e at j*n1+e): the sub-edge index, and the positions
+  // in NV(e,1:3) of its nodes IA (bits 0-1) and IB (bits 2-3).  The flux of
+  // slot j of element e is stored at position 3*(e-1)+j+1.
+  const int *ecv_c, *ecv_ab;
   const int *ed_ent;               // element -> 3 x (edge<<1 | side), sorted by edge
   const int *fo_off, *fo_obc;      // element -> FLUXOBN contributions (obc<<1 | slot)
   const int *node_obc;             // node -> OBC index or -1

@@ -7,8 +7,11 @@
 //   1. k_cvrec     one thread per node: ring gradient, ring bounds and the
 //                  viscosity factor of every layer, packed per node-layer
 //                  (CvRec), the node's ring kept in registers across layers;
-//   2. k_cvflux_*  one thread per sub-edge: the face flux of every layer,
-//                  evaluated once, written to cvflux[k][c] (coalesced);
+//   2. k_cvflux_*  one thread per element: the face fluxes of its three
+//                  sub-edges at every layer, each evaluated once from the
+//                  element's three node records (gathered once per layer
+//                  for the three faces) and its velocity, written to the
+//                  element-major positions 3*(e-1)+j+1 (coalesced);
 //   3. k_*_gather  one thread per node: the node's sub-edge fluxes summed in
 //                  increasing sub-edge order (the reference's scatter order,
 //                  so the sum is bit-identical), then the vertical flux and
@@ -114,27 +117,65 @@ __device__ __forceinline__ CvFace<FT> cv_face(FT un, FT dtij, FT dltx, FT dlty, 
   return r;
 }
 
-// Phase 2 of ADV_T/ADV_S, per sub-edge and layer, with the geometry held in
-// registers across layers.
+// The three sub-edges of element e: geometry in the flux type, with the
+// positions in NV(e,1:3) of each face's nodes IA and IB.
+template <typename FT>
+struct ElemFaces {
+  FT dltx[3], dlty[3], x1[3], y1[3], x2[3], y2[3], visc0[3];
+  int la[3], lb[3];
+};
+
+template <typename FT>
+__device__ __forceinline__ ElemFaces<FT> elem_faces(const DevGrid& g, int e) {
+  ElemFaces<FT> f;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int c = g.ecv_c[(long)j * g.n1 + e], ab = g.ecv_ab[(long)j * g.n1 + e];
+    f.la[j] = ab & 3;
+    f.lb[j] = ab >> 2;
+    f.dltx[j] = (FT)g.dltxe[c];
+    f.dlty[j] = (FT)g.dltye[c];
+    f.x1[j] = (FT)DLTXNCVE_(c, 1);
+    f.y1[j] = (FT)DLTYNCVE_(c, 1);
+    f.x2[j] = (FT)DLTXNCVE_(c, 2);
+    f.y2[j] = (FT)DLTYNCVE_(c, 2);
+    f.visc0[j] = (FT)(g.fm1 * 0.5 * (g.nn_hvc[NIEC_(c, 1)] + g.nn_hvc[NIEC_(c, 2)]));
+  }
+  return f;
+}
+
+template <typename FT>
+__device__ __forceinline__ const CvRecT<FT>& pick3(int l, const CvRecT<FT>& r0, const CvRecT<FT>& r1,
+                                                   const CvRecT<FT>& r2) {
+  return l == 0 ? r0 : (l == 1 ? r1 : r2);
+}
+
+// Phase 2 of ADV_T/ADV_S: the three face fluxes of one element per layer
+// (adv_t.F:440-500), with the geometry held in registers across layers.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
-  const int c = (int)xcd_thread();
-  if (c < 1 || c > g.NCV) return;
+  const int e = (int)xcd_thread();
+  if (e < 1 || e > g.N) return;
   const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
   FT* cvflux = (FT*)a.cvflux;
-  const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
-  const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
-  const FT x1 = (FT)DLTXNCVE_(c, 1), y1 = (FT)DLTYNCVE_(c, 1), x2 = (FT)DLTXNCVE_(c, 2), y2 = (FT)DLTYNCVE_(c, 2);
-  const FT visc0 = (FT)(g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]));
+  const ElemFaces<FT> f = elem_faces<FT>(g, e);
+  const int n0 = NV_(e, 1), n1 = NV_(e, 2), n2 = NV_(e, 3);
   const FT fact = (FT)g.fact;
-  const double dt1 = a.dt1[i1];
-  const long cs = (long)g.NCV + 1;
+  const double dt1 = a.dt1[e];
+  const long cs = (long)g.NCV + 1, p0 = 3L * (e - 1) + 1;
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
-    const FT dtij = (FT)(dt1 * DZ1(i1, k));
-    const FT un = (FT)a.v[eo + i1] * dltx - (FT)a.u[eo + i1] * dlty;
-    const CvFace<FT> r = cv_face<FT>(un, dtij, dltx, dlty, x1, y1, x2, y2, fact, visc0, rec[ko + ia], rec[ko + ib]);
-    cvflux[(long)(k - 1) * cs + c] = r.exflux;
+    const FT dtij = (FT)(dt1 * DZ1(e, k));
+    const FT vk = (FT)a.v[eo + e], uk = (FT)a.u[eo + e];
+    const CvRecT<FT> r0 = rec[ko + n0], r1 = rec[ko + n1], r2 = rec[ko + n2];
+    FT* out = cvflux + (long)(k - 1) * cs + p0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const FT un = vk * f.dltx[j] - uk * f.dlty[j];
+      const CvFace<FT> r = cv_face<FT>(un, dtij, f.dltx[j], f.dlty[j], f.x1[j], f.y1[j], f.x2[j], f.y2[j], fact,
+                                       f.visc0[j], pick3(f.la[j], r0, r1, r2), pick3(f.lb[j], r0, r1, r2));
+      out[j] = r.exflux;
+    }
   }
 }
 
@@ -151,7 +192,7 @@ __global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
   const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
   double xadv = 0.0;
   for (int j = 0; j < n; ++j) {
-    const int e = g.cvl_ent[(long)j * g.m1 + i], c = e >> 1;
+    const int e = g.cv_ent[g.cv_off[i] + j], c = e >> 1;
     const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
     const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
     const FT dtij = (FT)(a.dt1[i1] * DZ1(i1, k));
@@ -244,47 +285,52 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
   }
 }
 
-// Phase 2 of ADV_Q (adv_q.F:200-330): face flux on layer interfaces
-// k = 2..KBM1, velocities interpolated to the interface with DZ1 weights.
+// Phase 2 of ADV_Q (adv_q.F:200-330): the three face fluxes of one element
+// on layer interfaces k = 2..KBM1, the element velocity interpolated to the
+// interface with DZ1 weights (once for the three faces).
 template <typename FT>
 __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
-  const int c = (int)xcd_thread();
-  if (c < 1 || c > g.NCV) return;
+  const int e = (int)xcd_thread();
+  if (e < 1 || e > g.N) return;
   const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
   FT* cvflux = (FT*)a.cvflux;
-  const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
-  const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
-  const FT x1 = (FT)DLTXNCVE_(c, 1), y1 = (FT)DLTYNCVE_(c, 1), x2 = (FT)DLTXNCVE_(c, 2), y2 = (FT)DLTYNCVE_(c, 2);
-  const FT visc0 = (FT)(g.fm1 * 0.5 * (g.nn_hvc[ia] + g.nn_hvc[ib]));
+  const ElemFaces<FT> f = elem_faces<FT>(g, e);
+  const int n0 = NV_(e, 1), n1 = NV_(e, 2), n2 = NV_(e, 3);
   const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu;
-  const double dt1 = a.dt1[i1];
-  const long cs = (long)g.NCV + 1;
-  double um = a.u[i1], vm = a.v[i1];   // layer 1
+  const double dt1 = a.dt1[e];
+  const long cs = (long)g.NCV + 1, p0 = 3L * (e - 1) + 1;
+  double um = a.u[e], vm = a.v[e];   // layer 1
   for (int k = 2; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
-    const double uk = a.u[eo + i1], vk = a.v[eo + i1];
-    const double dz1k = DZ1(i1, k), dz1m = DZ1(i1, k - 1);
+    const double uk = a.u[eo + e], vk = a.v[eo + e];
+    const double dz1k = DZ1(e, k), dz1m = DZ1(e, k - 1);
     const FT uq = (FT)((uk * dz1m + um * dz1k) / (dz1k + dz1m));
     const FT vq = (FT)((vk * dz1m + vm * dz1k) / (dz1k + dz1m));
     um = uk;
     vm = vk;
-    const FT dtij = (FT)(dt1 * DZZ1(i1, k - 1));
-    const FT un = vq * dltx - uq * dlty;
-    const CvRecT<FT> ra = rec[ko + ia], rb = rec[ko + ib];
+    const FT dtij = (FT)(dt1 * DZZ1(e, k - 1));
+    const CvRecT<FT> r0 = rec[ko + n0], r1 = rec[ko + n1], r2 = rec[ko + n2];
+    FT* out = cvflux + (long)(k - 1) * cs + p0;
     const FT half = (FT)0.5, one = (FT)1.0;
-    FT fij1 = ra.f + x1 * ra.gx + y1 * ra.gy;
-    FT fij2 = rb.f + x2 * rb.gx + y2 * rb.gy;
-    if (fij1 < ra.fmin) fij1 = ra.fmin;
-    if (fij1 > ra.fmax) fij1 = ra.fmax;
-    if (fij2 < rb.fmin) fij2 = rb.fmin;
-    if (fij2 > rb.fmax) fij2 = rb.fmax;
-    const FT viscof = (fact * half * (ra.vhn + rb.vhn) + visc0) / rhprnu;
-    const FT txx = half * (ra.gxd + rb.gxd) * viscof;
-    const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
-    const FT fxx = -dtij * txx * dlty;
-    const FT fyy = dtij * tyy * dltx;
-    const FT sgn = fsign1(un);
-    cvflux[(long)(k - 1) * cs + c] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const FT un = vq * f.dltx[j] - uq * f.dlty[j];
+      const CvRecT<FT>& ra = pick3(f.la[j], r0, r1, r2);
+      const CvRecT<FT>& rb = pick3(f.lb[j], r0, r1, r2);
+      FT fij1 = ra.f + f.x1[j] * ra.gx + f.y1[j] * ra.gy;
+      FT fij2 = rb.f + f.x2[j] * rb.gx + f.y2[j] * rb.gy;
+      if (fij1 < ra.fmin) fij1 = ra.fmin;
+      if (fij1 > ra.fmax) fij1 = ra.fmax;
+      if (fij2 < rb.fmin) fij2 = rb.fmin;
+      if (fij2 > rb.fmax) fij2 = rb.fmax;
+      const FT viscof = (fact * half * (ra.vhn + rb.vhn) + f.visc0[j]) / rhprnu;
+      const FT txx = half * (ra.gxd + rb.gxd) * viscof;
+      const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
+      const FT fxx = -dtij * txx * f.dlty[j];
+      const FT fyy = dtij * tyy * f.dltx[j];
+      const FT sgn = fsign1(un);
+      out[j] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
+    }
   }
 }
 
