@@ -71,7 +71,12 @@ def main():
     print("bench: warmup done", file=sys.stderr, flush=True)
 
     # timed region: K internal steps, HIP events around every launch on the
-    # handle's stream give each kernel family's average device time
+    # stream it is enqueued on give each kernel family's average device time.
+    # The T/S and Q2/Q2L chains run side by side (concurrency 1); the 2-D
+    # stage pair stays on one stream here so that the dominant family's
+    # event time is its own (its overlap is timed with the variants).
+    dev.set_concurrency(1)
+    dev.synchronize()
     dev.set_profiling(True)
     dev.synchronize()
     t0 = time.perf_counter()
@@ -141,7 +146,7 @@ def main():
         "data": "synthetic (seeded jittered mesh, seamounts, M2/S2/K1 tide, Gaussian random wind)",
         "config": {"workload": "FVCOM INTERNAL_STEP, %dx%d-node jittered mesh, %d elements x %d layers, "
                                "DTE=2 s ISPLIT=10, MY2.5, plain FP64" % (args.nx, args.ny, n, kbm1),
-                   "elements": n, "layers": kbm1, "dti_s": dti, "mode": "eager"},
+                   "elements": n, "layers": kbm1, "dti_s": dti, "mode": "eager, concurrency 1"},
         "elem_layer_steps_per_s": rate,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": pk["achieved_GBps"], "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": pk["frac"], "traffic": traffic,
@@ -159,12 +164,15 @@ def main():
 
 
 def time_variants(F, model, steps, warmup, fp64_step_s, dti, units):
-    """The mixed-precision form, and mixed precision with the learned
-    closure, on the same workload (graph replays, same step count)."""
+    """FP64 with every independent pair of chains side by side, the
+    mixed-precision form, and mixed precision with the learned closure, on
+    the same workload (graph replays, same step count)."""
     from myfvcomcode_amd.closure import load_params
     out = {}
-    for name, closure in (("mixed", False), ("mixed_learned_closure", True)):
-        dev = F.DeviceModel(model, precision="mixed")
+    for name, prec, closure in (("fp64_graph_all_concurrent", "fp64", False), ("mixed", "mixed", False),
+                                ("mixed_learned_closure", "mixed", True)):
+        dev = F.DeviceModel(model, precision=prec)
+        dev.set_concurrency(3)
         if closure:
             dev.set_closure(1, load_params())
         dev.run(warmup, graph=True)

@@ -308,6 +308,13 @@ void fvgpu_destroy(fvgpu_handle* h);
 
 /* Workgroup size (64, 128 or 256) of one kernel family of the step.      */
 int  fvgpu_set_block(fvgpu_handle* h, const char* family, int32_t block);
+/* Concurrency of the step's independent chains on a second stream of the
+ * handle, a bit mask: 1 the S pass beside the T pass and Q2L advection
+ * beside Q2, 2 the 2-D edge terms beside the elevation update of every RK
+ * stage (default 3, both); 0 runs every chain on the main stream, so that
+ * each kernel family's timing is its own.  No counterpart in the
+ * reference; results do not depend on it.                                */
+int  fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode);
 /* Autotuner: times every kernel family of the step with workgroups of 64,
  * 128 and 256 (`steps` internal steps per candidate), keeps the fastest,
  * and restores the model state afterwards.  `report` receives

@@ -120,6 +120,7 @@ def gpu_lib():
                                         C.POINTER(C.c_int64)]
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
         L.fvgpu_set_block.argtypes = [vp, C.c_char_p, C.c_int32]
+        L.fvgpu_set_concurrency.argtypes = [vp, C.c_int32]
         L.fvgpu_autotune.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int64]
         fp = C.POINTER(C.c_float)
         L.fvgpu_set_closure.argtypes = [vp, C.c_int32, fp]
@@ -382,6 +383,12 @@ class DeviceModel:
 
     def set_block(self, family, size):
         _check(gpu_lib(), gpu_lib().fvgpu_set_block(self._h, family.encode(), int(size)), "fvgpu_set_block")
+
+    def set_concurrency(self, mode):
+        """Independent chains of the step on a second stream, a bit mask:
+        1 the T/S and Q2/Q2L pairs, 2 the 2-D stage pair (default 3, both),
+        0 every chain on the main stream (per-kernel timing)."""
+        _check(gpu_lib(), gpu_lib().fvgpu_set_concurrency(self._h, int(mode)), "fvgpu_set_concurrency")
 
     def closure_fallbacks(self):
         L = gpu_lib()

@@ -53,6 +53,7 @@ struct fvgpu_handle {
   // launch macros enqueue on.
   hipStream_t stream2 = nullptr, cur = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int concurrent = 3;     // bit 0: 3-D chain pairs, bit 1: the 2-D stage pair on stream2
   DevGrid g{};
   std::vector<void*> allocs;
   // device state
@@ -179,14 +180,22 @@ int block_of(const fvgpu_handle* h, const char* name) {
 // Fork: work enqueued on stream2 after this waits for everything enqueued
 // on the main stream so far.  Join: the main stream waits for stream2.
 // Under graph capture the events become dependency edges of the graph.
-void fork_streams(fvgpu_handle* h) {
+// `what` is the concurrency bit of the pair (FV_CONC_3D or FV_CONC_2D);
+// with that bit clear both chains go to the main stream in turn.
+enum { FV_CONC_3D = 1, FV_CONC_2D = 2 };
+void fork_streams(fvgpu_handle* h, int what) {
+  if (!(h->concurrent & what)) return;
   hipEventRecord(h->ev_fork, h->stream);
   hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
 }
-void join_streams(fvgpu_handle* h) {
+void join_streams(fvgpu_handle* h, int what) {
+  if (!(h->concurrent & what)) return;
   hipEventRecord(h->ev_join, h->stream2);
   hipStreamWaitEvent(h->stream, h->ev_join, 0);
 }
+// enqueue on the second chain's stream / back on the main stream
+void on_stream2(fvgpu_handle* h, int what) { h->cur = (h->concurrent & what) ? h->stream2 : h->stream; }
+void on_stream1(fvgpu_handle* h) { h->cur = h->stream; }
 
 const double ALPHA_RK[5] = {0.0, 0.2500, 1.0 / 3.0, 0.5000, 1.0};
 
@@ -224,12 +233,12 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext) {
   for (int K = 1; K <= 4; ++K) {
     ExtArgs a = make_ext_args(h, K, *cur[K], *nxt[K]);
     // the edge terms and the elevation both read only the stage input
-    fork_streams(h);
-    h->cur = h->stream2;
+    fork_streams(h, FV_CONC_2D);
+    on_stream2(h, FV_CONC_2D);
     TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
-    h->cur = h->stream;
+    on_stream1(h);
     TLAUNCH("ext_elevation", k_ext_elevation, g.M + 1, g, a);
-    join_streams(h);
+    join_streams(h, FV_CONC_2D);
     TLAUNCH("ext_momentum", k_ext_momentum<FT>, g.N + 1, g, a);
   }
 }
@@ -320,12 +329,12 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL16);
   } else if (h->md->p.vertical_closure) {
     // ADV_Q of Q2 on the main stream, of Q2L on stream2 (own scratch)
-    fork_streams(h);
-    h->cur = h->stream2;
+    fork_streams(h, FV_CONC_3D);
+    on_stream2(h, FV_CONC_3D);
     enqueue_adv_q<FT>(h, h->ib, a.q2l, a.q2lf, fct);
-    h->cur = h->stream;
+    on_stream1(h);
     enqueue_adv_q<FT>(h, a, a.q2, a.q2f, fct);
-    join_streams(h);
+    join_streams(h, FV_CONC_3D);
 #define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
 #define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
     // VDIF_Q, then Q2 = Q2F and Q2L = Q2LF (internal_step.F:1029-1035) in
@@ -339,12 +348,12 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   // scratch); the two passes share no written array
   const bool t_on = h->md->p.temperature_active, s_on = h->md->p.salinity_active;
   if (t_on && s_on) {
-    fork_streams(h);
-    h->cur = h->stream2;
+    fork_streams(h, FV_CONC_3D);
+    on_stream2(h, FV_CONC_3D);
     enqueue_ts_pass<FT>(h, h->ib, 2, fct);
-    h->cur = h->stream;
+    on_stream1(h);
     enqueue_ts_pass<FT>(h, a, 1, fct);
-    join_streams(h);
+    join_streams(h, FV_CONC_3D);
   } else if (t_on) {
     enqueue_ts_pass<FT>(h, a, 1, fct);
   } else if (s_on) {
@@ -401,15 +410,21 @@ int build_lists(fvgpu_handle* h) {
         if (nq == ia) la = q;
         if (nq == ib) lb = q;
       }
-      if (la < 0 || lb < 0) return fail("a sub-edge's nodes are not nodes of its element");
-      ec[(size_t)j * n1 + e] = c;
-      eab[(size_t)j * n1 + e] = la | (lb << 2);
-      pos[c] = 3 * (e - 1) + j + 1;
+      if (la < 0 || lb < 0 || la == lb) return fail("a sub-edge's nodes are not two nodes of its element");
+      // slot = the element edge: 0 joins NV(e,1)-NV(e,2), 1 NV(e,2)-NV(e,3),
+      // 2 NV(e,3)-NV(e,1); swap when IA is the second node of that pair
+      const int slot = (la + lb == 1) ? 0 : (la + lb == 3) ? 1 : 2;
+      const int swap = la != slot;
+      if (ec[(size_t)slot * n1 + e] != 0) return fail("two sub-edges of an element on one edge");
+      ec[(size_t)slot * n1 + e] = c;
+      eab[(size_t)slot * n1 + e] = swap;
+      pos[c] = 3 * (e - 1) + slot + 1;
+      (void)j;
     }
     for (int e = 1; e <= N; ++e)
       if (ecnt[e] != 3) return fail("an element has fewer than 3 control-volume sub-edges");
     if (dupload(h, &g.ecv_c, ec.data(), ec.size())) return 1;
-    if (dupload(h, &g.ecv_ab, eab.data(), eab.size())) return 1;
+    if (dupload(h, &g.ecv_swap, eab.data(), eab.size())) return 1;
   }
   g.ring_max = 1;
   g.ve_max = 1;
@@ -1187,6 +1202,14 @@ int fvgpu_set_block(fvgpu_handle* h, const char* family, int32_t block) {
   return 0;
 }
 
+int fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode) {
+  if (mode < 0 || mode > 3) return fail("concurrency must be 0..3");
+  h->concurrent = mode;
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  return 0;
+}
+
 int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
   static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
                                "cvflux_q", "adv_ts", "adv_q", "uvedge_adv", "uvedge_uv", "adv3d", "advuv", "vertvl",
@@ -1208,6 +1231,8 @@ int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_
   HIPCHK(hipMemcpyAsync(arena + off, h->clk, sizeof(DevClock), hipMemcpyDeviceToDevice, h->stream));
   const bool prof = h->profiling;
   const std::string only = h->profile_only;
+  const int conc = h->concurrent;
+  h->concurrent = 0;   // each family timed alone
   std::string rep;
   int rc = 0;
   for (const char* f : fams) {
@@ -1238,6 +1263,7 @@ int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_
   }
   h->profiling = prof;
   h->profile_only = only;
+  h->concurrent = conc;
   h->stats.clear();
   off = 0;
   for (auto& b : h->rw) {

@@ -52,11 +52,12 @@ struct DevGrid {
   int ring_max, ve_max;            // max NTSN, max NTVE
   const int *cv_cnt, *cvl_ent, *cvl_elem;
   const double *cvl_sdx, *cvl_sdy;
-  // element -> its 3 sub-edges in increasing index order, slot-major
-  // (slot j of element e at j*n1+e): the sub-edge index, and the positions
-  // in NV(e,1:3) of its nodes IA (bits 0-1) and IB (bits 2-3).  The flux of
-  // slot j of element e is stored at position 3*(e-1)+j+1.
-  const int *ecv_c, *ecv_ab;
+  // element -> its 3 sub-edges, slot-major (slot j of element e at
+  // j*n1+e), slot j being the sub-edge on the element edge from NV(e,j+1)
+  // to NV(e,(j+1)%3+1): the sub-edge index, and 1 if its IA is the second
+  // of those nodes.  The flux of slot j of element e is stored at position
+  // 3*(e-1)+j+1.
+  const int *ecv_c, *ecv_swap;
   const int *ed_ent;               // element -> 3 x (edge<<1 | side), sorted by edge
   const int *fo_off, *fo_obc;      // element -> FLUXOBN contributions (obc<<1 | slot)
   const int *node_obc;             // node -> OBC index or -1

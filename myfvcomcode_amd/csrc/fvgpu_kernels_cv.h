@@ -117,12 +117,14 @@ __device__ __forceinline__ CvFace<FT> cv_face(FT un, FT dtij, FT dltx, FT dlty, 
   return r;
 }
 
-// The three sub-edges of element e: geometry in the flux type, with the
-// positions in NV(e,1:3) of each face's nodes IA and IB.
+// The three sub-edges of element e in edge slots (slot j joins the
+// element's nodes j and j+1 mod 3, called A and B): geometry in the flux
+// type, with the interpolation offsets of A and B (DLTXNCVE/DLTYNCVE of the
+// side that node is on) and whether IA is B.
 template <typename FT>
 struct ElemFaces {
-  FT dltx[3], dlty[3], x1[3], y1[3], x2[3], y2[3], visc0[3];
-  int la[3], lb[3];
+  FT dltx[3], dlty[3], xa[3], ya[3], xb[3], yb[3], visc0[3];
+  bool swap[3];
 };
 
 template <typename FT>
@@ -130,28 +132,36 @@ __device__ __forceinline__ ElemFaces<FT> elem_faces(const DevGrid& g, int e) {
   ElemFaces<FT> f;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const int c = g.ecv_c[(long)j * g.n1 + e], ab = g.ecv_ab[(long)j * g.n1 + e];
-    f.la[j] = ab & 3;
-    f.lb[j] = ab >> 2;
+    const int c = g.ecv_c[(long)j * g.n1 + e];
+    const bool sw = g.ecv_swap[(long)j * g.n1 + e] != 0;
+    f.swap[j] = sw;
     f.dltx[j] = (FT)g.dltxe[c];
     f.dlty[j] = (FT)g.dltye[c];
-    f.x1[j] = (FT)DLTXNCVE_(c, 1);
-    f.y1[j] = (FT)DLTYNCVE_(c, 1);
-    f.x2[j] = (FT)DLTXNCVE_(c, 2);
-    f.y2[j] = (FT)DLTYNCVE_(c, 2);
+    const FT x1 = (FT)DLTXNCVE_(c, 1), y1 = (FT)DLTYNCVE_(c, 1), x2 = (FT)DLTXNCVE_(c, 2), y2 = (FT)DLTYNCVE_(c, 2);
+    f.xa[j] = sw ? x2 : x1;
+    f.ya[j] = sw ? y2 : y1;
+    f.xb[j] = sw ? x1 : x2;
+    f.yb[j] = sw ? y1 : y2;
     f.visc0[j] = (FT)(g.fm1 * 0.5 * (g.nn_hvc[NIEC_(c, 1)] + g.nn_hvc[NIEC_(c, 2)]));
   }
   return f;
 }
 
+// Face value of one side, limited to that node's ring bounds
+// (adv_t.F:447-466).
 template <typename FT>
-__device__ __forceinline__ const CvRecT<FT>& pick3(int l, const CvRecT<FT>& r0, const CvRecT<FT>& r1,
-                                                   const CvRecT<FT>& r2) {
-  return l == 0 ? r0 : (l == 1 ? r1 : r2);
+__device__ __forceinline__ FT face_value(const CvRecT<FT>& r, FT x, FT y) {
+  FT v = r.f + x * r.gx + y * r.gy;
+  if (v < r.fmin) v = r.fmin;
+  if (v > r.fmax) v = r.fmax;
+  return v;
 }
 
 // Phase 2 of ADV_T/ADV_S: the three face fluxes of one element per layer
 // (adv_t.F:440-500), with the geometry held in registers across layers.
+// The two nodes of a face enter the diffusive terms through sums, which are
+// the same in either order, and the upwind value through FIJ1 (side IA) and
+// FIJ2 (side IB), exchanged when IA is node B.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
   const int e = (int)xcd_thread();
@@ -159,22 +169,32 @@ __global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
   const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
   FT* cvflux = (FT*)a.cvflux;
   const ElemFaces<FT> f = elem_faces<FT>(g, e);
-  const int n0 = NV_(e, 1), n1 = NV_(e, 2), n2 = NV_(e, 3);
-  const FT fact = (FT)g.fact;
+  const int nd[3] = {NV_(e, 1), NV_(e, 2), NV_(e, 3)};
+  const FT fact = (FT)g.fact, half = (FT)0.5, one = (FT)1.0;
   const double dt1 = a.dt1[e];
   const long cs = (long)g.NCV + 1, p0 = 3L * (e - 1) + 1;
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
     const FT dtij = (FT)(dt1 * DZ1(e, k));
     const FT vk = (FT)a.v[eo + e], uk = (FT)a.u[eo + e];
-    const CvRecT<FT> r0 = rec[ko + n0], r1 = rec[ko + n1], r2 = rec[ko + n2];
+    CvRecT<FT> r[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r[q] = rec[ko + nd[q]];
     FT* out = cvflux + (long)(k - 1) * cs + p0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
+      const CvRecT<FT>& ra = r[j];
+      const CvRecT<FT>& rb = r[(j + 1) % 3];
+      const FT fa = face_value(ra, f.xa[j], f.ya[j]), fb = face_value(rb, f.xb[j], f.yb[j]);
+      const FT fij1 = f.swap[j] ? fb : fa, fij2 = f.swap[j] ? fa : fb;
       const FT un = vk * f.dltx[j] - uk * f.dlty[j];
-      const CvFace<FT> r = cv_face<FT>(un, dtij, f.dltx[j], f.dlty[j], f.x1[j], f.y1[j], f.x2[j], f.y2[j], fact,
-                                       f.visc0[j], pick3(f.la[j], r0, r1, r2), pick3(f.lb[j], r0, r1, r2));
-      out[j] = r.exflux;
+      const FT viscof = fact * half * (ra.vhn + rb.vhn) + f.visc0[j];
+      const FT txx = half * (ra.gxd + rb.gxd) * viscof;
+      const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
+      const FT fxx = -dtij * txx * f.dlty[j];
+      const FT fyy = dtij * tyy * f.dltx[j];
+      const FT sgn = fsign1(un);
+      out[j] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
     }
   }
 }
@@ -287,7 +307,8 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
 
 // Phase 2 of ADV_Q (adv_q.F:200-330): the three face fluxes of one element
 // on layer interfaces k = 2..KBM1, the element velocity interpolated to the
-// interface with DZ1 weights (once for the three faces).
+// interface with DZ1 weights (once for the three faces); faces as in
+// k_cvflux_ts.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
   const int e = (int)xcd_thread();
@@ -295,8 +316,8 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
   const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
   FT* cvflux = (FT*)a.cvflux;
   const ElemFaces<FT> f = elem_faces<FT>(g, e);
-  const int n0 = NV_(e, 1), n1 = NV_(e, 2), n2 = NV_(e, 3);
-  const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu;
+  const int nd[3] = {NV_(e, 1), NV_(e, 2), NV_(e, 3)};
+  const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu, half = (FT)0.5, one = (FT)1.0;
   const double dt1 = a.dt1[e];
   const long cs = (long)g.NCV + 1, p0 = 3L * (e - 1) + 1;
   double um = a.u[e], vm = a.v[e];   // layer 1
@@ -309,20 +330,17 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
     um = uk;
     vm = vk;
     const FT dtij = (FT)(dt1 * DZZ1(e, k - 1));
-    const CvRecT<FT> r0 = rec[ko + n0], r1 = rec[ko + n1], r2 = rec[ko + n2];
+    CvRecT<FT> r[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r[q] = rec[ko + nd[q]];
     FT* out = cvflux + (long)(k - 1) * cs + p0;
-    const FT half = (FT)0.5, one = (FT)1.0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
+      const CvRecT<FT>& ra = r[j];
+      const CvRecT<FT>& rb = r[(j + 1) % 3];
+      const FT fa = face_value(ra, f.xa[j], f.ya[j]), fb = face_value(rb, f.xb[j], f.yb[j]);
+      const FT fij1 = f.swap[j] ? fb : fa, fij2 = f.swap[j] ? fa : fb;
       const FT un = vq * f.dltx[j] - uq * f.dlty[j];
-      const CvRecT<FT>& ra = pick3(f.la[j], r0, r1, r2);
-      const CvRecT<FT>& rb = pick3(f.lb[j], r0, r1, r2);
-      FT fij1 = ra.f + f.x1[j] * ra.gx + f.y1[j] * ra.gy;
-      FT fij2 = rb.f + f.x2[j] * rb.gx + f.y2[j] * rb.gy;
-      if (fij1 < ra.fmin) fij1 = ra.fmin;
-      if (fij1 > ra.fmax) fij1 = ra.fmax;
-      if (fij2 < rb.fmin) fij2 = rb.fmin;
-      if (fij2 > rb.fmax) fij2 = rb.fmax;
       const FT viscof = (fact * half * (ra.vhn + rb.vhn) + f.visc0[j]) / rhprnu;
       const FT txx = half * (ra.gxd + rb.gxd) * viscof;
       const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
