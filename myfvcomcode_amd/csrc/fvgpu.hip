@@ -263,7 +263,7 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
   FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
   TLAUNCH("bcond_ts", k_bcond_ts, g.IOBCN + 1, g, a, which);
   TLAUNCH("copy", k_copy, mk, f1, ff, mk);
-  TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, f1, which == 1 ? a.t : a.s, g.KB);
+  TLAUNCH("n2e3d", k_n2e3d, g.N, g, f1, which == 1 ? a.t : a.s, g.KB);
 }
 
 // ADV_Q(Q, QF) with FCT_Q2 (adv_q.F, fct_q2.F) on h->cur with the scratch
@@ -298,7 +298,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
 #define VERTVL_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
   FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
-  TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.wts, a.w, g.KB);
+  TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
   TLAUNCH("uvedge_uv", (k_uvedge<1, FT>), g.NE + 1, g, a, a.u, a.v);
@@ -343,7 +343,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
   }
-  TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.km, a.km1, g.KB);
+  TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.km, a.km1, g.KB);
   // ADV_T .. T1 = TF1 on the main stream, the same for S on stream2 (own
   // scratch); the two passes share no written array
   const bool t_on = h->md->p.temperature_active, s_on = h->md->p.salinity_active;
@@ -361,7 +361,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   }
   if (!h->md->p.barotropic) {
     TLAUNCH("dens2", k_dens2, mk, g, a);
-    TLAUNCH("n2e3d", k_n2e3d, (long)g.N * g.KB, g, a.rho1, a.rho, g.KB);
+    TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.rho1, a.rho, g.KB);
   }
   TLAUNCH("copy", k_copy, nk, a.u, a.uf, nk);
   TLAUNCH("copy", k_copy, nk, a.v, a.vf, nk);

@@ -215,14 +215,16 @@ __global__ void k_adjust2(DevGrid g, IntArgs a) {
   }
 }
 
-// N2E3D (mod_main.F:1633-1645) over layers 1..kmax.
-__global__ void k_n2e3d(DevGrid g, const double* nvar, double* evar, int kmax) {
-  const long t = xcd_thread();
-  const long tot = (long)g.N * kmax;
-  if (t >= tot) return;
-  const int i = (int)(t % g.N) + 1, k = (int)(t / g.N) + 1;
+// N2E3D (mod_main.F:1633-1645) over layers 1..kmax, one thread per element
+// column with its three nodes in registers.
+__global__ __launch_bounds__(256) void k_n2e3d(DevGrid g, const double* nvar, double* evar, int kmax) {
+  const int i = (int)xcd_thread() + 1;
+  if (i > g.N) return;
+  const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
   const double one_third = 1.0 / 3.0;
-  evar[EI(g, i, k)] = one_third * (nvar[NI(g, NV_(i, 1), k)] + nvar[NI(g, NV_(i, 2), k)] + nvar[NI(g, NV_(i, 3), k)]);
+#pragma unroll 4
+  for (int k = 1; k <= kmax; ++k)
+    evar[EI(g, i, k)] = one_third * (nvar[NI(g, j1, k)] + nvar[NI(g, j2, k)] + nvar[NI(g, j3, k)]);
 }
 
 // WREAL (wreal.F:32-140).

@@ -100,6 +100,7 @@ def adv_ts(s, real=R8):
     reads = (M * (3 * I4 + 4 * real) + 2 * NCV * I4   # counts, flags; ART1, DT, DTFA, NN_HVC; slot lists
              + NCV * KBM1 * real                       # face fluxes
              + M * KBM1 * real + M * KB * real         # T1, WTS
+             + M * KBM1 * 2 * real                     # FCT ring bounds (RLOHI)
              + M * (real + 8 * real))                  # bottom VISCOFH and record
     writes = M * KBM1 * real + 3 * M * real            # TF1; PFPXB, PFPYB, AH_BOTTOM
     return reads + writes
@@ -108,7 +109,8 @@ def adv_ts(s, real=R8):
 def adv_q(s, real=R8):
     """Ordered sub-edge sums + vertical flux + update (Q2 or Q2L)."""
     M, NCV, KB, KBM1 = s["M"], s["NCV"], s["KB"], s["KBM1"]
-    reads = M * (I4 + 3 * real) + 2 * NCV * I4 + NCV * (KBM1 - 1) * real + 2 * M * KB * real
+    reads = (M * (I4 + 3 * real) + 2 * NCV * I4 + NCV * (KBM1 - 1) * real + 2 * M * KB * real
+             + M * (KBM1 - 1) * 2 * real)              # FCT ring bounds (RLOHI)
     writes = M * KB * real
     return reads + writes
 
@@ -189,9 +191,15 @@ def vdif_ts(s, real=R8):
     return M * KB * 2 * real + M * 8 * real + M * KBM1 * real
 
 
+def n2e3d(s, real=R8):
+    """N2E3D of one node-layer field (KB layers): element mean of its nodes."""
+    M, N, KB = s["M"], s["N"], s["KB"]
+    return M * KB * real + N * 3 * I4 + N * KB * real
+
+
 FAMILIES = {f.__name__: f for f in (
     ext_elevation, advave_edge, ext_momentum, cvrec_ts, cvrec_q, cvflux_ts, cvflux_q, adv_ts, adv_q,
-    uvedge_adv, uvedge_uv, adv3d, advuv, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts)}
+    uvedge_adv, uvedge_uv, adv3d, advuv, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts, n2e3d)}
 
 
 def bytes_per_launch(name, model, real=R8):
