@@ -94,7 +94,9 @@ __global__ __launch_bounds__(256) void k_closure_features(DevGrid g, IntArgs a, 
     vm = vk;
     const long s = (long)(k - 2) * g.M + (i - 1);
     float4* xs = (float4*)(x + s * MLP_IN);
-    xs[0] = make_float4((float)asinh(n2 / 1e-5), (float)(log10(s2 + 1e-10) * 0.25 + 1.0), (float)ZN(i, k),
+    // the arguments in FP64, the transcendentals in FP32 (the features are
+    // FP32 values)
+    xs[0] = make_float4(asinhf((float)(n2 / 1e-5)), log10f((float)(s2 + 1e-10)) * 0.25f + 1.0f, (float)ZN(i, k),
                         ustar / 0.01f);
     xs[1] = make_float4(ld, 1.0f, 0.0f, 0.0f);
     if (y) {
@@ -324,7 +326,25 @@ __global__ void k_mlp_pack(const float* P, MlpPacked* out) {
   if (l < MLP_OUT) out->b3[l] = P[MLP_OFF_B3 + l];
 }
 
-// Inference of n samples; each wavefront loops over groups of 32 samples.
+// SiLU for the BF16 inference: the hardware exp2 and reciprocal (about 1
+// ulp of FP32, far below the BF16 rounding of the activations they feed).
+__device__ __forceinline__ float mlp_silu_fast(float z) {
+  const float log2e = 1.4426950408889634f;
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * log2e));
+}
+// Four FP32 values to a BF16 fragment, round to nearest even (the packed
+// hardware conversion; the same bits as mlp_to_bf16 for finite values).
+__device__ __forceinline__ mlp_bf16x4 mlp_pack_bf16(float a, float b, float c, float d) {
+  mlp_bf16x4 v;
+  v[0] = __builtin_bit_cast(short, (__bf16)a);
+  v[1] = __builtin_bit_cast(short, (__bf16)b);
+  v[2] = __builtin_bit_cast(short, (__bf16)c);
+  v[3] = __builtin_bit_cast(short, (__bf16)d);
+  return v;
+}
+
+// Inference of n samples; each wavefront loops over groups of 32 samples,
+// the next group's inputs loaded before the current group's layers.
 // Accumulator layout of the 32x32 MFMA: lane l, register v holds
 // D[row = 8*(v/4) + 4*(l/32) + v%4][col = l % 32]; with samples as columns,
 // registers 4q..4q+3 of row tile t are the B fragment (k-step 4t+q) of the
@@ -347,14 +367,19 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restri
   for (int s = 0; s < 8; ++s) w3[s] = W->w3[s][l];
   const float b30 = W->b3[0], b31 = W->b3[1];
   const long groups = (n + 31) / 32;
+  // B operand of layer 1: X^T, lane holds inputs 4*half..4*half+3 of its sample
+  auto load_x = [&](long grp) {
+    const long smp = grp * 32 + col;
+    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (grp < groups && smp < n) xv = ((const float4*)(X + smp * MLP_IN))[half];
+    return xv;
+  };
+  float4 xnext = load_x(blockIdx.x);
   for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {
     const long smp = grp * 32 + col;
-    // B operand of layer 1: X^T, lane holds inputs 4*half..4*half+3 of its sample
-    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (smp < n) xv = ((const float4*)(X + smp * MLP_IN))[half];
-    mlp_bf16x4 xb;
-    xb[0] = (short)mlp_to_bf16(xv.x); xb[1] = (short)mlp_to_bf16(xv.y);
-    xb[2] = (short)mlp_to_bf16(xv.z); xb[3] = (short)mlp_to_bf16(xv.w);
+    const float4 xv = xnext;
+    xnext = load_x(grp + gridDim.x);
+    const mlp_bf16x4 xb = mlp_pack_bf16(xv.x, xv.y, xv.z, xv.w);
     mlp_bf16x4 h1b[2][4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -362,8 +387,8 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restri
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1[t], xb, acc, 0, 0, 0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) h1b[t][q][e] = (short)mlp_to_bf16(mlp_silu(acc[4 * q + e]));
+        h1b[t][q] = mlp_pack_bf16(mlp_silu_fast(acc[4 * q]), mlp_silu_fast(acc[4 * q + 1]),
+                                  mlp_silu_fast(acc[4 * q + 2]), mlp_silu_fast(acc[4 * q + 3]));
     }
     mlp_bf16x4 h2b[2][4];
 #pragma unroll
@@ -375,8 +400,8 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restri
       for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[t][s], h1b[s / 4][s % 4], acc, 0, 0, 0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) h2b[t][q][e] = (short)mlp_to_bf16(mlp_silu(acc[4 * q + e]));
+        h2b[t][q] = mlp_pack_bf16(mlp_silu_fast(acc[4 * q]), mlp_silu_fast(acc[4 * q + 1]),
+                                  mlp_silu_fast(acc[4 * q + 2]), mlp_silu_fast(acc[4 * q + 3]));
     }
     mlp_f32x16 o = {};
 #pragma unroll
