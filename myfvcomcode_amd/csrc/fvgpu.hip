@@ -256,8 +256,10 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
 #define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
   FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
   TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.N + 1, g, a);
-#define ADV_TS_CALL(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
-  FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL);
+#define ADV_TS_CALL10(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, 10, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
+#define ADV_TS_CALL16(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, 16, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
+  if (g.ring_max <= 10) FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL10);
+  else FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL16);
   if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
 #define VDIF_TS_CALL(KM) TLAUNCH("vdif_ts", k_vdif_ts_r<KM>, g.M + 1, g, a, ff, which)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
@@ -275,8 +277,10 @@ void enqueue_adv_q(fvgpu_handle* h, IntArgs& a, double* q, double* qf, int fct) 
 #define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
   FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
   TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.N + 1, g, a);
-#define ADV_Q_CALL(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, FT>), g.M + 1, g, a, q, qf, fct)
-  FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL);
+#define ADV_Q_CALL10(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, 10, FT>), g.M + 1, g, a, q, qf, fct)
+#define ADV_Q_CALL16(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, 16, FT>), g.M + 1, g, a, q, qf, fct)
+  if (g.ring_max <= 10) FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL10);
+  else FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL16);
 }
 
 template <typename FT>
@@ -321,8 +325,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     TLAUNCH("closure_apply", k_closure_apply, g.M, g, a, h->cl_yp, a.gx, h->cl_flag, h->cl_nfall);
     TLAUNCH("copy", k_copy, mk, a.q2f, a.q2, mk);
     TLAUNCH("copy", k_copy, mk, a.q2lf, a.q2l, mk);
-#define VDIF_QM_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, h->cl_flag)
-#define VDIF_QM_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, h->cl_flag)
+#define VDIF_QM_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.vhpair, h->cl_flag)
+#define VDIF_QM_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.vhpair, h->cl_flag)
     // (vdif_q also stores Q2 = Q2F, Q2L = Q2LF for its columns; the others
     // hold Q2F = Q2 from the copies above)
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL);
@@ -335,11 +339,10 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     on_stream1(h);
     enqueue_adv_q<FT>(h, a, a.q2, a.q2f, fct);
     join_streams(h, FV_CONC_3D);
-#define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
-#define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.rlohi, (const int*)nullptr)
+#define VDIF_Q_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.vhpair, (const int*)nullptr)
+#define VDIF_Q_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.vhpair, (const int*)nullptr)
     // VDIF_Q, then Q2 = Q2F and Q2L = Q2LF (internal_step.F:1029-1035) in
-    // the same kernel.  `rlohi` is free here (ADV_Q is done, ADV_T/ADV_S
-    // rebuild it) and holds the elimination coefficients.
+    // the same kernel, the elimination coefficients in `vhpair`.
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
   }
@@ -646,7 +649,7 @@ int upload_state(fvgpu_handle* h) {
     a.ef2 = p2; a.ef3 = p3; a.efvisc = pv; a.rec = pr; a.cvflux = pc;
   }
   RW(elij2, nullptr, (size_t)g.NE + 1);
-  if (dalloc(h, &a.rlohi, mk)) return 1;
+  if (dalloc(h, &a.vhpair, mk)) return 1;
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
   RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
   RW(uard, nullptr, n1); RW(vard, nullptr, n1); RW(fluxobn, nullptr, n1);
@@ -669,7 +672,7 @@ int upload_state(fvgpu_handle* h) {
   HIPCHK(hipMemcpy(h->clk, &c, sizeof(c), hipMemcpyHostToDevice));
   a.clk = h->clk;
   // the stream2 chains: the same arrays, their own scratch (node-layer
-  // records and FCT bounds, sub-edge fluxes, bottom gradients, OBC fluxes)
+  // records, face fluxes, bottom gradients, OBC fluxes)
   IntArgs& b = h->ib;
   b = a;
   {
@@ -679,7 +682,6 @@ int upload_state(fvgpu_handle* h) {
     if (dalloc(h, &pc, ((size_t)g.NCV + 1) * g.KB)) return 1;
     b.rec = pr; b.cvflux = pc;
   }
-  if (dalloc(h, &b.rlohi, mk)) return 1;
   if (dalloc(h, &b.pfpxb, m1) || dalloc(h, &b.pfpyb, m1) || dalloc(h, &b.ah_bottom, m1)) return 1;
   if (dalloc(h, &b.xflux_obc, (size_t)g.IOBCN * g.KB)) return 1;
   return 0;

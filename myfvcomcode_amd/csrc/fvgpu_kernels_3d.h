@@ -31,7 +31,7 @@ struct IntArgs {
   void* ef3;             // per edge-layer momentum terms (3-D), (NE+1) x KB EdgeFlux4T<FT>
   void* efvisc;          // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB FT
   void* rec;             // node-layer records of the field being advected, CvRecT<FT>
-  double2* rlohi;        // node-layer min/max of the field over the NBSN ring (FCT bounds)
+  double2* vhpair;       // node-layer pair scratch: VDIF_Q's elimination coefficients
   void* cvflux;          // per sub-edge, per layer face flux (NCV+1) x KB FT
   // 2-D
   double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;

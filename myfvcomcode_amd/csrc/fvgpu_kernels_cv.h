@@ -14,8 +14,10 @@
 //                  element-major positions 3*(e-1)+j+1 (coalesced);
 //   3. k_*_gather  one thread per node: the node's sub-edge fluxes summed in
 //                  increasing sub-edge order (the reference's scatter order,
-//                  so the sum is bit-identical), then the vertical flux and
-//                  the update.
+//                  so the sum is bit-identical), then the vertical flux, the
+//                  update and FCT, whose ring bounds come from the field
+//                  itself (the ring's values are in cache beside the node's
+//                  own column) rather than through HBM from phase 1.
 // Node lists are read slot-major (cvl_ent) so consecutive nodes read
 // consecutive addresses, and workgroups are placed XCD-contiguously.
 #pragma once
@@ -23,10 +25,9 @@
 
 
 // Ring gradient / bounds of field f at node i for the layers [k0, k1]
-// (adv_t.F:219-376 and :469-480; adv_q.F the same on layer interfaces),
-// and the ring-only min/max that FCT_T/FCT_S/FCT_Q2 clip to
-// (fct_t.F:74-80).  RMAX >= NTSN bounds the ring held in registers.  The
-// sums are FP64; the record is stored in the flux type FT.
+// (adv_t.F:219-376 and :469-480; adv_q.F the same on layer interfaces).
+// RMAX >= NTSN bounds the ring held in registers.  The sums are FP64; the
+// record is stored in the flux type FT.
 // kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
 // DZ-weighted interface viscosity (adv_q.F) times NN_HVC.
 template <int KIND, int RMAX, typename FT>
@@ -86,9 +87,35 @@ __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const doubl
     r.gyd = (FT)(s4 / art2);
     r.vhn = (FT)vhn;
     rec[ko + i] = r;
-    // the ring alone (entry NTSN repeats entry 1): the FCT bounds, FP64
-    a.rlohi[ko + i] = make_double2(lo, hi);
   }
+}
+
+// Min and max of field f over the NBSN ring of node i at layer offset ko,
+// the ring alone (entry NTSN repeats entry 1), in the order k_cvrec forms
+// them: the FCT bounds (fct_t.F:74-80, fct_q2.F:60-66).
+template <int RMAX>
+__device__ __forceinline__ void ring_bounds(const int (&rn)[RMAX], int nt, const double* f, long ko, double& lo,
+                                            double& hi) {
+  lo = f[ko + rn[0]];
+  hi = lo;
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j >= 1 && j < nt - 1) {
+      const double v = f[ko + rn[j]];
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+  });
+}
+
+template <int RMAX>
+__device__ __forceinline__ int load_ring(const DevGrid& g, int i, int (&rn)[RMAX]) {
+  const int o = g.nbsn_off[i], nt = g.ntsn[i];
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    rn[j] = j < nt ? g.nbsn[o + j] : 0;
+  });
+  return nt;
 }
 
 // Face flux of ADV_T/ADV_S through sub-edge c at layer k (adv_t.F:440-500),
@@ -231,7 +258,7 @@ __global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
 // Phase 3 of ADV_T/ADV_S (adv_t.F:507-1052): ordered sub-edge sums (in FP64
 // whatever the flux type), bottom gradients for VDIF_TS, vertical flux,
 // rivers, the update TF1 = (T1 - XFLUX/ART1*DTI/(DT*DZ))*DT/DTFA and FCT.
-template <int SMAX, typename FT>
+template <int SMAX, int RMAX, typename FT>
 __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, const double* f1, double* ff,
                                                         const double* fdis, int which, int fct) {
   const int i = (int)xcd_thread();
@@ -246,6 +273,8 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
   const double art1 = g.art1[i];
   const bool isobc = g.isonb[i] == 2;
   const long cs = (long)g.NCV + 1;
+  int rn[RMAX];
+  const int nt = clip ? load_ring<RMAX>(g, i, rn) : 0;
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1;
     const FT* fl = (const FT*)a.cvflux + (long)(k - 1) * cs;
@@ -294,8 +323,8 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
     if (which == 1) xflux = xflux - 0.0 * art1;
     double x = (fk - xflux / art1 * (g.dti / (a.dt[i] * dzk))) * (a.dt[i] / a.dtfa[i]);
     if (clip) {  // FCT_T / FCT_S (fct_t.F:74-101)
-      const double2 lh = a.rlohi[ko + i];
-      double tmin = lh.x, tmax = lh.y;
+      double tmin, tmax;
+      ring_bounds<RMAX>(rn, nt, f1, ko, tmin, tmax);
       if (has_up) { const double v = nu / du; tmax = fmax(tmax, v); tmin = fmin(tmin, v); }
       if (has_dn) { const double v = nd / dd; tmax = fmax(tmax, v); tmin = fmin(tmin, v); }
       if (tmin - x > 0.0) x = tmin;
@@ -353,7 +382,7 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
 }
 
 // Phase 3 of ADV_Q (adv_q.F:340-390): ordered sums, vertical flux, update.
-template <int SMAX, typename FT>
+template <int SMAX, int RMAX, typename FT>
 __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, const double* q, double* qf, int fct) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
@@ -370,6 +399,8 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
   const double art1 = g.art1[i], dti = g.dti, dtn = a.dt[i], dn = a.d[i];
   const long cs = (long)g.NCV + 1;
   const bool clip = fct && !g.fct_skip[i];
+  int rn[RMAX];
+  const int nt = clip ? load_ring<RMAX>(g, i, rn) : 0;
   qf[NI(g, i, 1)] = 0.0;
   qf[NI(g, i, g.KB)] = 0.0;
   for (int k = 2; k <= g.KBM1; ++k) {
@@ -389,8 +420,8 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
     const double qk = q[ko + i];
     double x = (qk - xflux / art1 * (dti / (dtn * DZZ(i, k - 1)))) * (dtn / dn);
     if (clip) {  // FCT_Q2 / FCT_Q2L (fct_q2.F:60-95)
-      const double2 lh = a.rlohi[ko + i];
-      double qmin = lh.x, qmax = lh.y;
+      double qmin, qmax;
+      ring_bounds<RMAX>(rn, nt, q, ko, qmin, qmax);
       if (k != 2) {
         const double v = (qk * DZZ(i, k - 2) + q[ko - g.m1 + i] * DZZ(i, k - 1)) / (DZZ(i, k - 1) + DZZ(i, k - 2));
         qmax = fmax(qmax, v); qmin = fmin(qmin, v);
