@@ -62,6 +62,7 @@ struct fvgpu_handle {
   IntArgs ia{};
   IntArgs ib{};           // ia with its own scratch, for the chain on stream2
   double *xflux_obcn = nullptr, *fobn = nullptr;
+  double* phi = nullptr;  // per sub-edge EXTEL term of the 2-D stage input, element-major
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -219,14 +220,20 @@ ExtArgs make_ext_args(fvgpu_handle* h, int K, const Ext2DState& cur, const Ext2D
   a.qdis = h->ia.qdis;
   a.ef = h->ia.ef2;
   a.elij = h->ia.elij2;
+  a.phi = h->phi;
   a.clk = h->clk;
   return a;
 }
 
+// `fresh`: the 2-D state may have changed since the last stage (first
+// external step of an internal step, or a direct EXTERNAL_STEP call), so
+// the EXTEL terms are formed from it first; otherwise the last stage's
+// k_ext_momentum left them.
 template <typename FT>
-void enqueue_external_step_t(fvgpu_handle* h, int iext) {
+void enqueue_external_step_t(fvgpu_handle* h, int iext, bool fresh) {
   const DevGrid& g = h->g;
   LAUNCH("ext_clock", k_ext_clock, 1, 64, h->clk, iext);
+  if (fresh) TLAUNCH("ext_phi", k_ext_phi, g.N, g, h->ext[0].ua, h->ext[0].va, h->ext[0].d1, h->phi);
   // stage buffers: 1: A->B, 2: B->C, 3: C->B, 4: B->A (in place on the RK base)
   const Ext2DState* cur[5] = {nullptr, &h->ext[0], &h->ext[1], &h->ext[2], &h->ext[1]};
   const Ext2DState* nxt[5] = {nullptr, &h->ext[1], &h->ext[2], &h->ext[1], &h->ext[0]};
@@ -298,7 +305,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   TLAUNCH("advave_int", k_advave_int<FT>, g.N + 1, g, a);
   TLAUNCH("zero", k_zero, g.m1, a.egf, g.m1);
   if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
-  for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext);
+  for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext, iext == 1);
   TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
 #define VERTVL_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
   FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
@@ -372,8 +379,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
 }
 
 void enqueue_external_step(fvgpu_handle* h, int iext) {
-  if (h->precision == FV_MIXED) enqueue_external_step_t<float>(h, iext);
-  else enqueue_external_step_t<double>(h, iext);
+  if (h->precision == FV_MIXED) enqueue_external_step_t<float>(h, iext, true);
+  else enqueue_external_step_t<double>(h, iext, true);
 }
 
 void enqueue_internal_step(fvgpu_handle* h) {
@@ -426,6 +433,11 @@ int build_lists(fvgpu_handle* h) {
     }
     for (int e = 1; e <= N; ++e)
       if (ecnt[e] != 3) return fail("an element has fewer than 3 control-volume sub-edges");
+    std::vector<double> edx(3 * n1, 0.0), edy(3 * n1, 0.0);
+    for (size_t q = 0; q < 3 * n1; ++q)
+      if (ec[q] > 0) { edx[q] = md->dltxe[ec[q]]; edy[q] = md->dltye[ec[q]]; }
+    if (dupload(h, &g.ecv_dx, edx.data(), edx.size())) return 1;
+    if (dupload(h, &g.ecv_dy, edy.data(), edy.size())) return 1;
     if (dupload(h, &g.ecv_c, ec.data(), ec.size())) return 1;
     if (dupload(h, &g.ecv_swap, eab.data(), eab.size())) return 1;
   }
@@ -658,6 +670,7 @@ int upload_state(fvgpu_handle* h) {
   RW(uard_obcn, nullptr, (size_t)g.IOBCN);
 #undef RW
   if (dalloc(h, &h->xflux_obcn, (size_t)g.IOBCN)) return 1;
+  if (dalloc(h, &h->phi, (size_t)g.NCV + 1)) return 1;
   if (dalloc(h, &h->fobn, (size_t)g.IOBCN)) return 1;
   if (dalloc(h, &h->clk, 1)) return 1;
   {

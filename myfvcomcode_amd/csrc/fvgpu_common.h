@@ -58,6 +58,7 @@ struct DevGrid {
   // of those nodes.  The flux of slot j of element e is stored at position
   // 3*(e-1)+j+1.
   const int *ecv_c, *ecv_swap;
+  const double *ecv_dx, *ecv_dy;   // DLTXE/DLTYE of slot j's sub-edge, same layout
   const int *ed_ent;               // element -> 3 x (edge<<1 | side), sorted by edge
   const int *fo_off, *fo_obc;      // element -> FLUXOBN contributions (obc<<1 | slot)
   const int *node_obc;             // node -> OBC index or -1

@@ -45,6 +45,8 @@ struct ExtArgs {
   const double *qdis;    // QDIS (internal-mode river discharge)
   void* ef;              // per-edge ADVAVE terms of the stage input, EdgeFlux4T<FT>
   double* elij;          // per-edge mean elevation of the stage input
+  double* phi;           // per sub-edge EXTEL terms (read for the stage input,
+                         // written by k_ext_momentum for the next stage)
   DevClock* clk;
 };
 
@@ -90,12 +92,16 @@ __device__ __forceinline__ double tide_force(const DevGrid& g, int obc, double r
 __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
+  // EXTEL_EDGE sum over the node's sub-edges in sub-edge order.  The term of
+  // sub-edge c, D1*(-UA*DLTYE + VA*DLTXE) of its element for side IB, is
+  // exactly the negative for side IA (DLTXE/DLTYE negated; round to
+  // nearest is symmetric), so the stored term is added or subtracted.
   double xflux = 0.0;
   const int n = g.cv_cnt[i];
   for (int j = 0; j < n; ++j) {
-    const long q = (long)j * g.m1 + i;
-    const int i1 = g.cvl_elem[q];
-    xflux = xflux + a.cur.d1[i1] * (-a.cur.ua[i1] * g.cvl_sdy[q] + a.cur.va[i1] * g.cvl_sdx[q]);
+    const int ent = g.cvl_ent[(long)j * g.m1 + i];
+    const double t = a.phi[ent >> 1];
+    xflux = (ent & 1) ? xflux + t : xflux - t;
   }
   const int obc = g.node_obc[i];
   double xobc = 0.0;
@@ -228,6 +234,25 @@ __device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const voi
   }
 }
 
+// EXTEL terms of element i's three sub-edges (side IB), element-major.
+__device__ __forceinline__ void ext_phi_store(const DevGrid& g, int i, double ua, double va, double d1,
+                                              double* phi) {
+  const long p0 = 3L * (i - 1) + 1;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const long q = (long)j * g.n1 + i;
+    phi[p0 + j] = d1 * (-ua * g.ecv_dy[q] + va * g.ecv_dx[q]);
+  }
+}
+
+// The EXTEL terms of a 2-D state (before its first stage).
+__global__ __launch_bounds__(256) void k_ext_phi(DevGrid g, const double* ua, const double* va, const double* d1,
+                                                 double* phi) {
+  const int i = (int)xcd_thread() + 1;
+  if (i > g.N) return;
+  ext_phi_store(g, i, ua[i], va[i], d1[i], phi);
+}
+
 template <typename FT>
 __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
   const int i = (int)xcd_thread();
@@ -263,6 +288,7 @@ __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
   a.nxt.va[i] = vaf;
   a.nxt.el1[i] = elf1;
   a.nxt.d1[i] = d1n;
+  ext_phi_store(g, i, uaf, vaf, d1n, a.phi);
   if (a.K == 3) {
     a.uard[i] = a.uard[i] + uaf * d1n;
     a.vard[i] = a.vard[i] + vaf * d1n;

@@ -28,8 +28,8 @@ def ext_elevation(s, real=R8):
     """EXTEL_EDGE gather + elevation update (one RK stage)."""
     M, N, NCV = s["M"], s["N"], s["NCV"]
     reads = (M * (3 * I4 + 3 * real)          # cv_cnt, node_obc, node_riv; ELRK, ART1, H
-             + 2 * NCV * (I4 + 2 * real)      # slot-major sub-edge lists: element, signed DLTXE/DLTYE
-             + N * 3 * real)                  # D1, UA, VA of the stage input
+             + 2 * NCV * I4                   # slot-major sub-edge lists (term position, side)
+             + NCV * real)                    # the stage input's EXTEL term of every sub-edge
     writes = M * 2 * real                     # EL, D of the stage output
     return reads + writes
 
@@ -51,8 +51,9 @@ def ext_momentum(s, real=R8):
     elem_ints = 9     # NV(3), edge list(3), IUCP, ISBCE, river offset
     reads = (N * (elem_reals * real + elem_ints * I4)
              + NE * (4 * real + real + I4 + 2 * real)  # EdgeFlux4, ELIJ, ISBC, DLTXC, DLTYC
+             + N * 6 * real                            # DLTXE/DLTYE of the element's 3 sub-edges
              + M * real)                               # new EL
-    writes = N * 4 * real                              # UA, VA, EL1, D1
+    writes = N * 4 * real + N * 3 * real               # UA, VA, EL1, D1; EXTEL terms of the next stage
     return reads + writes
 
 
