@@ -370,7 +370,8 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
       const FT fa = face_value(ra, f.xa[j], f.ya[j]), fb = face_value(rb, f.xb[j], f.yb[j]);
       const FT fij1 = f.swap[j] ? fb : fa, fij2 = f.swap[j] ? fa : fb;
       const FT un = vq * f.dltx[j] - uq * f.dlty[j];
-      const FT viscof = (fact * half * (ra.vhn + rb.vhn) + f.visc0[j]) / rhprnu;
+      FT viscof = fact * half * (ra.vhn + rb.vhn) + f.visc0[j];
+      if (rhprnu != one) viscof = viscof / rhprnu;   // x / 1 is x: skipped when HPRNU = 1
       const FT txx = half * (ra.gxd + rb.gxd) * viscof;
       const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
       const FT fxx = -dtij * txx * f.dlty[j];
