@@ -284,15 +284,28 @@ __host__ __device__ inline unsigned short mlp_to_bf16(float f) {   // round to n
 }
 
 // BF16 A-operand fragments of the three layers, laid out per lane, built
-// from the FP32 parameters once (k_mlp_pack).  For a 32x32x8 MFMA, lane l
-// holds A[row = l % 32][k = 4*(l/32) .. 4*(l/32)+3].
-//   W1aug: 2 row tiles (hidden 0-31, 32-63), K = 8 (5 inputs, bias, 0, 0)
-//   W2:    2 row tiles x 8 k-steps
-//   W3:    1 row tile (rows 0,1 used) x 8 k-steps
+// from the FP32 parameters once (k_mlp_pack).  Lane l holds row l % 32 of
+// the weights and K elements of lane half l / 32:
+//   W1aug: 32x32x8, 2 row tiles (hidden 0-31, 32-63), K = 8 (5 inputs,
+//          bias, 0, 0), elements k = 4*(l/32) .. +3
+//   W2:    32x32x16, 2 row tiles x 4 K-steps of 16 hidden units (mlp_hid16)
+//   W3:    32x32x16, 1 row tile (rows 0,1 used) x 4 K-steps
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
+
+// Hidden unit fed by element e (0..7) of lane half h in K-step s of a
+// 32x32x16 MFMA whose B operand is taken straight from the accumulators of
+// the previous layer: tile s/2, registers 8*(s%2)..8*(s%2)+7, whose row is
+// 8*(v/4) + 4*h + v%4.  The weights (A operand) are packed with the same map,
+// so every product pairs the right hidden unit whatever the instruction's
+// internal order of K.
+__host__ __device__ inline int mlp_hid16(int s, int h, int e) {
+  return 32 * (s / 2) + 16 * (s % 2) + 8 * (e / 4) + 4 * h + e % 4;
+}
+
 struct MlpPacked {
-  mlp_bf16x4 w1[2][64];
-  mlp_bf16x4 w2[2][8][64];
-  mlp_bf16x4 w3[8][64];
+  mlp_bf16x4 w1[2][64];        // layer 1: 32x32x8, K = the 8 inputs
+  mlp_bf16x8 w2[2][4][64];     // layer 2: tile, K-step of 16, lane
+  mlp_bf16x8 w3[4][64];        // layer 3: K-step of 16, lane (rows 0, 1 used)
   float b2[MLP_H];
   float b3[MLP_OUT];
 };
@@ -300,7 +313,7 @@ struct MlpPacked {
 __global__ void k_mlp_pack(const float* P, MlpPacked* out) {
   const int l = threadIdx.x;   // one wavefront
   if (l >= 64) return;
-  const int r = l % 32, kb = 4 * (l / 32);
+  const int r = l % 32, h = l / 32, kb = 4 * h;
   for (int t = 0; t < 2; ++t) {
     mlp_bf16x4 v;
     for (int q = 0; q < 4; ++q) {
@@ -311,15 +324,16 @@ __global__ void k_mlp_pack(const float* P, MlpPacked* out) {
       v[q] = (short)mlp_to_bf16(w);
     }
     out->w1[t][l] = v;
-    for (int s = 0; s < 8; ++s) {
-      mlp_bf16x4 u;
-      for (int q = 0; q < 4; ++q) u[q] = (short)mlp_to_bf16(P[MLP_OFF_W2 + (32 * t + r) * MLP_H + 8 * s + kb + q]);
+    for (int s = 0; s < 4; ++s) {
+      mlp_bf16x8 u;
+      for (int e = 0; e < 8; ++e) u[e] = (__bf16)P[MLP_OFF_W2 + (32 * t + r) * MLP_H + mlp_hid16(s, h, e)];
       out->w2[t][s][l] = u;
     }
   }
-  for (int s = 0; s < 8; ++s) {
-    mlp_bf16x4 u;
-    for (int q = 0; q < 4; ++q) u[q] = (short)(r < MLP_OUT ? mlp_to_bf16(P[MLP_OFF_W3 + r * MLP_H + 8 * s + kb + q]) : 0);
+  for (int s = 0; s < 4; ++s) {
+    mlp_bf16x8 u;
+    for (int e = 0; e < 8; ++e)
+      u[e] = r < MLP_OUT ? (__bf16)P[MLP_OFF_W3 + r * MLP_H + mlp_hid16(s, h, e)] : (__bf16)0.0f;
     out->w3[s][l] = u;
   }
   if (l < MLP_H) out->b2[l] = P[MLP_OFF_B2 + l];
@@ -343,8 +357,18 @@ __device__ __forceinline__ mlp_bf16x4 mlp_pack_bf16(float a, float b, float c, f
   return v;
 }
 
+// Eight accumulator registers of one tile as the B operand of a K-step.
+__device__ __forceinline__ mlp_bf16x8 mlp_act8(const mlp_f32x16& acc, int p) {
+  mlp_bf16x8 v;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (__bf16)mlp_silu_fast(acc[8 * p + e]);
+  return v;
+}
+
 // Inference of n samples; each wavefront loops over groups of 32 samples,
-// the next group's inputs loaded before the current group's layers.
+// the next group's inputs loaded before the current group's layers.  Layer
+// 1 runs on the 32x32x8 BF16 MFMA (K = the 8 inputs), layers 2 and 3 on
+// the 32x32x16 form with the K order of mlp_hid16.
 // Accumulator layout of the 32x32 MFMA: lane l, register v holds
 // D[row = 8*(v/4) + 4*(l/32) + v%4][col = l % 32]; with samples as columns,
 // registers 4q..4q+3 of row tile t are the B fragment (k-step 4t+q) of the
@@ -353,18 +377,19 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restri
                                                        long n, float* Yp) {
   const int l = threadIdx.x;
   const int col = l % 32, half = l / 32;
-  mlp_bf16x4 w1[2], w2[2][8], w3[8];
+  mlp_bf16x4 w1[2];
+  mlp_bf16x8 w2[2][4], w3[4];
   float b2r[2][16];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     w1[t] = W->w1[t][l];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) w2[t][s] = W->w2[t][s][l];
+    for (int s = 0; s < 4; ++s) w2[t][s] = W->w2[t][s][l];
 #pragma unroll
     for (int v = 0; v < 16; ++v) b2r[t][v] = W->b2[32 * t + 8 * (v / 4) + 4 * half + (v % 4)];
   }
 #pragma unroll
-  for (int s = 0; s < 8; ++s) w3[s] = W->w3[s][l];
+  for (int s = 0; s < 4; ++s) w3[s] = W->w3[s][l];
   const float b30 = W->b3[0], b31 = W->b3[1];
   const long groups = (n + 31) / 32;
   // B operand of layer 1: X^T, lane holds inputs 4*half..4*half+3 of its sample
@@ -380,32 +405,26 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const MlpPacked* __restri
     const float4 xv = xnext;
     xnext = load_x(grp + gridDim.x);
     const mlp_bf16x4 xb = mlp_pack_bf16(xv.x, xv.y, xv.z, xv.w);
-    mlp_bf16x4 h1b[2][4];
+    mlp_f32x16 a1[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      mlp_f32x16 acc = {};
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1[t], xb, acc, 0, 0, 0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        h1b[t][q] = mlp_pack_bf16(mlp_silu_fast(acc[4 * q]), mlp_silu_fast(acc[4 * q + 1]),
-                                  mlp_silu_fast(acc[4 * q + 2]), mlp_silu_fast(acc[4 * q + 3]));
+      const mlp_f32x16 z = {};
+      a1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1[t], xb, z, 0, 0, 0);
     }
-    mlp_bf16x4 h2b[2][4];
+    mlp_bf16x8 h1[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) h1[s] = mlp_act8(a1[s / 2], s % 2);
+    mlp_f32x16 a2[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      mlp_f32x16 acc;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc[v] = b2r[t][v];
+      for (int v = 0; v < 16; ++v) a2[t][v] = b2r[t][v];
 #pragma unroll
-      for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[t][s], h1b[s / 4][s % 4], acc, 0, 0, 0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        h2b[t][q] = mlp_pack_bf16(mlp_silu_fast(acc[4 * q]), mlp_silu_fast(acc[4 * q + 1]),
-                                  mlp_silu_fast(acc[4 * q + 2]), mlp_silu_fast(acc[4 * q + 3]));
+      for (int s = 0; s < 4; ++s) a2[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2[t][s], h1[s], a2[t], 0, 0, 0);
     }
     mlp_f32x16 o = {};
 #pragma unroll
-    for (int s = 0; s < 8; ++s) o = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3[s], h2b[s / 4][s % 4], o, 0, 0, 0);
+    for (int s = 0; s < 4; ++s) o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w3[s], mlp_act8(a2[s / 2], s % 2), o, 0, 0, 0);
     if (half == 0 && smp < n) {   // rows 0 and 1 of the output tile live in lanes 0..31, registers 0 and 1
       Yp[smp * 2 + 0] = mlp_softplus(o[0] + b30) + MLP_KEPS;
       Yp[smp * 2 + 1] = mlp_softplus(o[1] + b31) + MLP_KEPS;
