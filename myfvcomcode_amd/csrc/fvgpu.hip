@@ -408,11 +408,11 @@ int build_lists(fvgpu_handle* h) {
   std::vector<int> pos(NCV + 1, 0);
   {
     const size_t n1 = (size_t)N + 1;
-    std::vector<int> ec(3 * n1, 0), eab(3 * n1, 0), ecnt(n1, 0);
+    std::vector<int> ec(3 * n1, 0), esw(3 * n1, 0), ecnt(n1, 0);
     for (int c = 1; c <= NCV; ++c) {
       const int e = md->ntrg[c];
-      const int j = ecnt[e]++;
-      if (j >= 3) return fail("an element has more than 3 control-volume sub-edges");
+      if (e < 1 || e > N) return fail("a sub-edge's element NTRG is out of range");
+      if (ecnt[e]++ >= 3) return fail("an element has more than 3 control-volume sub-edges");
       const int ia = md->niec[c], ib = md->niec[(size_t)(NCV + 1) + c];
       int la = -1, lb = -1;
       for (int q = 0; q < 3; ++q) {
@@ -427,9 +427,8 @@ int build_lists(fvgpu_handle* h) {
       const int swap = la != slot;
       if (ec[(size_t)slot * n1 + e] != 0) return fail("two sub-edges of an element on one edge");
       ec[(size_t)slot * n1 + e] = c;
-      eab[(size_t)slot * n1 + e] = swap;
+      esw[(size_t)slot * n1 + e] = swap;
       pos[c] = 3 * (e - 1) + slot + 1;
-      (void)j;
     }
     for (int e = 1; e <= N; ++e)
       if (ecnt[e] != 3) return fail("an element has fewer than 3 control-volume sub-edges");
@@ -439,7 +438,7 @@ int build_lists(fvgpu_handle* h) {
     if (dupload(h, &g.ecv_dx, edx.data(), edx.size())) return 1;
     if (dupload(h, &g.ecv_dy, edy.data(), edy.size())) return 1;
     if (dupload(h, &g.ecv_c, ec.data(), ec.size())) return 1;
-    if (dupload(h, &g.ecv_swap, eab.data(), eab.size())) return 1;
+    if (dupload(h, &g.ecv_swap, esw.data(), esw.size())) return 1;
   }
   g.ring_max = 1;
   g.ve_max = 1;
