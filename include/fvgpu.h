@@ -289,6 +289,18 @@ int  fvgpu_external_step(fvgpu_handle* h, int32_t iext);
 int  fvgpu_get_state(fvgpu_handle* h, const char* name, double* host, int64_t len);
 int  fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64_t len);
 int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
+/* Time-varying forcing between internal steps, taking effect from the
+ * next step.  The reference interpolates its forcing files in time inside
+ * the step (UPDATE_RIVERS at IntTime, bcond_gcn.F:633-641; UPDATE_WIND and
+ * ASIMPLE_DRAG at IntTime and at each ExtTime, bcond_gcn.F:661-665,
+ * 878-882); here the caller supplies the values for the coming internal
+ * step, held over its external steps:
+ *   "wind"  UUWIND(0:N) then VVWIND(0:N), element wind speed (m/s); the
+ *           stress follows PSIMPLE_DRAG (mod_bulk.F:58-90).  The model must
+ *           have been created with wind on.
+ *   "qdis"  river discharge per river (m^3/s, before RAMP)
+ *   "tdis", "sdis"  river temperature / salinity per river               */
+int  fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int64_t len);
 /* Kernel-level timing of the last fvgpu_run: average device time of the
  * named kernel family in microseconds, measured with HIP events on the
  * handle's stream.  Returns non-zero if the name is unknown.             */

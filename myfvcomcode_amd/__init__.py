@@ -120,6 +120,7 @@ def gpu_lib():
                                         C.POINTER(C.c_int64)]
         L.fvgpu_set_profiling.argtypes = [vp, C.c_int32]
         L.fvgpu_set_block.argtypes = [vp, C.c_char_p, C.c_int32]
+        L.fvgpu_set_forcing.argtypes = [vp, C.c_char_p, C.c_void_p, C.c_int64]
         L.fvgpu_set_concurrency.argtypes = [vp, C.c_int32]
         L.fvgpu_autotune.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int64]
         fp = C.POINTER(C.c_float)
@@ -383,6 +384,13 @@ class DeviceModel:
 
     def set_block(self, family, size):
         _check(gpu_lib(), gpu_lib().fvgpu_set_block(self._h, family.encode(), int(size)), "fvgpu_set_block")
+
+    def set_forcing(self, name, values):
+        """Replace a forcing from the next step on: "wind" (UUWIND then
+        VVWIND over elements 0..N), "qdis", "tdis" or "sdis" (per river)."""
+        L = gpu_lib()
+        v = np.ascontiguousarray(values, np.float64)
+        _check(L, L.fvgpu_set_forcing(self._h, name.encode(), v.ctypes.data, len(v)), "set_forcing " + name)
 
     def set_concurrency(self, mode):
         """Independent chains of the step on a second stream, a bit mask:

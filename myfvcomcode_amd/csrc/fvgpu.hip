@@ -388,6 +388,20 @@ void enqueue_internal_step(fvgpu_handle* h) {
   else enqueue_internal_step_t<double>(h);
 }
 
+// Wind stress of element wind speeds (PSIMPLE_DRAG, mod_bulk.F:58-90),
+// elements 1..n1-1.
+void wind_stress(const double* uu, const double* vv, size_t n1, double* wx, double* wy) {
+  for (size_t i = 1; i < n1; ++i) {
+    const double tx = uu[i], ty = vv[i];
+    const double wds = std::sqrt(tx * tx + ty * ty);
+    double cd = (double)1.2E-3f;
+    if (wds >= 11.0) cd = (0.49 + 0.065 * wds) * 1.E-3;
+    if (wds >= 25.0) cd = (0.49 + 0.065 * 25.0) * 1.E-3;
+    wx[i] = 1.2 * cd * tx * wds;
+    wy[i] = 1.2 * cd * ty * wds;
+  }
+}
+
 // ------------------------------------------------------ derived structures
 int build_lists(fvgpu_handle* h) {
   const fv_model* md = h->md;
@@ -571,18 +585,8 @@ int upload_grid(fvgpu_handle* h) {
   for (size_t i = 0; i < nr; ++i) { cq[i] = std::cos(md->angleq[i]); sq_[i] = std::sin(md->angleq[i]); }
   if (dupload(h, &g.cos_angleq, cq.data(), nr)) return 1;
   if (dupload(h, &g.sin_angleq, sq_.data(), nr)) return 1;
-  // wind stress of the constant wind speed (PSIMPLE_DRAG, mod_bulk.F:58-90)
   std::vector<double> wx(n1, 0.0), wy(n1, 0.0);
-  if (p.wind_on)
-    for (size_t i = 1; i < n1; ++i) {
-      const double tx = md->uuwind[i], ty = md->vvwind[i];
-      const double wds = std::sqrt(tx * tx + ty * ty);
-      double cd = (double)1.2E-3f;
-      if (wds >= 11.0) cd = (0.49 + 0.065 * wds) * 1.E-3;
-      if (wds >= 25.0) cd = (0.49 + 0.065 * 25.0) * 1.E-3;
-      wx[i] = 1.2 * cd * tx * wds;
-      wy[i] = 1.2 * cd * ty * wds;
-    }
+  if (p.wind_on) wind_stress(md->uuwind, md->vvwind, n1, wx.data(), wy.data());
   if (dupload(h, &g.wsx, wx.data(), n1)) return 1;
   if (dupload(h, &g.wsy, wy.data(), n1)) return 1;
   // sigma: one column when every column is identical
@@ -821,6 +825,28 @@ int fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64
   if (!p) return fail(std::string("unknown state array ") + name);
   if ((size_t)len != l) return fail(std::string("length mismatch for ") + name);
   HIPCHK(hipMemcpyAsync(p, host, l * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int64_t len) {
+  const DevGrid& g = h->g;
+  const std::string n = name ? name : "";
+  const size_t n1 = g.n1, nr = (size_t)g.NUMQBC;
+  if (n == "wind") {
+    if (!h->md->p.wind_on) return fail("the model was created without wind");
+    if ((size_t)len != 2 * n1) return fail("wind: expected UUWIND then VVWIND, 2*(N+1) values");
+    std::vector<double> wx(n1, 0.0), wy(n1, 0.0);
+    wind_stress(host, host + n1, n1, wx.data(), wy.data());
+    HIPCHK(hipMemcpyAsync((void*)g.wsx, wx.data(), n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync((void*)g.wsy, wy.data(), n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  } else if (n == "qdis" || n == "tdis" || n == "sdis") {
+    if ((size_t)len != nr) return fail(n + ": expected one value per river (NUMQBC)");
+    const double* dst = n == "qdis" ? g.qdis0 : n == "tdis" ? g.tdis : g.sdis;
+    if (nr) HIPCHK(hipMemcpyAsync((void*)dst, host, nr * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  } else {
+    return fail("unknown forcing " + n + " (wind, qdis, tdis, sdis)");
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }

@@ -155,6 +155,16 @@ module fvgpu
       integer(c_int64_t), value :: n
       integer(c_int) :: rc
     end function
+    ! time-varying forcing for the coming internal step: "wind" (UUWIND(0:N)
+    ! then VVWIND(0:N)), "qdis", "tdis", "sdis" (per river)
+    function fvgpu_set_forcing(h, name, host, n) bind(c, name="fvgpu_set_forcing") result(rc)
+      import :: c_int, c_int64_t, c_ptr, c_char, c_double
+      type(c_ptr), value :: h
+      character(kind=c_char) :: name(*)
+      real(c_double) :: host(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
     function fvgpu_autotune(h, steps, report, report_len) bind(c, name="fvgpu_autotune") result(rc)
       import :: c_int, c_int32_t, c_int64_t, c_ptr
       type(c_ptr), value :: h

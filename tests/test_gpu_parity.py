@@ -134,3 +134,52 @@ def test_state_written_between_steps(built, graph):
     rep = {}
     worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
     assert not bad, bad
+
+
+def test_forcing_changed_between_steps(built):
+    """Wind speed changed between internal steps through fvgpu_set_forcing
+    advances exactly as the oracle reading the changed wind."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    m = F.Model.build(1, nx=61, ny=31, kb=9, n_seamounts=2)
+    assert m.hdr.p.wind_on
+    dev, orc = F.DeviceModel(m), Oracle(m)
+    print("case: jittered mesh 61x31 KB=9 with wind, 2 steps, wind changed, 2 more", flush=True)
+    dev.run(2, graph=True)
+    orc.run(2)
+    uu, vv = m.field("uuwind"), m.field("vvwind")
+    rng = np.random.default_rng(5)
+    uu[1:] = 1.5 * uu[1:] + rng.standard_normal(uu.size - 1)    # the oracle reads the model's arrays
+    vv[1:] = 0.5 * vv[1:] - 3.0
+    dev.set_forcing("wind", np.concatenate([uu, vv]))
+    dev.run(2, graph=True)
+    orc.run(2)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    assert not bad, bad
+
+
+def test_river_forcing_changed_between_steps(built):
+    """River discharge, temperature and salinity changed between internal
+    steps through fvgpu_set_forcing advance exactly as in the oracle."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    m = F.Model.build(0)
+    q, t, s = m.field("qdis0"), m.field("tdis"), m.field("sdis")
+    assert q.size >= 1
+    dev, orc = F.DeviceModel(m), Oracle(m)
+    print("case: 81x21 estuary channel, 2 steps, river changed, 3 more", flush=True)
+    dev.run(2, graph=False)
+    orc.run(2)
+    q[:] = 3.0 * q
+    t[:] = t + 4.0
+    s[:] = s + 1.5
+    for name, v in (("qdis", q), ("tdis", t), ("sdis", s)):
+        dev.set_forcing(name, v)
+    dev.run(3, graph=False)
+    orc.run(3)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    assert not bad, bad
+    with pytest.raises(RuntimeError):
+        dev.set_forcing("wind", np.zeros(2 * (m.n + 1)))    # the channel has no wind
