@@ -289,6 +289,16 @@ int  fvgpu_external_step(fvgpu_handle* h, int32_t iext);
 int  fvgpu_get_state(fvgpu_handle* h, const char* name, double* host, int64_t len);
 int  fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64_t len);
 int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
+/* EnKF ensemble update A <- A*X5 (the update that ends ANALYSIS,
+ * mod_enkfassim.F:2746-3010, applied by ENKF_ASS_EVE to every member's
+ * state, mod_enkfassim.F:308-753) for one state array of a batched union
+ * (fv_model_union), in place on the device: member i (0-based, nrens <= 64)
+ * owns nodes node_off[i]+1 .. node_off[i]+m and elements elem_off[i]+1 ..
+ * elem_off[i]+n of the union; x5 is nrens x nrens, column-major
+ * (X5(i,j) at x5[i + j*nrens]).  Each new value is the sum over i in
+ * increasing order of A(p,i)*X5(i,j).                                    */
+int  fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                           const int64_t* elem_off, int64_t m, int64_t n, const double* x5);
 /* Time-varying forcing between internal steps, taking effect from the
  * next step.  The reference interpolates its forcing files in time inside
  * the step (UPDATE_RIVERS at IntTime, bcond_gcn.F:633-641; UPDATE_WIND and

@@ -63,6 +63,7 @@ struct fvgpu_handle {
   IntArgs ib{};           // ia with its own scratch, for the chain on stream2
   double *xflux_obcn = nullptr, *fobn = nullptr;
   double* phi = nullptr;  // per sub-edge EXTEL term of the 2-D stage input, element-major
+  double* ens_buf = nullptr;  // X5 and member offsets of fvgpu_ensemble_update
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -825,6 +826,47 @@ int fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64
   if (!p) return fail(std::string("unknown state array ") + name);
   if ((size_t)len != l) return fail(std::string("length mismatch for ") + name);
   HIPCHK(hipMemcpyAsync(p, host, l * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                          const int64_t* elem_off, int64_t m, int64_t n, const double* x5) {
+  const DevGrid& g = h->g;
+  if (nrens < 2 || nrens > FV_ENS_MAX) return fail("ensemble size must be 2..64");
+  size_t l = 0;
+  double* arr = lookup(h, name ? name : "", &l);
+  if (!arr) return fail(std::string("unknown state array ") + (name ? name : ""));
+  const size_t m1 = g.m1, n1 = g.n1, kb = (size_t)g.KB;
+  bool node;
+  int nl;
+  if (l == m1) { node = true; nl = 1; }
+  else if (l == n1) { node = false; nl = 1; }
+  else if (l == m1 * kb) { node = true; nl = (int)kb; }
+  else if (l == n1 * kb) { node = false; nl = (int)kb; }
+  else return fail("ensemble update: array extent not recognised");
+  const int64_t len = node ? m : n;
+  const int64_t* off = node ? node_off : elem_off;
+  const size_t ls = node ? m1 : n1;
+  if (len < 1) return fail("ensemble update: empty member slice");
+  for (int i = 0; i < nrens; ++i)
+    if (off[i] < 0 || (size_t)(off[i] + len + 1) > ls) return fail("ensemble update: member slice outside the array");
+  if (!h->ens_buf) {
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, (FV_ENS_MAX * FV_ENS_MAX + FV_ENS_MAX) * sizeof(double)));
+    h->allocs.push_back(q);
+    h->ens_buf = (double*)q;
+  }
+  double* dx5 = h->ens_buf;
+  long* doff = (long*)(h->ens_buf + FV_ENS_MAX * FV_ENS_MAX);
+  std::vector<long> offl(off, off + nrens);
+  HIPCHK(hipMemcpyAsync(dx5, x5, (size_t)nrens * nrens * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(doff, offl.data(), (size_t)nrens * sizeof(long), hipMemcpyHostToDevice, h->stream));
+  const long total = (long)len * nl;
+  const unsigned blocks = (unsigned)((total + FV_ENS_TILE - 1) / FV_ENS_TILE);
+  hipLaunchKernelGGL(k_ens_update, dim3(blocks), dim3(256), 0, h->stream, arr, (const long*)doff, (long)len,
+                     (long)ls, nl, (int)nrens, (const double*)dx5);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }

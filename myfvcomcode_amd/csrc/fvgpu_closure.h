@@ -462,3 +462,42 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
     a.kh[q] = (double)fmaxf(Yp[2 * s + 1], 1e-6f);
   }
 }
+
+// ------------------------------------------------------------ ensemble update
+// A <- A*X5, the ensemble update of the EnKF analysis (`multa`, the last
+// step of ANALYSIS, mod_enkfassim.F:2746-3010), over the member slices of
+// one array of a batched union: member i's entries of layer k are
+// arr[k*ls + off[i] + 1 + p], p < len, for nl layers.  A block stages a tile
+// of FV_ENS_TILE positions of every member and X5 in LDS, then writes each
+// member's new values in place; each output is summed over the members in
+// increasing order (sum_i A(p,i)*X5(i,j)), so a plain loop reproduces it.
+#define FV_ENS_MAX 64
+#define FV_ENS_TILE 32
+__global__ __launch_bounds__(256) void k_ens_update(double* arr, const long* off, long len, long ls, int nl,
+                                                    int nrens, const double* x5) {
+  __shared__ double sx[FV_ENS_MAX * FV_ENS_MAX];
+  __shared__ double sa[FV_ENS_MAX][FV_ENS_TILE + 1];
+  for (int t = threadIdx.x; t < nrens * nrens; t += blockDim.x) sx[t] = x5[t];
+  const long total = len * nl, p0 = (long)blockIdx.x * FV_ENS_TILE;
+  for (int t = threadIdx.x; t < nrens * FV_ENS_TILE; t += blockDim.x) {
+    const int i = t / FV_ENS_TILE, q = t % FV_ENS_TILE;
+    const long p = p0 + q;
+    double v = 0.0;
+    if (p < total) {
+      const long k = p / len, loc = p - k * len;
+      v = arr[k * ls + off[i] + 1 + loc];
+    }
+    sa[i][q] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nrens * FV_ENS_TILE; t += blockDim.x) {
+    const int j = t / FV_ENS_TILE, q = t % FV_ENS_TILE;
+    const long p = p0 + q;
+    if (p >= total) continue;
+    double acc = 0.0;
+    for (int i = 0; i < nrens; ++i) acc = acc + sa[i][q] * sx[i + j * nrens];
+    const long k = p / len, loc = p - k * len;
+    arr[k * ls + off[j] + 1 + loc] = acc;
+  }
+}
+
