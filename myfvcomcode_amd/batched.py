@@ -45,7 +45,30 @@ def build(members=64, hetero=True, log=None):
     return parts, union, offs
 
 
-def run(members=64, steps=5, precision="mixed", hetero=True, tune=True, log=None):
+def time_analysis(dev, union, offs, members, reps=3):
+    """One EnKF ensemble update (A <- A*X5, fvgpu_ensemble_update) of every
+    prognostic array of the members: average seconds and the bytes each
+    value moves (read once, written once)."""
+    from . import enkf
+    ens = enkf.Ensemble(union, offs, members)
+    X5 = np.eye(members) + 0.01 * np.random.default_rng(0).standard_normal((members, members))
+    names = ("el", "ua", "va", "u", "v", "t1", "s1")
+    nbytes = 0
+    kb = union.kb
+    for nm in names:
+        per_layer = ens.m if nm in ("el", "t1", "s1") else ens.n
+        nbytes += 2 * 8 * members * per_layer * (kb if nm in ("u", "v", "t1", "s1") else 1)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for nm in names:
+            ens.update(dev, nm, X5)
+    dev.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"arrays": list(names), "seconds": dt, "bytes": nbytes, "GBps": nbytes / dt / 1e9}
+
+
+def run(members=64, steps=5, precision="mixed", hetero=True, tune=True, log=None, analysis=False):
     t0 = time.time()
     parts, union, offs = build(members, hetero, log)
     if log:
@@ -84,6 +107,8 @@ def run(members=64, steps=5, precision="mixed", hetero=True, tune=True, log=None
             a, b = a.reshape(kb, -1)[:, on + 1: on + n + 1], b.reshape(kb, -1)[:, 1:]
         same = same and bool(np.array_equal(a, b))
     out["member0_matches_alone"] = same
+    if analysis:
+        out["enkf_update"] = time_analysis(dev, union, offs, members)
     return out
 
 
@@ -94,10 +119,11 @@ def main(argv=None):
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-hetero", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--analysis", action="store_true", help="also time one EnKF ensemble update of the members")
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
-    print(json.dumps(run(args.members, args.steps, args.precision, not args.no_hetero, not args.no_tune, log)),
-          flush=True)
+    print(json.dumps(run(args.members, args.steps, args.precision, not args.no_hetero, not args.no_tune, log,
+                         args.analysis)), flush=True)
     return 0
 
 
