@@ -469,35 +469,63 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
 // one array of a batched union: member i's entries of layer k are
 // arr[k*ls + off[i] + 1 + p], p < len, for nl layers.  A block stages a tile
 // of FV_ENS_TILE positions of every member and X5 in LDS, then writes each
-// member's new values in place; each output is summed over the members in
-// increasing order (sum_i A(p,i)*X5(i,j)), so a plain loop reproduces it.
+// member's new values in place.  Each of the 256 threads forms 4 positions
+// x 4 members (register blocking: 8 LDS reads per 16 FMAs); each output is
+// summed over the members in increasing order (sum_i A(p,i)*X5(i,j)), so a
+// plain loop reproduces it.
 #define FV_ENS_MAX 64
-#define FV_ENS_TILE 32
+#define FV_ENS_TILE 64
 __global__ __launch_bounds__(256) void k_ens_update(double* arr, const long* off, long len, long ls, int nl,
                                                     int nrens, const double* x5) {
   __shared__ double sx[FV_ENS_MAX * FV_ENS_MAX];
   __shared__ double sa[FV_ENS_MAX][FV_ENS_TILE + 1];
+  __shared__ long so[FV_ENS_MAX];
   for (int t = threadIdx.x; t < nrens * nrens; t += blockDim.x) sx[t] = x5[t];
+  if (threadIdx.x < nrens) so[threadIdx.x] = off[threadIdx.x];
+  __syncthreads();
   const long total = len * nl, p0 = (long)blockIdx.x * FV_ENS_TILE;
-  for (int t = threadIdx.x; t < nrens * FV_ENS_TILE; t += blockDim.x) {
-    const int i = t / FV_ENS_TILE, q = t % FV_ENS_TILE;
+  {
+    // 256 threads: position q = t % 64 is the same in every pass, members
+    // t / 64, t / 64 + 4, ...; the layer split of the position once
+    const int q = threadIdx.x % FV_ENS_TILE, i0 = threadIdx.x / FV_ENS_TILE;
     const long p = p0 + q;
-    double v = 0.0;
+    long base = -1;
     if (p < total) {
-      const long k = p / len, loc = p - k * len;
-      v = arr[k * ls + off[i] + 1 + loc];
+      const long k = p / len;
+      base = k * ls + 1 + (p - k * len);
     }
-    sa[i][q] = v;
+    for (int i = i0; i < nrens; i += 256 / FV_ENS_TILE) sa[i][q] = base >= 0 ? arr[base + so[i]] : 0.0;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < nrens * FV_ENS_TILE; t += blockDim.x) {
-    const int j = t / FV_ENS_TILE, q = t % FV_ENS_TILE;
-    const long p = p0 + q;
+  const int tq = threadIdx.x % 16, tj = threadIdx.x / 16;   // positions tq+16r, members tj+16c
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+  for (int i = 0; i < nrens; ++i) {
+    double av[4], xv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) av[r] = sa[i][tq + 16 * r];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = tj + 16 * c;
+      xv[c] = j < nrens ? sx[i + j * nrens] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[r][c] = acc[r][c] + av[r] * xv[c];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const long p = p0 + tq + 16 * r;
     if (p >= total) continue;
-    double acc = 0.0;
-    for (int i = 0; i < nrens; ++i) acc = acc + sa[i][q] * sx[i + j * nrens];
-    const long k = p / len, loc = p - k * len;
-    arr[k * ls + off[j] + 1 + loc] = acc;
+    const long k = p / len, base = k * ls + 1 + (p - k * len);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = tj + 16 * c;
+      if (j < nrens) arr[base + so[j]] = acc[r][c];
+    }
   }
 }
-
