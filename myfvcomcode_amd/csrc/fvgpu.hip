@@ -271,7 +271,8 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
   if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
 #define VDIF_TS_CALL(KM) TLAUNCH("vdif_ts", k_vdif_ts_r<KM>, g.M + 1, g, a, ff, which)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
-  TLAUNCH("bcond_ts", k_bcond_ts, g.IOBCN + 1, g, a, which);
+  TLAUNCH("bcond_ts", k_bcond_ts_sums, g.IOBCN + 1, g, a, which, a.obcsum);
+  if (g.IOBCN > 0) TLAUNCH("bcond_ts", k_bcond_ts_layers, (long)g.IOBCN * g.KBM1, g, a, which, (const double*)a.obcsum);
   TLAUNCH("copy", k_copy, mk, f1, ff, mk);
   TLAUNCH("n2e3d", k_n2e3d, g.N, g, f1, which == 1 ? a.t : a.s, g.KB);
 }
@@ -666,6 +667,7 @@ int upload_state(fvgpu_handle* h) {
   }
   RW(elij2, nullptr, (size_t)g.NE + 1);
   if (dalloc(h, &a.vhpair, mk)) return 1;
+  if (dalloc(h, &a.obcsum, 3 * (size_t)g.IOBCN)) return 1;
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
   RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
   RW(uard, nullptr, n1); RW(vard, nullptr, n1); RW(fluxobn, nullptr, n1);
@@ -701,6 +703,7 @@ int upload_state(fvgpu_handle* h) {
   }
   if (dalloc(h, &b.pfpxb, m1) || dalloc(h, &b.pfpyb, m1) || dalloc(h, &b.ah_bottom, m1)) return 1;
   if (dalloc(h, &b.xflux_obc, (size_t)g.IOBCN * g.KB)) return 1;
+  if (dalloc(h, &b.obcsum, 3 * (size_t)g.IOBCN)) return 1;
   return 0;
 }
 

@@ -32,6 +32,7 @@ struct IntArgs {
   void* efvisc;          // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB FT
   void* rec;             // node-layer records of the field being advected, CvRecT<FT>
   double2* vhpair;       // node-layer pair scratch: VDIF_Q's elimination coefficients
+  double* obcsum;        // BCOND_TS column sums, 3 per OBC node
   void* cvflux;          // per sub-edge, per layer face flux (NCV+1) x KB FT
   // 2-D
   double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;

@@ -5,67 +5,84 @@
 #pragma once
 #include "fvgpu_kernels_3d.h"
 
-// BCOND_TS(NCON) (bcond_ts.F:28-268), one thread per OBC node; thread 0
-// also sets the specified river values.  The reference sets TF1 and SF1 at
-// the river nodes in both calls; each call here sets its own tracer only,
-// which leaves T1 and S1 unchanged (each copy F1 = FF follows its own call)
-// and lets the T and S passes run concurrently.
-__global__ void k_bcond_ts(DevGrid g, IntArgs a, int ncon) {
+// BCOND_TS(NCON) (bcond_ts.F:28-268) in two kernels.  Part 1, one thread
+// per OBC node: the column sums of the node (T2D, the interior neighbour's
+// T2D_NEXT, the OBC advective flux XFLUX2D) in layer order, into
+// sums[3*t .. 3*t+2]; thread 0 also sets the specified river values first.
+// The reference sets TF1 and SF1 at the river nodes in both calls; each call
+// here sets its own tracer only, which leaves T1 and S1 unchanged (each copy
+// F1 = FF follows its own call) and lets the T and S passes run
+// concurrently.
+__global__ void k_bcond_ts_sums(DevGrid g, IntArgs a, int ncon, double* sums) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double* ff = ncon == 1 ? a.tf1 : a.sf1;
+  const double* ff = ncon == 1 ? a.tf1 : a.sf1;
   const double* f1 = ncon == 1 ? a.t1 : a.s1;
   if (t == 0 && g.river_ts_specified) {
+    double* fw = ncon == 1 ? a.tf1 : a.sf1;
     const double* fdis = ncon == 1 ? g.tdis : g.sdis;
     for (int r = 0; r < g.NUMQBC; ++r) {
       const int j11 = g.inodeq[r];
-      for (int k = 1; k <= g.KBM1; ++k) ff[NI(g, j11, k)] = fdis[r];
+      for (int k = 1; k <= g.KBM1; ++k) fw[NI(g, j11, k)] = fdis[r];
     }
   }
   if (!g.obc_on || t >= g.IOBCN) return;
   const int j = g.i_obc_n[t], j1 = g.next_obc[t];
-  const int KBM1 = g.KBM1;
   double t2d = 0.0, t2d_next = 0.0, xflux2d = 0.0;
-  for (int k = 1; k <= KBM1; ++k) {
+  for (int k = 1; k <= g.KBM1; ++k) {
     t2d = t2d + f1[NI(g, j, k)] * DZ(j, k);
     t2d_next = t2d_next + ff[NI(g, j1, k)] * DZ(j1, k);
     xflux2d = xflux2d + a.xflux_obc[(long)(k - 1) * g.IOBCN + t];
   }
-  if (a.uard_obcn[t] > 0.0) {
-    const double tmp = xflux2d + t2d * a.uard_obcn[t];
-    const double t2d_obc = (t2d * a.dt[j] - tmp * g.dti / g.art1[j]) / a.d[j];
-    const double cc = sqrt(g.grav_n[j] * g.h[j]) * g.dti / g.dltn_obc[t], cp = cc + 1.0;
-    double ttmp[FV_MAXKB + 1];
-    for (int k = 1; k <= KBM1; ++k) {
-      const double pert_next = ff[NI(g, j1, k)] - t2d_next, pert = f1[NI(g, j, k)] - t2d;
-      if (g.type_tsobc == 1) ttmp[k] = ff[NI(g, j1, k)] - t2d_next;
-      else if (g.type_tsobc == 2) ttmp[k] = (cc * pert_next + pert) / cp;
-      else ttmp[k] = (cc * pert_next + pert * (1.0 - g.dti / 10800.0)) / cp;
-    }
-    const int o = g.nbsn_off[j], nt = g.ntsn[j];
-    for (int k = 1; k <= KBM1; ++k) {
-      double x = t2d_obc + ttmp[k];
-      const long ko = (long)(k - 1) * g.m1;
-      double tmax = f1[ko + g.nbsn[o]], tmin = tmax;
-      for (int jj = 1; jj < nt; ++jj) { const double y = f1[ko + g.nbsn[o + jj]]; tmax = fmax(tmax, y); tmin = fmin(tmin, y); }
-      const double fk = f1[ko + j];
-      if (k == 1) {
-        const double v = (fk * DZ(j, k + 1) + f1[ko + g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k + 1));
-        tmax = fmax(tmax, v); tmin = fmin(tmin, v);
-      } else if (k == KBM1) {
-        const double v = (fk * DZ(j, k - 1) + f1[ko - g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k - 1));
-        tmax = fmax(tmax, v); tmin = fmin(tmin, v);
-      } else {
-        const double v1 = (fk * DZ(j, k - 1) + f1[ko - g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k - 1));
-        const double v2 = (fk * DZ(j, k + 1) + f1[ko + g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k + 1));
-        tmax = fmax(tmax, fmax(v1, v2)); tmin = fmin(tmin, fmin(v1, v2));
-      }
-      if (tmin - x > 0.0) x = tmin;
-      if (x - tmax > 0.0) x = tmax;
-      ff[ko + j] = x;
-    }
-  } else {
-    for (int k = 1; k <= KBM1; ++k) ff[NI(g, j, k)] = f1[NI(g, j, k)];
+  sums[3 * t] = t2d;
+  sums[3 * t + 1] = t2d_next;
+  sums[3 * t + 2] = xflux2d;
+}
+
+// Part 2, one thread per OBC node and layer: the radiated value (TYPE_TSOBC
+// 1-3) limited to the ring and vertical-neighbour bounds, or F1 itself when
+// the node has no inflow.  NEXT_OBC is an interior node, so no thread reads
+// what another writes.
+__global__ void k_bcond_ts_layers(DevGrid g, IntArgs a, int ncon, const double* sums) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!g.obc_on || idx >= (long)g.IOBCN * g.KBM1) return;
+  const int t = (int)(idx % g.IOBCN), k = (int)(idx / g.IOBCN) + 1;
+  double* ff = ncon == 1 ? a.tf1 : a.sf1;
+  const double* f1 = ncon == 1 ? a.t1 : a.s1;
+  const int j = g.i_obc_n[t], j1 = g.next_obc[t];
+  const int KBM1 = g.KBM1;
+  if (!(a.uard_obcn[t] > 0.0)) {
+    ff[NI(g, j, k)] = f1[NI(g, j, k)];
+    return;
   }
+  const double t2d = sums[3 * t], t2d_next = sums[3 * t + 1], xflux2d = sums[3 * t + 2];
+  const double tmp = xflux2d + t2d * a.uard_obcn[t];
+  const double t2d_obc = (t2d * a.dt[j] - tmp * g.dti / g.art1[j]) / a.d[j];
+  const double cc = sqrt(g.grav_n[j] * g.h[j]) * g.dti / g.dltn_obc[t], cp = cc + 1.0;
+  const double pert_next = ff[NI(g, j1, k)] - t2d_next, pert = f1[NI(g, j, k)] - t2d;
+  double ttmp;
+  if (g.type_tsobc == 1) ttmp = ff[NI(g, j1, k)] - t2d_next;
+  else if (g.type_tsobc == 2) ttmp = (cc * pert_next + pert) / cp;
+  else ttmp = (cc * pert_next + pert * (1.0 - g.dti / 10800.0)) / cp;
+  double x = t2d_obc + ttmp;
+  const long ko = (long)(k - 1) * g.m1;
+  const int o = g.nbsn_off[j], nt = g.ntsn[j];
+  double tmax = f1[ko + g.nbsn[o]], tmin = tmax;
+  for (int jj = 1; jj < nt; ++jj) { const double y = f1[ko + g.nbsn[o + jj]]; tmax = fmax(tmax, y); tmin = fmin(tmin, y); }
+  const double fk = f1[ko + j];
+  if (k == 1) {
+    const double v = (fk * DZ(j, k + 1) + f1[ko + g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k + 1));
+    tmax = fmax(tmax, v); tmin = fmin(tmin, v);
+  } else if (k == KBM1) {
+    const double v = (fk * DZ(j, k - 1) + f1[ko - g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k - 1));
+    tmax = fmax(tmax, v); tmin = fmin(tmin, v);
+  } else {
+    const double v1 = (fk * DZ(j, k - 1) + f1[ko - g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k - 1));
+    const double v2 = (fk * DZ(j, k + 1) + f1[ko + g.m1 + j] * DZ(j, k)) / (DZ(j, k) + DZ(j, k + 1));
+    tmax = fmax(tmax, fmax(v1, v2)); tmin = fmin(tmin, fmin(v1, v2));
+  }
+  if (tmin - x > 0.0) x = tmin;
+  if (x - tmax > 0.0) x = tmax;
+  ff[ko + j] = x;
 }
 
 // DENS2 (eqs_of_state.F:254-403): RHO1 at every node layer, then zero the
