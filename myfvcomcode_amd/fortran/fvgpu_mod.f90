@@ -165,6 +165,25 @@ module fvgpu
       integer(c_int64_t), value :: n
       integer(c_int) :: rc
     end function
+    ! EnKF ensemble update A <- A*X5 of one state array over the members of
+    ! a batched union (X5 nrens x nrens, column-major, as ANALYSIS forms it)
+    function fvgpu_ensemble_update(h, name, nrens, node_off, elem_off, m, n, x5) &
+         bind(c, name="fvgpu_ensemble_update") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_char, c_double
+      type(c_ptr), value :: h
+      character(kind=c_char) :: name(*)
+      integer(c_int32_t), value :: nrens
+      integer(c_int64_t) :: node_off(*), elem_off(*)
+      integer(c_int64_t), value :: m, n
+      real(c_double) :: x5(*)
+      integer(c_int) :: rc
+    end function
+    function fvgpu_set_concurrency(h, mode) bind(c, name="fvgpu_set_concurrency") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: mode
+      integer(c_int) :: rc
+    end function
     function fvgpu_autotune(h, steps, report, report_len) bind(c, name="fvgpu_autotune") result(rc)
       import :: c_int, c_int32_t, c_int64_t, c_ptr
       type(c_ptr), value :: h
