@@ -3,7 +3,9 @@
 // One Runge-Kutta stage of EXTERNAL_STEP (FVCOM_source/external_step.F:159-438)
 // is three kernels:
 //   k_ext_elevation  node-parallel: EXTEL_EDGE (extel_edge.F:32-229) as a
-//                    gather over each node's control-volume sub-edges, the
+//                    gather of each node's control-volume sub-edge terms
+//                    (formed per sub-edge by the previous stage's
+//                    k_ext_momentum, or by k_ext_phi), the
 //                    OBC tide and RK blend (BCOND_GCN(1,K), bcond_gcn.F:116;
 //                    BCOND_ASL, mod_obcs.F:509-592), the OBC flux of FLUX_OBN
 //                    (mod_obcs.F:1181-1215), D = H + EL, DEPTH_CHECK, and the
@@ -14,7 +16,8 @@
 //                    element's three edge terms gathered in edge order,
 //                    EXTUV_EDGE (extuv_edge.F:33-196),
 //                    the wall projection BCOND_GCN(2,K) (bcond_gcn.F:325-420),
-//                    D1 = H1 + EL1 and the stage-3 UARD/VARD accumulation.
+//                    D1 = H1 + EL1, the stage-3 UARD/VARD accumulation and
+//                    the EXTEL terms of its sub-edges for the next stage.
 // The per-node and per-element gather lists are sorted by sub-edge / edge
 // index, so every sum is formed in the order of the reference's scatter loop.
 #pragma once
