@@ -40,6 +40,7 @@ def lib():
         L.orc_iint.restype = C.c_longlong
         L.orc_iint.argtypes = [vp]
         L.orc_depth_fail.argtypes = [vp]
+        L.orc_routine.argtypes = [vp, C.c_char_p, C.c_int]
         L.orc_budgets.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib
@@ -69,6 +70,10 @@ class Oracle:
                  "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf"}
         elem3 = {"u", "v", "w", "ww", "uf", "vf", "advx", "advy", "drhox", "drhoy", "rho", "rmean",
                  "t", "s", "km1", "viscofm"}
+        if name == "uard_obcn":
+            return self.model.hdr.iobcn
+        if name == "xflux_obc":
+            return self.model.hdr.iobcn * kb
         if name in node2:
             return m1
         if name in elem2:
@@ -87,6 +92,13 @@ class Oracle:
     def put(self, name, values):
         v = np.ascontiguousarray(values, np.float64)
         if lib().orc_put(self._o, name.encode(), v.ctypes.data, len(v)) != 0:
+            raise KeyError(name)
+
+    def routine(self, name, arg=0):
+        """Run one routine of the step on the current state (viscof_h,
+        wreal, vdif_q, bcond_ts with NCON = arg, adjust2d3d of type arg,
+        bottom_roughness, bcond_gcn2 on UAF/VAF, bcond_gcn5 on U/V)."""
+        if lib().orc_routine(self._o, name.encode(), int(arg)) != 0:
             raise KeyError(name)
 
     @property

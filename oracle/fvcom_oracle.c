@@ -16,8 +16,10 @@
  * GRID_METRICS output); state arrays are copied into the oracle.
  *
  * Pinning (what checks each routine beyond restatement) is listed in
- * tests/test_oracle.py; routines without such a check are marked
- * "parity unpinned" in their comment.
+ * tests/test_oracle.py (whole steps: budgets, seiche, tide, densities) and
+ * tests/test_oracle_pins.py (single routines through orc_routine against
+ * closed forms and published results); routines without such a check are
+ * marked "parity unpinned" in their comment.
  */
 #include "../include/fvgpu.h"
 #include "../include/fv_detmath.h"
@@ -180,6 +182,8 @@ static double* orc_lookup(orc* o, const char* name, size_t* len) {
       {"smean1", o->smean1, mk}, {"q2", o->q2, mk}, {"q2l", o->q2l, mk}, {"q2f", o->q2f, mk},
       {"q2lf", o->q2lf, mk}, {"l", o->l, mk}, {"km", o->km, mk}, {"kh", o->kh, mk},
       {"kq", o->kq, mk}, {"wts", o->wts, mk}, {"viscofh", o->viscofh, mk},
+      {"uard_obcn", o->uard_obcn, (size_t)o->IOBCN},
+      {"xflux_obc", o->xflux_obc, (size_t)o->IOBCN * o->KB},
   };
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); ++i)
     if (strcmp(t[i].n, name) == 0) { *len = t[i].l; return t[i].p; }
@@ -1467,3 +1471,23 @@ double orc_fv_cos(double x) { return fv_cos(x); }
 double orc_fv_tanh(double x) { return fv_tanh(x); }
 double orc_fv_log(double x) { return fv_log(x); }
 double orc_fv_expm1(double x) { return fv_expm1(x); }
+
+/* Test hook: one routine of the step on the oracle's current state, so that
+ * the tests can pin it by itself against closed forms.  BCOND_GCN(2/3/5)
+ * are the wall projections of the step's own loops above.  Returns 1 for an
+ * unknown name. */
+int orc_routine(orc* o, const char* name, int arg) {
+  if (strcmp(name, "viscof_h") == 0) viscof_h(o);
+  else if (strcmp(name, "wreal") == 0) wreal(o);
+  else if (strcmp(name, "vdif_q") == 0) vdif_q(o);
+  else if (strcmp(name, "bcond_ts") == 0) bcond_ts(o, arg);
+  else if (strcmp(name, "adjust2d3d") == 0) adjust2d3d(o, arg);
+  else if (strcmp(name, "bottom_roughness") == 0) bottom_roughness(o);
+  else if (strcmp(name, "bcond_gcn2") == 0)
+    for (int i = 1; i <= o->N; ++i) wall_project(o, i, &o->uaf[i], &o->vaf[i]);
+  else if (strcmp(name, "bcond_gcn5") == 0)
+    for (int i = 1; i <= o->N; ++i)
+      for (int k = 1; k <= o->KBM1; ++k) wall_project(o, i, &EL3(o->u, i, k), &EL3(o->v, i, k));
+  else return 1;
+  return 0;
+}

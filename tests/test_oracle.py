@@ -10,6 +10,9 @@ routine is checked against what the mathematics fixes:
   ADV_S/VDIF_TS     salt content conserved in a closed basin
   DENS2             published sigma-t values of the UNESCO-type polynomial
   fv_detmath        deterministic tanh/cos/log within a few ulp of libm
+
+Single routines (VISCOF_H, WREAL, BCOND_GCN, VDIF_Q, BCOND_TS, ADJUST2D3D,
+BOTTOM_ROUGHNESS) are pinned one at a time in test_oracle_pins.py.
 """
 import ctypes as C
 import math

@@ -1,0 +1,443 @@
+"""Pinning of the oracle's boundary conditions, mixing and diagnostic
+routines one at a time (oracle.Oracle.routine runs a single routine on a
+state the test sets), each against a closed form or a published result that
+does not come from the code under test:
+
+  VISCOF_H      Smagorinsky deformation of a linear velocity field is exact
+                at interior nodes (viscofh.F:60-119)
+  WREAL         sigma-to-z vertical velocity of linear D and elevation
+                fields, w = omega + u.grad(z) + dz/dt (wreal.F:106-125)
+  BCOND_GCN     wall projection parallel to the solid boundary edge, the
+                river-mouth rule in the river's frame (bcond_gcn.F:325-616)
+  VDIF_Q        MY2.5 (Mellor and Yamada 1982): implicit dissipation q^3/B1 l
+                with the wall function W, buoyancy production, Galperin et
+                al. (1988) stability functions and length-scale limit
+                l <= 0.53 q/N, Craig-Banner surface and B1^(2/3) u*^2 bottom
+                values (vdif_q.F:34-551)
+  BCOND_TS      Blumberg and Kantha (1985) implicit radiation of the
+                perturbation with 3 h relaxation about the 2-D upstream
+                update of the column mean (bcond_ts.F:96-160,
+                mod_obcs.F:1223-1295)
+
+The routines are written into the step exactly as the HIP path runs them,
+so these checks pin both.
+"""
+import numpy as np
+import pytest
+
+import myfvcomcode_amd as F
+from oracle import Oracle
+
+A1, B1, A2, B2, C1, E1, E2 = 0.92, 16.6, 0.74, 10.1, 0.08, 1.80, 1.33
+KAPPA, SMALL = 0.40, 1e-8
+# MY82 neutral stability functions (published: S_M = 0.393, S_H = 0.494)
+SM0 = A1 * (1 - 3 * C1 - 6 * A1 / B1)
+SH0 = A2 * (1 - 6 * A1 / B1)
+
+
+def _linear_uv(m, a, b, c, d):
+    kb, n1 = m.kb, m.n + 1
+    xc, yc = m.field("xc"), m.field("yc")
+    u = np.zeros((kb, n1))
+    v = np.zeros((kb, n1))
+    for k in range(kb - 1):
+        s = 1 + 0.3 * k
+        u[k, 1:] = s * (a * xc[1:] + b * yc[1:] + 0.3)
+        v[k, 1:] = s * (c * xc[1:] + d * yc[1:] - 0.2)
+    return u, v
+
+
+def test_viscofh_linear_field_exact_at_interior_nodes(built):
+    """For u = a x + b y, v = c x + d y at the element centroids, the
+    centroid-ring circulation over ART1 is the exact gradient at interior
+    nodes, so VISCOFH = sqrt(a^2 + d^2 + (b + c)^2 / 2) * ART1."""
+    m = F.Model.build(0, nx=21, ny=7, kb=6)
+    o = Oracle(m)
+    a, b, c, d = 2e-4, -1.5e-4, 0.7e-4, 3e-4
+    u, v = _linear_uv(m, a, b, c, d)
+    o.put("u", u.ravel())
+    o.put("v", v.ravel())
+    o.routine("viscof_h")
+    kb, m1 = m.kb, m.m + 1
+    vh = o.get("viscofh").reshape(kb, m1)
+    art1 = m.field("art1")
+    inner = np.where(m.field("isonb")[1:] == 0)[0] + 1
+    assert len(inner) > 50
+    for k in range(kb - 1):
+        exp = (1 + 0.3 * k) * np.sqrt(a * a + d * d + 0.5 * (b + c) ** 2) * art1[inner]
+        assert np.max(np.abs(vh[k, inner] / exp - 1)) < 1e-13, k
+
+
+def test_wreal_sigma_to_z_velocity(built):
+    """With D, ELF and EL linear in x, y the element gradients and the
+    three-node mean are exact: WW = (W_k + W_k+1)/2 + u (sigma dD/dx +
+    dzeta/dx) + v (sigma dD/dy + dzeta/dy) + (1 + sigma) (zeta_c - ET1)/DTI."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8)
+    kb, n1 = m.kb, m.n + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    o = Oracle(m)
+    vx, vy, xc, yc = m.field("vx"), m.field("vy"), m.field("xc"), m.field("yc")
+    zz1 = m.field("zz1").reshape(kb, n1)
+    gD, gF, gE = (1e-4, -2e-4), (3e-6, 1e-6), (-2e-6, 4e-6)
+    o.put("d", 10 + gD[0] * vx + gD[1] * vy)
+    o.put("elf", 0.2 + gF[0] * vx + gF[1] * vy)
+    o.put("el", 0.1 + gE[0] * vx + gE[1] * vy)
+    rng = np.random.default_rng(2)
+    et1 = 0.05 * rng.standard_normal(n1)
+    u, v = rng.standard_normal((kb, n1)), rng.standard_normal((kb, n1))
+    w = 1e-3 * rng.standard_normal((kb, n1))
+    o.put("et1", et1)
+    for nm, a in (("u", u), ("v", v), ("w", w)):
+        o.put(nm, a.ravel())
+    o.routine("wreal")
+    ww = o.get("ww").reshape(kb, n1)
+    e = slice(1, n1)
+    zeta_c = 0.1 + gE[0] * xc[e] + gE[1] * yc[e]
+    for k in range(kb - 1):
+        s = zz1[k, e]
+        exp = (0.5 * (w[k, e] + w[k + 1, e]) + u[k, e] * (s * gD[0] + gF[0]) + v[k, e] * (s * gD[1] + gF[1])
+               + (1 + s) * (zeta_c - et1[e]) / dti)
+        assert np.max(np.abs(ww[k, e] - exp)) < 1e-15, k
+
+
+def _mouth_elements(m):
+    nbve, off, ntve = m.field("nbve"), m.field("nbve_off"), m.field("ntve")
+    mouth = {}
+    for j, i1 in enumerate(m.field("inodeq")):
+        i1 = int(i1)
+        for e in (nbve[off[i1]], nbve[off[i1] + ntve[i1] - 1]):
+            mouth.setdefault(int(e), j)
+    return mouth
+
+
+def _rotated_estuary(deg, kb=4):
+    """The estuary channel turned by `deg` degrees, so that no wall is
+    parallel to an axis, with its open boundary and its river node."""
+    c = F.Model.build(0, nx=21, ny=7, kb=kb)
+    x, y = c.field("vx")[1:], c.field("vy")[1:]
+    th = np.deg2rad(deg)
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    m = F.Model.from_mesh(x * np.cos(th) - y * np.sin(th), x * np.sin(th) + y * np.cos(th), nv,
+                          c.field("h")[1:].copy(), kb, c.field("i_obc_n").copy(), F.Model.case_defaults(0).p)
+    m.set_river(int(c.field("inodeq")[0]), 20.0)
+    m.cold_start()
+    return m
+
+
+@pytest.mark.parametrize("which", ["bcond_gcn2", "bcond_gcn5"])
+def test_wall_projection_follows_the_boundary_edge(built, which):
+    """ISBCE = 1 elements: the velocity becomes parallel to the element's
+    solid edge with its tangential component kept (ALPHA carries the
+    reference's 3.1415926/2, hence the 2e-7 bound); river-mouth elements
+    keep the along-river component and lose only inflow against it; every
+    other element is untouched."""
+    m = _rotated_estuary(30.0)
+    assert m.hdr.numqbc >= 1
+    n1, kb = m.n + 1, m.kb
+    o = Oracle(m)
+    rng = np.random.default_rng(3)
+    if which == "bcond_gcn2":
+        u0, v0 = rng.standard_normal(n1), rng.standard_normal(n1)
+        u0[0] = v0[0] = 0.0
+        o.put("uaf", u0)
+        o.put("vaf", v0)
+        o.routine(which)
+        u1, v1 = o.get("uaf")[None, :], o.get("vaf")[None, :]
+        u0, v0 = u0[None, :], v0[None, :]
+    else:
+        u0, v0 = rng.standard_normal((kb, n1)), rng.standard_normal((kb, n1))
+        u0[-1] = v0[-1] = 0.0
+        u0[:, 0] = v0[:, 0] = 0.0
+        o.put("u", u0.ravel())
+        o.put("v", v0.ravel())
+        o.routine(which)
+        u1, v1 = o.get("u").reshape(kb, n1)[:-1], o.get("v").reshape(kb, n1)[:-1]
+        u0, v0 = u0[:-1], v0[:-1]
+    isbce, nbe = m.field("isbce"), m.field("nbe").reshape(3, n1)
+    nv, vx, vy, angleq = m.field("nv").reshape(3, n1), m.field("vx"), m.field("vy"), m.field("angleq")
+    mouth = _mouth_elements(m)
+    n_wall = n_mouth = clipped = 0
+    for e in range(1, n1):
+        if isbce[e] == 1 and e in mouth:
+            aq = angleq[mouth[e]]
+            nrm, tan = (np.cos(aq), np.sin(aq)), (-np.sin(aq), np.cos(aq))
+            un0 = u0[:, e] * nrm[0] + v0[:, e] * nrm[1]
+            assert np.allclose(u1[:, e] * nrm[0] + v1[:, e] * nrm[1], np.maximum(un0, 0.0), atol=1e-15)
+            assert np.allclose(u1[:, e] * tan[0] + v1[:, e] * tan[1], u0[:, e] * tan[0] + v0[:, e] * tan[1],
+                               atol=1e-15)
+            n_mouth += 1
+            clipped += int(np.sum(un0 < 0))
+        elif isbce[e] == 1:
+            j = [q for q in range(3) if nbe[q, e] == 0][0]
+            a, b = nv[(j + 1) % 3, e], nv[(j + 2) % 3, e]
+            t = np.array([vx[b] - vx[a], vy[b] - vy[a]])
+            t /= np.hypot(*t)
+            mag = np.hypot(u0[:, e], v0[:, e])
+            assert np.all(np.abs(u1[:, e] * t[1] - v1[:, e] * t[0]) <= 2e-7 * mag)
+            tan0 = u0[:, e] * t[0] + v0[:, e] * t[1]
+            assert np.all(np.abs(u1[:, e] * t[0] + v1[:, e] * t[1] - tan0) <= 2e-7 * mag)
+            n_wall += 1
+        elif isbce[e] == 3:
+            assert np.all(u1[:, e] == 0) and np.all(v1[:, e] == 0)
+        else:
+            assert np.array_equal(u1[:, e], u0[:, e]) and np.array_equal(v1[:, e], v0[:, e])
+    assert n_wall > 20 and n_mouth >= 1 and clipped >= 1
+
+
+def _vq_case(kb=8):
+    return F.Model.build(0, nx=21, ny=7, kb=kb, p_umol=0.0)
+
+
+def _vq_setup(m, o, rho_k, q2k, lk, wind=(0.0, 0.0), bot=(0.0, 0.0), kh=0.0):
+    kb, m1, n1 = m.kb, m.m + 1, m.n + 1
+    q2 = np.zeros((kb, m1))
+    q2l = np.zeros((kb, m1))
+    q2[:, 1:] = q2k[:, None]
+    q2l[:, 1:] = (q2k * lk)[:, None]
+    for nm, a in (("q2", q2), ("q2l", q2l), ("q2f", q2), ("q2lf", q2l)):
+        o.put(nm, a.ravel())
+    for nm, val in (("km", 0.0), ("kh", kh), ("kq", 0.0)):
+        a = np.zeros((kb, m1))
+        a[:, 1:] = val
+        o.put(nm, a.ravel())
+    rho = np.zeros((kb, m1))
+    rho[:, 1:] = rho_k[:, None]
+    o.put("rho1", rho.ravel())
+    for nm, val in (("wusurf", wind[0]), ("wvsurf", wind[1]), ("wubot", bot[0]), ("wvbot", bot[1])):
+        a = np.full(n1, val)
+        a[0] = 0.0
+        o.put(nm, a)
+    return q2, q2l
+
+
+def _stab(gh):
+    """Galperin et al. (1988) quasi-equilibrium stability functions."""
+    sh = SH0 / (1 - 3 * A2 * (B2 + 6 * A1) * gh)
+    sm = (SM0 + sh * (18 * A1 * A1 + 9 * A1 * A2) * gh) / (1 - 9 * A1 * A2 * gh)
+    return sm, sh
+
+
+@pytest.mark.parametrize("case", ["neutral", "unstable"])
+def test_vdif_q_my25_column(built, case):
+    """No shear, no vertical diffusion (KQ = UMOL = 0): each level solves
+    q2' (1 + 2 dt q / (B1 l)) = q2 + 2 dt P_b and
+    q2l' (1 + dt q W / (B1 l)) = q2l + dt E1 l P_b, with P_b = KH N^2 of the
+    old KH and W = 1 + E2 (l / kappa (1/d_surface + 1/d_bottom))^2; the new
+    KM, KH are the means of the old values and l q S_M, l q S_H with
+    Galperin's S_M(G_H), S_H(G_H), G_H capped at 0.0233."""
+    m = _vq_case()
+    kb, m1 = m.kb, m.m + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    Z = m.field("z").reshape(kb, m1)
+    DZZ = m.field("dzz").reshape(kb, m1)
+    grav = m.field("grav_n")
+    o = Oracle(m)
+    d = o.get("d")
+    q2k = np.linspace(1e-4, 4e-4, kb)
+    lk = np.linspace(0.5, 2.0, kb)
+    if case == "neutral":
+        rho_k, kh0 = np.full(kb, 0.02), 0.0
+    else:
+        rho_k = 0.025 - np.concatenate([[0.0], np.cumsum(np.geomspace(1e-10, 1e-6, kb - 1))])
+        kh0 = 1e-3
+    q2, q2l = _vq_setup(m, o, rho_k, q2k, lk, kh=kh0)
+    o.routine("vdif_q")
+    Q2F = o.get("q2f").reshape(kb, m1)
+    Q2LF = o.get("q2lf").reshape(kb, m1)
+    KM = o.get("km").reshape(kb, m1)
+    KH = o.get("kh").reshape(kb, m1)
+    q2a = np.abs(q2 + SMALL)
+    la = q2l / q2a
+    i = slice(1, m1)
+    n_cap = 0
+    for k in range(1, kb - 1):
+        boygr = grav[i] * (rho_k[k - 1] - rho_k[k]) / (DZZ[k - 1, i] * d[i])
+        gh = np.minimum(la[k, i] ** 2 / q2a[k, i] * boygr, 0.0233)
+        n_cap += int(np.sum(gh == 0.0233))
+        prod = kh0 * boygr
+        diss = np.sqrt(q2a[k, i]) / (B1 * la[k, i] + SMALL)
+        W = 1 + E2 * ((1 / np.abs(Z[k, i] - Z[0, i]) + 1 / np.abs(Z[k, i] - Z[kb - 1, i]))
+                      * la[k, i] / (d[i] * KAPPA)) ** 2
+        assert np.max(np.abs(Q2F[k, i] / ((q2[k, i] + 2 * dti * prod) / (1 + 2 * dti * diss)) - 1)) < 1e-14
+        assert np.max(np.abs(Q2LF[k, i] / ((q2l[k, i] + dti * prod * la[k, i] * E1) / (1 + dti * diss * W)) - 1)) < 1e-14
+        sm, sh = _stab(gh)
+        kn = la[k, i] * np.sqrt(q2a[k, i])
+        assert np.max(np.abs(KM[k, i] / (0.5 * kn * sm) - 1)) < 1e-14
+        assert np.max(np.abs(KH[k, i] / (0.5 * (kn * sh * m.hdr.p.vprnu + kh0)) - 1)) < 1e-14
+    if case == "neutral":
+        assert round(SM0, 3) == 0.393 and round(SH0, 3) == 0.494
+    else:
+        assert 0 < n_cap < (kb - 2) * (m1 - 1)   # both branches of the cap taken
+
+
+def test_vdif_q_stable_length_scale_limit(built):
+    """Stable stratification: l is limited to sqrt(0.281) q / N = 0.53 q / N
+    (Galperin et al. 1988) and Q2L follows as Q2 * L."""
+    m = _vq_case()
+    kb, m1 = m.kb, m.m + 1
+    DZZ = m.field("dzz").reshape(kb, m1)
+    grav = m.field("grav_n")
+    o = Oracle(m)
+    d = o.get("d")
+    rho_k = 0.020 + 2e-3 * np.arange(kb)
+    _vq_setup(m, o, rho_k, np.linspace(1e-4, 4e-4, kb), np.linspace(0.5, 2.0, kb))
+    o.routine("vdif_q")
+    L = o.get("l").reshape(kb, m1)
+    Q2L = o.get("q2l").reshape(kb, m1)
+    Q2 = o.get("q2").reshape(kb, m1)
+    i = slice(1, m1)
+    for k in range(1, kb - 1):
+        n2 = -grav[i] * (rho_k[k - 1] - rho_k[k]) / (DZZ[k - 1, i] * d[i])
+        r = L[k, i] * np.sqrt(n2 / Q2[k, i])
+        assert np.all(r <= np.sqrt(0.281) * (1 + 1e-12))
+        assert np.all(r > 0.5300)          # the limit binds at this stratification
+        assert np.array_equal(Q2L[k, i], Q2[k, i] * L[k, i])
+
+
+def test_vdif_q_surface_and_bottom_values(built):
+    """Wind stress |tau|: q2 = (15.8 alpha_CB)^(2/3) |tau| at the surface
+    (Craig and Banner 1994, alpha_CB = 100), L(1) = kappa * 2e5 |tau| / g
+    and L >= kappa l0 in the upper half; bottom stress: q2 = B1^(2/3) |tau_b|.
+    The reference's exponent 0.6666667 bounds the agreement at 3e-7."""
+    m = _vq_case()
+    kb, m1 = m.kb, m.m + 1
+    Z = m.field("z").reshape(kb, m1)
+    grav = m.field("grav_n")
+    o = Oracle(m)
+    q2k, lk = np.linspace(1e-4, 4e-4, kb), np.linspace(0.01, 0.05, kb)
+    q2, q2l = _vq_setup(m, o, np.full(kb, 0.02), q2k, lk, wind=(3e-4, 4e-4), bot=(-3e-4, 4e-4))
+    o.routine("vdif_q")
+    Q2F = o.get("q2f").reshape(kb, m1)
+    L = o.get("l").reshape(kb, m1)
+    tau = 5e-4
+    l0 = 2e5 * tau / grav[1:]
+    assert np.max(np.abs(Q2F[0, 1:] / ((15.8 * 100) ** (2 / 3) * tau) - 1)) < 3e-7
+    assert np.max(np.abs(Q2F[kb - 1, 1:] / (16.6 ** (2 / 3) * tau) - 1)) < 3e-7
+    assert np.max(np.abs(L[0, 1:] / (KAPPA * l0) - 1)) < 1e-15
+    assert np.all(L[kb - 1, 1:] == 0.0)
+    la = q2l / np.abs(q2 + SMALL)
+    raised = 0
+    for k in range(1, kb - 1):
+        up = Z[k, 1:] > -0.5
+        assert np.array_equal(L[k, 1:], np.where(up, np.maximum(la[k, 1:], KAPPA * l0), la[k, 1:]))
+        raised += int(np.sum(up & (KAPPA * l0 > la[k, 1:])))
+    assert raised > 0
+
+
+def test_bcond_ts_blumberg_kantha(built):
+    """TYPE_TSOBC = 3 at outflow OBC nodes: the column mean is the 2-D
+    upstream update (T2D DT - (XFLUX2D + T2D UARD_OBCN) DTI / ART1) / D and
+    the perturbation about it radiates implicitly with c = sqrt(g H) and
+    relaxes on 3 h: p' = (cc p_next + p (1 - DTI / 10800)) / (1 + cc),
+    cc = c DTI / DLTN (Blumberg and Kantha 1985); inflow nodes keep T1.  The
+    ring values bracket the result, so the clip to the local range is
+    inactive here."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8, river=0)
+    assert m.hdr.p.type_tsobc == 3
+    kb, m1, nob = m.kb, m.m + 1, m.hdr.iobcn
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    o = Oracle(m)
+    obc, nxt, dln = m.field("i_obc_n"), m.field("next_obc"), m.field("dltn_obc")
+    DZ = m.field("dz").reshape(kb, m1)
+    art1, h, g = m.field("art1"), m.field("h"), m.field("grav_n")
+    nbsn, off, ntsn = m.field("nbsn"), m.field("nbsn_off"), m.field("ntsn")
+    rng = np.random.default_rng(5)
+    P = 15 + rng.standard_normal(kb - 1)
+    P2 = 14 + rng.standard_normal(kb - 1)
+    t1 = np.zeros((kb, m1))
+    tf1 = np.zeros((kb, m1))
+    t1[:-1, 1:] = P[:, None]
+    obcs, used = set(int(j) for j in obc), set()
+    for q, j in enumerate(obc):
+        c = 20.0 if q % 2 == 0 else -20.0
+        t1[:-1, j] = P + c
+        ring = [int(x) for x in nbsn[off[j]:off[j] + ntsn[j]]
+                if int(x) not in obcs and int(x) != j and int(x) not in used]
+        t1[:-1, ring[0]] = P + 2 * c
+        used.add(ring[0])
+    tf1[:-1, 1:] = P2[:, None]
+    d = o.get("d")
+    dt = d * (1 + 0.01 * rng.standard_normal(m1))
+    uo = np.abs(rng.standard_normal(nob)) * 50.0
+    uo[1] = -1.0
+    xf = np.zeros((kb, nob))
+    xf[:-1] = rng.standard_normal((kb - 1, nob)) * 30.0
+    for nm, a in (("dt", dt), ("t1", t1.ravel()), ("tf1", tf1.ravel()), ("uard_obcn", uo),
+                  ("xflux_obc", xf.ravel())):
+        o.put(nm, a)
+    o.routine("bcond_ts", 1)
+    out = o.get("tf1").reshape(kb, m1)
+    for q, j in enumerate(obc):
+        if uo[q] <= 0:
+            assert np.array_equal(out[:-1, j], t1[:-1, j])
+            continue
+        j1 = nxt[q]
+        t2d = np.sum(t1[:-1, j] * DZ[:-1, j])
+        t2n = np.sum(tf1[:-1, j1] * DZ[:-1, j1])
+        mean = (t2d * dt[j] - (np.sum(xf[:-1, q]) + t2d * uo[q]) * dti / art1[j]) / d[j]
+        cc = np.sqrt(g[j] * h[j]) * dti / dln[q]
+        exp = mean + (cc * (tf1[:-1, j1] - t2n) + (t1[:-1, j] - t2d) * (1 - dti / 10800.0)) / (1 + cc)
+        ring = nbsn[off[j]:off[j] + ntsn[j]]
+        assert np.all(t1[:-1, ring].min(axis=1) < exp) and np.all(exp < t1[:-1, ring].max(axis=1))
+        assert np.max(np.abs(out[:-1, j] - exp)) < 1e-12
+        # the perturbation carries no depth mean
+        assert abs(np.sum(out[:-1, j] * DZ[:-1, j]) - mean) < 1e-12
+
+
+def test_adjust2d3d_depth_mean_matches_external_mode(built):
+    """ADJUST2D3D (adjust2d3d.F): type 1 shifts every layer of U by one
+    amount so that sum_k U DZ1 = UA; type 2 makes the transport
+    sum_k U DZ1 DT1 equal the external mode's time mean UARD / ISPLIT.  The
+    vertical shear U_k - U_k+1 is kept in both."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8)
+    kb, n1 = m.kb, m.n + 1
+    dz1 = m.field("dz1").reshape(kb, n1)
+    rng = np.random.default_rng(7)
+    for typ in (1, 2):
+        o = Oracle(m)
+        u0, v0 = rng.standard_normal((kb, n1)), rng.standard_normal((kb, n1))
+        u0[-1] = v0[-1] = 0.0
+        u0[:, 0] = v0[:, 0] = 0.0
+        ua, va = rng.standard_normal(n1), rng.standard_normal(n1)
+        dt1 = 5 + rng.random(n1)
+        for nm, a in (("u", u0.ravel()), ("v", v0.ravel()), ("ua", ua), ("va", va), ("uard", ua * 10),
+                      ("vard", va * 10), ("dt1", dt1)):
+            o.put(nm, a)
+        o.routine("adjust2d3d", typ)
+        u, v = o.get("u").reshape(kb, n1), o.get("v").reshape(kb, n1)
+        e = slice(1, n1)
+        for x, x0, mean2d in ((u, u0, ua), (v, v0, va)):
+            depth_sum = np.sum(x[:-1, e] * dz1[:-1, e], axis=0)
+            target = mean2d[e] if typ == 1 else mean2d[e] * 10 / m.hdr.p.isplit / dt1[e]
+            assert np.max(np.abs(depth_sum - target)) < 1e-13
+            assert np.max(np.abs(np.diff(x[:-1, e] - x0[:-1, e], axis=0))) < 1e-13
+        if typ == 2:
+            assert np.max(np.abs(o.get("uard")[e] - ua[e] * 10 / m.hdr.p.isplit)) < 1e-15
+
+
+def test_bottom_roughness_law_of_the_wall(built):
+    """BR_ORIG (brough.F:41-215): CBC = max((kappa / ln(z_b / z0))^2, CBCMIN)
+    with z_b the height of the lowest layer centre above the bottom (the
+    depth taken as at least 3 m), and the quadratic drag
+    (WUBOT, WVBOT) = -CBC |u_b| u_b."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8)
+    kb, n1 = m.kb, m.n + 1
+    o = Oracle(m)
+    rng = np.random.default_rng(8)
+    dt1 = np.concatenate([[0.0], rng.uniform(0.5, 400.0, n1 - 1)])
+    u, v = rng.standard_normal((kb, n1)), rng.standard_normal((kb, n1))
+    for nm, a in (("dt1", dt1), ("u", u.ravel()), ("v", v.ravel())):
+        o.put(nm, a)
+    o.routine("bottom_roughness")
+    zz1, z1 = m.field("zz1").reshape(kb, n1), m.field("z1").reshape(kb, n1)
+    z0 = m.field("cc_z0b")
+    e = slice(1, n1)
+    zb = (zz1[kb - 2, e] - z1[kb - 1, e]) * np.maximum(dt1[e], 3.0)
+    cbc = np.maximum((0.4 / np.log(zb / z0[e])) ** 2, m.hdr.p.cbcmin)
+    got = o.get("cbc")[e]
+    assert np.max(np.abs(got / cbc - 1)) < 1e-12
+    assert np.any(got > m.hdr.p.cbcmin) and np.any(got == m.hdr.p.cbcmin)
+    ub, vb = u[kb - 2, e], v[kb - 2, e]
+    sp = np.hypot(ub, vb)
+    assert np.max(np.abs(o.get("wubot")[e] + got * sp * ub)) < 1e-15
+    assert np.max(np.abs(o.get("wvbot")[e] + got * sp * vb)) < 1e-15
