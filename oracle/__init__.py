@@ -70,6 +70,8 @@ class Oracle:
                  "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf"}
         elem3 = {"u", "v", "w", "ww", "uf", "vf", "advx", "advy", "drhox", "drhoy", "rho", "rmean",
                  "t", "s", "km1", "viscofm"}
+        if name == "ramp":
+            return 1
         if name == "uard_obcn":
             return self.model.hdr.iobcn
         if name == "xflux_obc":
@@ -97,7 +99,8 @@ class Oracle:
     def routine(self, name, arg=0):
         """Run one routine of the step on the current state (viscof_h,
         wreal, vdif_q, bcond_ts with NCON = arg, adjust2d3d of type arg,
-        bottom_roughness, bcond_gcn2 on UAF/VAF, bcond_gcn5 on U/V)."""
+        bottom_roughness, baropg, vdif_uv, bcond_gcn2 on UAF/VAF,
+        bcond_gcn5 on U/V)."""
         if lib().orc_routine(self._o, name.encode(), int(arg)) != 0:
             raise KeyError(name)
 

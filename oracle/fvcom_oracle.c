@@ -182,6 +182,7 @@ static double* orc_lookup(orc* o, const char* name, size_t* len) {
       {"smean1", o->smean1, mk}, {"q2", o->q2, mk}, {"q2l", o->q2l, mk}, {"q2f", o->q2f, mk},
       {"q2lf", o->q2lf, mk}, {"l", o->l, mk}, {"km", o->km, mk}, {"kh", o->kh, mk},
       {"kq", o->kq, mk}, {"wts", o->wts, mk}, {"viscofh", o->viscofh, mk},
+      {"ramp", &o->ramp, 1},
       {"uard_obcn", o->uard_obcn, (size_t)o->IOBCN},
       {"xflux_obc", o->xflux_obc, (size_t)o->IOBCN * o->KB},
   };
@@ -1483,6 +1484,8 @@ int orc_routine(orc* o, const char* name, int arg) {
   else if (strcmp(name, "bcond_ts") == 0) bcond_ts(o, arg);
   else if (strcmp(name, "adjust2d3d") == 0) adjust2d3d(o, arg);
   else if (strcmp(name, "bottom_roughness") == 0) bottom_roughness(o);
+  else if (strcmp(name, "baropg") == 0) baropg(o);
+  else if (strcmp(name, "vdif_uv") == 0) vdif_uv(o);
   else if (strcmp(name, "bcond_gcn2") == 0)
     for (int i = 1; i <= o->N; ++i) wall_project(o, i, &o->uaf[i], &o->vaf[i]);
   else if (strcmp(name, "bcond_gcn5") == 0)

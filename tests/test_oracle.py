@@ -12,7 +12,8 @@ routine is checked against what the mathematics fixes:
   fv_detmath        deterministic tanh/cos/log within a few ulp of libm
 
 Single routines (VISCOF_H, WREAL, BCOND_GCN, VDIF_Q, BCOND_TS, ADJUST2D3D,
-BOTTOM_ROUGHNESS) are pinned one at a time in test_oracle_pins.py.
+BOTTOM_ROUGHNESS, BAROPG, VDIF_UV) are pinned one at a time in
+test_oracle_pins.py.
 """
 import ctypes as C
 import math

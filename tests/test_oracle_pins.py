@@ -14,6 +14,11 @@ does not come from the code under test:
                 al. (1988) stability functions and length-scale limit
                 l <= 0.53 q/N, Craig-Banner surface and B1^(2/3) u*^2 bottom
                 values (vdif_q.F:34-551)
+  BAROPG        hydrostatic baroclinic force on a flat bottom (baropg.F)
+  VDIF_UV       column momentum budget with surface and implicit bottom
+                stress; decay of the discrete Neumann cosine modes
+  ADJUST2D3D    depth mean = UA, transport = UARD / ISPLIT
+  BOTTOM_ROUGHNESS  law of the wall with the CBCMIN floor
   BCOND_TS      Blumberg and Kantha (1985) implicit radiation of the
                 perturbation with 3 h relaxation about the 2-D upstream
                 update of the column mean (bcond_ts.F:96-160,
@@ -441,3 +446,93 @@ def test_bottom_roughness_law_of_the_wall(built):
     sp = np.hypot(ub, vb)
     assert np.max(np.abs(o.get("wubot")[e] + got * sp * ub)) < 1e-15
     assert np.max(np.abs(o.get("wvbot")[e] + got * sp * vb)) < 1e-15
+
+
+def test_baropg_hydrostatic_flat_bottom(built):
+    """Flat bottom, density anomaly linear in x and y plus any function of
+    sigma: the layer's baroclinic force is g RAMP (d rho/dx) (-sigma_k D)
+    (D DZ1_k) ART, the hydrostatic pressure gradient g rho_x (eta - z)
+    integrated over the layer (baropg.F:33-166); the sigma part cancels
+    around each element."""
+    c = F.Model.build(0, nx=21, ny=7, kb=8, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    m = F.Model.from_mesh(x, y, nv, np.full(c.m, 12.0), 8, [], F.Model.case_defaults(0).p)
+    m.cold_start()
+    kb, m1, n1 = m.kb, m.m + 1, m.n + 1
+    o = Oracle(m)
+    vx, vy = m.field("vx"), m.field("vy")
+    zz = m.field("zz").reshape(kb, m1)
+    zz1 = m.field("zz1").reshape(kb, n1)
+    gx, gy, beta, D, ramp = 2e-7, -3e-7, 1e-3, 12.5, 0.7
+    rho1 = np.zeros((kb, m1))
+    rho1[:-1, 1:] = 0.02 + gx * vx[1:] + gy * vy[1:] + beta * zz[:-1, 1:]
+    rho = np.zeros((kb, n1))
+    rho[:-1, 1:] = 0.02 + beta * zz1[:-1, 1:]
+    for nm, a in (("rho1", rho1.ravel()), ("rmean1", np.zeros(kb * m1)), ("rho", rho.ravel()),
+                  ("rmean", np.zeros(kb * n1)), ("dt", np.full(m1, D)), ("dt1", np.full(n1, D)),
+                  ("ramp", np.array([ramp]))):
+        o.put(nm, a)
+    o.routine("baropg")
+    drx, dry = o.get("drhox").reshape(kb, n1), o.get("drhoy").reshape(kb, n1)
+    art, dz1, ge = m.field("art"), m.field("dz1").reshape(kb, n1), m.field("grav_e")
+    e = slice(1, n1)
+    for k in range(kb - 1):
+        f = ge[e] * ramp * (-zz1[k, e]) * D * D * dz1[k, e] * art[e]
+        assert np.max(np.abs(drx[k, e] / (gx * f) - 1)) < 1e-11, k
+        assert np.max(np.abs(dry[k, e] / (gy * f) - 1)) < 1e-11, k
+    # RHO1 and RHO come back unchanged (the mean is removed and restored)
+    assert np.max(np.abs(o.get("rho1") - rho1.ravel())) < 1e-17
+
+
+def test_vdif_uv_column_budget_and_modes(built):
+    """VDIF_UV (vdif_uv.F:34-189), sponge off: the implicit solve changes
+    the column momentum by the surface stress (none at elements touching the
+    open boundary) and the implicit quadratic bottom drag,
+    sum_k (UF' - UF) DZ1 D / DTI = -WUSURF - CBC |u_b| UF'_KBM1; with no
+    stress and a uniform K the cosine modes of the Neumann problem decay by
+    1 / (1 + 4 DTI (K + UMOL) sin^2(pi m / 2N) / (dsigma D)^2)."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8)
+    m.field("cc_sponge")[:] = 0.0
+    kb, n1 = m.kb, m.n + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    dz1 = m.field("dz1").reshape(kb, n1)
+    o = Oracle(m)
+    rng = np.random.default_rng(4)
+    uf0, vf0 = rng.standard_normal((kb, n1)), rng.standard_normal((kb, n1))
+    uf0[-1] = vf0[-1] = 0.0
+    uf0[:, 0] = vf0[:, 0] = 0.0
+    u, v = rng.standard_normal((kb, n1)), rng.standard_normal((kb, n1))
+    ws, vs = 1e-4 * rng.standard_normal(n1), 1e-4 * rng.standard_normal(n1)
+    cbc = 2.5e-3 + 1e-3 * rng.random(n1)
+    for nm, a in (("uf", uf0.ravel()), ("vf", vf0.ravel()), ("u", u.ravel()), ("v", v.ravel()),
+                  ("km1", (np.abs(rng.standard_normal((kb, n1))) * 1e-2).ravel()),
+                  ("wusurf", ws), ("wvsurf", vs), ("cbc", cbc)):
+        o.put(nm, a)
+    o.routine("vdif_uv")
+    uf, vf, d1 = o.get("uf").reshape(kb, n1), o.get("vf").reshape(kb, n1), o.get("d1")
+    e = slice(1, n1)
+    isonb, nv = m.field("isonb"), m.field("nv").reshape(3, n1)
+    open_el = (isonb[nv[0]] == 2) | (isonb[nv[1]] == 2) | (isonb[nv[2]] == 2)
+    assert np.any(open_el[1:])
+    tmp = np.where(open_el, 0.0, 1.0)
+    tps = cbc * np.hypot(u[kb - 2], v[kb - 2])
+    for x, x0, s in ((uf, uf0, ws), (vf, vf0, vs)):
+        change = np.sum((x[:-1, e] - x0[:-1, e]) * dz1[:-1, e], axis=0) * d1[e] / dti
+        assert np.max(np.abs(change - (-s[e] * tmp[e] - tps[e] * x[kb - 2, e]))) < 1e-15
+    assert np.max(np.abs(o.get("wubot")[e] + tps[e] * u[kb - 2, e])) < 1e-17
+    # modes (the sigma levels are single-precision fractions: 3e-8 spread)
+    N, K = kb - 1, 3e-2
+    for mode in (1, 3):
+        prof = np.cos(np.pi * mode * (np.arange(1, N + 1) - 0.5) / N)
+        x0 = np.zeros((kb, n1))
+        x0[:-1, 1:] = prof[:, None]
+        for nm, a in (("uf", x0.ravel()), ("vf", 2 * x0.ravel()), ("km1", np.full(kb * n1, K)),
+                      ("wusurf", np.zeros(n1)), ("wvsurf", np.zeros(n1)), ("cbc", np.zeros(n1))):
+            o.put(nm, a)
+        o.routine("vdif_uv")
+        uf, vf, d1 = o.get("uf").reshape(kb, n1), o.get("vf").reshape(kb, n1), o.get("d1")
+        fac = 1 + 4 * dti * (K + m.hdr.p.umol) * np.sin(np.pi * mode / (2 * N)) ** 2 / (d1[1:] / N) ** 2
+        assert np.max(np.abs(uf[:-1, 1:] - prof[:, None] / fac[None, :])) < 1e-7
+        assert np.max(np.abs(vf[:-1, 1:] - 2 * prof[:, None] / fac[None, :])) < 2e-7
+        assert np.min(fac) > 1.01        # the damping is well resolved by the bound
