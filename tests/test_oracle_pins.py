@@ -14,6 +14,8 @@ does not come from the code under test:
                 al. (1988) stability functions and length-scale limit
                 l <= 0.53 q/N, Craig-Banner surface and B1^(2/3) u*^2 bottom
                 values (vdif_q.F:34-551)
+  ADVAVE_EDGE_GCN  pressure terms of a linear elevation; closing momentum
+                sums of divergence-free linear velocities
   BAROPG        hydrostatic baroclinic force on a flat bottom (baropg.F)
   VDIF_UV       column momentum budget with surface and implicit bottom
                 stress; decay of the discrete Neumann cosine modes
@@ -536,3 +538,53 @@ def test_vdif_uv_column_budget_and_modes(built):
         assert np.max(np.abs(uf[:-1, 1:] - prof[:, None] / fac[None, :])) < 1e-7
         assert np.max(np.abs(vf[:-1, 1:] - 2 * prof[:, None] / fac[None, :])) < 2e-7
         assert np.min(fac) > 1.01        # the damping is well resolved by the bound
+
+
+def test_advave_pressure_and_uniform_momentum(built):
+    """ADVAVE_EDGE_GCN (advave_edge_gcn.F:546-884) on a flat bottom with
+    constant horizontal viscosity: for a linear elevation the edge-midpoint
+    pressure terms sum to (PSTX, PSTY) = g D1 grad(eta) ART on every element;
+    for UA = U0 and VA = V0 + c x (divergence-free, reconstructed exactly by
+    the least-squares gradients) the x-momentum flux U0 (u.n) and the
+    constant stress c nu close around every element whose neighbours are
+    interior too, so ADVUA = 0 there, while ADVVA = D U0 c ART; for
+    UA = U0 + a y, VA = V0 the y-momentum closes (ADVVA = 0)."""
+    c = F.Model.build(0, nx=21, ny=7, kb=6, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    p = F.Model.case_defaults(0).p
+    p.horizontal_closure = 0
+    m = F.Model.from_mesh(x, y, nv, np.full(c.m, 12.0), 6, [], p)
+    m.cold_start()
+    m.field("cc_hvc")[:] = 7.0
+    m1, n1 = m.m + 1, m.n + 1
+    o = Oracle(m)
+    xc, vx, vy = m.field("xc"), m.field("vx"), m.field("vy")
+    U0, V0, cx, D, gx, gy = 0.8, -0.3, 2e-4, 12.0, 3e-6, -2e-6
+    ua, va = np.full(n1, U0), V0 + cx * xc
+    ua[0] = va[0] = 0.0
+    for nm, a in (("ua", ua), ("va", va), ("d", np.full(m1, D)), ("d1", np.full(n1, D)),
+                  ("el", 0.1 + gx * vx + gy * vy)):
+        o.put(nm, a)
+    o.routine("advave")
+    art, ge = m.field("art"), m.field("grav_e")
+    e = np.arange(1, n1)
+    assert np.max(np.abs(o.get("pstx")[e] / (ge[e] * D * gx * art[e]) - 1)) < 1e-12
+    assert np.max(np.abs(o.get("psty")[e] / (ge[e] * D * gy * art[e]) - 1)) < 1e-12
+    isbce, nbe = m.field("isbce"), m.field("nbe").reshape(3, n1)
+    deep = np.array([i for i in e if isbce[i] == 0 and all(nbe[j, i] != 0 and isbce[nbe[j, i]] == 0
+                                                           for j in range(3))])
+    assert len(deep) > 100
+    # terms of one edge are D |u.n| U0 ~ 1e4 here
+    assert np.max(np.abs(o.get("advua")[deep])) < 1e-10
+    # y-momentum: v (u.n) is linear along each edge, so the midpoint rule is
+    # exact and ADVVA = D U0 dv/dx ART
+    assert np.max(np.abs(o.get("advva")[deep] / (D * U0 * cx * art[deep]) - 1)) < 1e-12
+    # UA = U0 + a y, VA = V0: V0 (u.n) and the constant stress a nu close
+    ua = U0 + 3e-4 * m.field("yc")
+    va = np.full(n1, V0)
+    ua[0] = va[0] = 0.0
+    o.put("ua", ua)
+    o.put("va", va)
+    o.routine("advave")
+    assert np.max(np.abs(o.get("advva")[deep])) < 1e-10
