@@ -16,6 +16,7 @@ does not come from the code under test:
                 values (vdif_q.F:34-551)
   ADVAVE_EDGE_GCN  pressure terms of a linear elevation; closing momentum
                 sums of divergence-free linear velocities
+  ADVECTION_EDGE_GCN  the same, layer by layer
   BAROPG        hydrostatic baroclinic force on a flat bottom (baropg.F)
   VDIF_UV       column momentum budget with surface and implicit bottom
                 stress; decay of the discrete Neumann cosine modes
@@ -588,3 +589,40 @@ def test_advave_pressure_and_uniform_momentum(built):
     o.put("va", va)
     o.routine("advave")
     assert np.max(np.abs(o.get("advva")[deep])) < 1e-10
+
+
+def test_advection_edge_layers_close(built):
+    """ADVECTION_EDGE_GCN (advection_edge_gcn.F:29-734) per layer, flat
+    bottom, constant viscosity: U = U0_k, V = V0 + c_k x gives ADVX = 0 and
+    ADVY = D DZ_k U0_k c_k ART at elements with interior neighbours (the
+    2-D closed forms with the layer's thickness)."""
+    c = F.Model.build(0, nx=21, ny=7, kb=6, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    p = F.Model.case_defaults(0).p
+    p.horizontal_closure = 0
+    m = F.Model.from_mesh(x, y, nv, np.full(c.m, 12.0), 6, [], p)
+    m.cold_start()
+    m.field("cc_hvc")[:] = 7.0
+    kb, m1, n1 = m.kb, m.m + 1, m.n + 1
+    o = Oracle(m)
+    xc, D = m.field("xc"), 11.0
+    u, v = np.zeros((kb, n1)), np.zeros((kb, n1))
+    U0 = 0.8 - 0.1 * np.arange(kb - 1)
+    cx = 2e-4 * (1 + np.arange(kb - 1))
+    u[:-1, 1:] = U0[:, None]
+    v[:-1, 1:] = -0.3 + cx[:, None] * xc[None, 1:]
+    o.put("u", u.ravel())
+    o.put("v", v.ravel())
+    o.put("dt", np.full(m1, D))
+    o.routine("advection")
+    advx, advy = o.get("advx").reshape(kb, n1), o.get("advy").reshape(kb, n1)
+    dz1, art = m.field("dz1").reshape(kb, n1), m.field("art")
+    isbce, nbe = m.field("isbce"), m.field("nbe").reshape(3, n1)
+    deep = np.array([i for i in range(1, n1) if isbce[i] == 0 and all(
+        nbe[j, i] != 0 and isbce[nbe[j, i]] == 0 for j in range(3))])
+    assert len(deep) > 100
+    for k in range(kb - 1):
+        assert np.max(np.abs(advx[k, deep])) < 1e-10, k
+        exp = D * dz1[k, deep] * U0[k] * cx[k] * art[deep]
+        assert np.max(np.abs(advy[k, deep] / exp - 1)) < 1e-12, k
