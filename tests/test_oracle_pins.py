@@ -17,6 +17,7 @@ does not come from the code under test:
   ADVAVE_EDGE_GCN  pressure terms of a linear elevation; closing momentum
                 sums of divergence-free linear velocities
   ADVECTION_EDGE_GCN  the same, layer by layer
+  ADV_UV_EDGE_GCN  UF/VF term by term: advection, pressure, Coriolis, omega
   BAROPG        hydrostatic baroclinic force on a flat bottom (baropg.F)
   VDIF_UV       column momentum budget with surface and implicit bottom
                 stress; decay of the discrete Neumann cosine modes
@@ -626,3 +627,52 @@ def test_advection_edge_layers_close(built):
         assert np.max(np.abs(advx[k, deep])) < 1e-10, k
         exp = D * dz1[k, deep] * U0[k] * cx[k] * art[deep]
         assert np.max(np.abs(advy[k, deep] / exp - 1)) < 1e-12, k
+
+
+def test_adv_uv_momentum_terms(built):
+    """ADV_UV_EDGE_GCN (adv_uv_edge_gcn.F:31-1198) on a flat bottom with
+    constant viscosity, depth-uniform U = U0 and V = V0 + c x, a linear
+    EGF, Coriolis f and any omega W: each term of the momentum equation has
+    its closed form,
+      UF = DT1/D1 (U0 + DTI f V - DTI g EGF_x) - DTI U0 (W_k - W_k+1) / (D1 DZ_k)
+      VF = DT1/D1 (V - DTI f U0 - DTI g EGF_y) - DTI DT U0 c / D1
+           - DTI V (W_k - W_k+1) / (D1 DZ_k)
+    at elements with interior neighbours (W_1 = W_KB = 0)."""
+    c = F.Model.build(0, nx=21, ny=7, kb=6, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    p = F.Model.case_defaults(0).p
+    p.horizontal_closure = 0
+    m = F.Model.from_mesh(x, y, nv, np.full(c.m, 12.0), 6, [], p)
+    m.cold_start()
+    m.field("cc_hvc")[:] = 7.0
+    f = 1e-4
+    m.field("cor")[:] = f
+    kb, m1, n1 = m.kb, m.m + 1, m.n + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    o = Oracle(m)
+    xc, vx, vy = m.field("xc"), m.field("vx"), m.field("vy")
+    D, D1, U0, V0, cx, gx, gy = 11.0, 11.3, 0.6, -0.3, 2e-4, 3e-6, -2e-6
+    rng = np.random.default_rng(9)
+    u, v, w = np.zeros((kb, n1)), np.zeros((kb, n1)), np.zeros((kb, n1))
+    u[:-1, 1:] = U0
+    v[:-1, 1:] = V0 + cx * xc[None, 1:]
+    w[1:-1, 1:] = 1e-4 * rng.standard_normal((kb - 2, n1 - 1))
+    for nm, a in (("u", u.ravel()), ("v", v.ravel()), ("w", w.ravel()), ("dt", np.full(m1, D)),
+                  ("dt1", np.full(n1, D)), ("d1", np.full(n1, D1)), ("egf", gx * vx + gy * vy),
+                  ("drhox", np.zeros(kb * n1)), ("drhoy", np.zeros(kb * n1))):
+        o.put(nm, a)
+    o.routine("adv_uv")
+    uf, vf = o.get("uf").reshape(kb, n1), o.get("vf").reshape(kb, n1)
+    dz1, ge = m.field("dz1").reshape(kb, n1), m.field("grav_e")
+    isbce, nbe = m.field("isbce"), m.field("nbe").reshape(3, n1)
+    e = np.array([i for i in range(1, n1) if isbce[i] == 0 and all(
+        nbe[j, i] != 0 and isbce[nbe[j, i]] == 0 for j in range(3))])
+    assert len(e) > 100
+    for k in range(kb - 1):
+        dw = (w[k, e] - w[k + 1, e]) / (D1 * dz1[k, e])
+        vk = v[k, e]
+        eu = D / D1 * (U0 + dti * f * vk - dti * ge[e] * gx) - dti * U0 * dw
+        ev = D / D1 * (vk - dti * f * U0 - dti * ge[e] * gy) - dti * D * U0 * cx / D1 - dti * vk * dw
+        assert np.max(np.abs(uf[k, e] - eu)) < 1e-12, k
+        assert np.max(np.abs(vf[k, e] - ev)) < 1e-12, k
