@@ -245,7 +245,8 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext, bool fresh) {
     on_stream2(h, FV_CONC_2D);
     TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.cur.ua, a.cur.va, a.cur.d, a.cur.el, a.ef, a.elij);
     on_stream1(h);
-    TLAUNCH("ext_elevation", k_ext_elevation, g.M + 1, g, a);
+#define EXTEL_CALL(SM) TLAUNCH("ext_elevation", k_ext_elevation<SM>, g.M + 1, g, a)
+    FV_SMAX_DISPATCH(g.cv_max, EXTEL_CALL);
     join_streams(h, FV_CONC_2D);
     TLAUNCH("ext_momentum", k_ext_momentum<FT>, g.N + 1, g, a);
   }

@@ -92,6 +92,7 @@ __device__ __forceinline__ double tide_force(const DevGrid& g, int obc, double r
   return force * ramp;
 }
 
+template <int SMAX>
 __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   const int i = (int)xcd_thread();
   if (i > g.M) return;
@@ -99,13 +100,24 @@ __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   // sub-edge c, D1*(-UA*DLTYE + VA*DLTXE) of its element for side IB, is
   // exactly the negative for side IA (DLTXE/DLTYE negated; round to
   // nearest is symmetric), so the stored term is added or subtracted.
-  double xflux = 0.0;
+  // SMAX >= the node's sub-edge count: the slots past it re-read slot 0,
+  // so all list and term loads are unconditional and issue together.
   const int n = g.cv_cnt[i];
-  for (int j = 0; j < n; ++j) {
-    const int ent = g.cvl_ent[(long)j * g.m1 + i];
-    const double t = a.phi[ent >> 1];
-    xflux = (ent & 1) ? xflux + t : xflux - t;
-  }
+  int ce[SMAX];
+  static_for<SMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    ce[j] = g.cvl_ent[(long)(j < n ? j : 0) * g.m1 + i];
+  });
+  double t[SMAX];
+  static_for<SMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    t[j] = a.phi[ce[j] >> 1];
+  });
+  double xflux = 0.0;
+  static_for<SMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < n) xflux = (ce[j] & 1) ? xflux + t[j] : xflux - t[j];
+  });
   const int obc = g.node_obc[i];
   double xobc = 0.0;
   if (obc >= 0) { xobc = xflux; xflux = 0.0; a.xflux_obcn[obc] = xobc; }

@@ -232,11 +232,18 @@ __global__ __launch_bounds__(256) void k_vdif_q_r(DevGrid g, IntArgs a, double2*
     constexpr int j = decltype(jc)::value;
     re[j] = j < nt ? g.nbve[o + j] : 0;
   });
+  // ring values loaded unconditionally (slots past the ring read element
+  // 0) so that the loads issue together; the sum takes the ring in order
   auto ring_avg = [&](const double* f, long eo) {
+    double fv[RMAX];
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      fv[j] = f[eo + re[j]];
+    });
     double x = 0.0;
     static_for<RMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      if (j < nt) x = x + f[eo + re[j]];
+      if (j < nt) x = x + fv[j];
     });
     return x / ntd;
   };

@@ -48,15 +48,24 @@ __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const doubl
   const double art2 = g.art2[i], nn = g.nn_hvc[i];
   for (int k = k0; k <= k1; ++k) {
     const long ko = (long)(k - 1) * g.m1;
+    // the ring's values are loaded unconditionally (slots past the ring
+    // read node 0) so that the loads issue together; the sums take the
+    // ring's own terms in ring order
+    double fv[RMAX], mv[RMAX];
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      fv[j] = f[ko + rn[j]];
+      mv[j] = mean ? mean[ko + rn[j]] : cmean;
+    });
     double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-    double f1 = f[ko + rn[0]];
-    double m1v = mean ? mean[ko + rn[0]] : cmean;
+    double f1 = fv[0];
+    double m1v = mv[0];
     double lo = f1, hi = f1;
     static_for<RMAX - 1>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j < nt - 1) {
-        const double f2 = f[ko + rn[j + 1]];
-        const double m2v = mean ? mean[ko + rn[j + 1]] : cmean;
+        const double f2 = fv[j + 1];
+        const double m2v = mv[j + 1];
         const double ffd = 0.5 * (f1 + f2 - m1v - m2v);
         const double ff1 = 0.5 * (f1 + f2);
         s1 = s1 + ff1 * rdy[j];
@@ -98,10 +107,13 @@ __device__ __forceinline__ void ring_bounds(const int (&rn)[RMAX], int nt, const
                                             double& hi) {
   lo = f[ko + rn[0]];
   hi = lo;
+  // Slots past the ring repeat its first node (min and max are unchanged),
+  // so every load is unconditional and all of them issue before the first
+  // wait.
   static_for<RMAX>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
-    if (j >= 1 && j < nt - 1) {
-      const double v = f[ko + rn[j]];
+    if (j >= 1) {
+      const double v = f[ko + (j < nt - 1 ? rn[j] : rn[0])];
       lo = fmin(lo, v);
       hi = fmax(hi, v);
     }
@@ -278,11 +290,18 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1;
     const FT* fl = (const FT*)a.cvflux + (long)(k - 1) * cs;
+    // all SMAX loads issue together (slots past the node's count read the
+    // unused entry 0); the sum takes the node's own terms in sub-edge order
+    FT xv[SMAX];
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      xv[j] = fl[ce[j] >> 1];
+    });
     double xflux = 0.0;
     static_for<SMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j < n) {
-        const double x = (double)fl[ce[j] >> 1];
+        const double x = (double)xv[j];
         if (ce[j] & 1) xflux = xflux - x;
         else xflux = xflux + x;
       }
@@ -296,22 +315,26 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
     const double fk = f1[ko + i], dzk = DZ(i, k);
     // DZ-weighted values at the layer's upper (nu/du) and lower (nd/dd)
     // interfaces: the vertical flux (adv_t.F:873-881) and the FCT bounds
-    // (fct_t.F:82-101) use the same expressions
+    // (fct_t.F:82-101) use the same expressions.  The neighbours are loaded
+    // unconditionally (layer 1 reads itself as its upper neighbour, level
+    // KB exists below KBM1) so that the loads issue together.
+    const long kup = k != 1 ? ko - g.m1 : ko;
+    const double fup = f1[kup + i], dzm = DZ(i, k != 1 ? k - 1 : k);
+    const double fdn = f1[ko + g.m1 + i], dzp = DZ(i, k + 1);
+    const double wk = a.wts[ko + i], wkp = a.wts[ko + g.m1 + i];
     double temp, nu = 0.0, du = 1.0, nd = 0.0, dd = 1.0;
     const bool has_up = k != 1, has_dn = k == 1 || k != g.KBM1;   // the reference's branch order
     if (has_up) {
-      const double dzm = DZ(i, k - 1);
-      nu = fk * dzm + f1[ko - g.m1 + i] * dzk;
+      nu = fk * dzm + fup * dzk;
       du = dzk + dzm;
     }
     if (has_dn) {
-      const double dzp = DZ(i, k + 1);
-      nd = fk * dzp + f1[ko + g.m1 + i] * dzk;
+      nd = fk * dzp + fdn * dzk;
       dd = dzk + dzp;
     }
-    if (k == 1) temp = -a.wts[ko + g.m1 + i] * nd / dd;
-    else if (k == g.KBM1) temp = a.wts[ko + i] * nu / du;
-    else temp = a.wts[ko + i] * nu / du - a.wts[ko + g.m1 + i] * nd / dd;
+    if (k == 1) temp = -wkp * nd / dd;
+    else if (k == g.KBM1) temp = wk * nu / du;
+    else temp = wk * nu / du - wkp * nd / dd;
     if (isobc) xflux = temp * art1;
     else xflux = xflux + temp * art1;
     if (!g.river_ts_specified) {
@@ -353,15 +376,19 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
   for (int k = 2; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
     const double uk = a.u[eo + e], vk = a.v[eo + e];
-    const double dz1k = DZ1(e, k), dz1m = DZ1(e, k - 1);
+    const double dz1k = DZ1(e, k), dz1m = DZ1(e, k - 1), dzz1 = DZZ1(e, k - 1);
+    CvRecT<FT> r[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r[q] = rec[ko + nd[q]];
+    // Every load of the layer is issued before the two divisions: without
+    // the barrier the scheduler trades them for registers and waits on each
+    // record part in turn.
+    __builtin_amdgcn_sched_barrier(0);
     const FT uq = (FT)((uk * dz1m + um * dz1k) / (dz1k + dz1m));
     const FT vq = (FT)((vk * dz1m + vm * dz1k) / (dz1k + dz1m));
     um = uk;
     vm = vk;
-    const FT dtij = (FT)(dt1 * DZZ1(e, k - 1));
-    CvRecT<FT> r[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) r[q] = rec[ko + nd[q]];
+    const FT dtij = (FT)(dt1 * dzz1);
     FT* out = cvflux + (long)(k - 1) * cs + p0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -407,11 +434,18 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
   for (int k = 2; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.m1;
     const FT* fl = (const FT*)a.cvflux + (long)(k - 1) * cs;
+    // all SMAX loads issue together (slots past the node's count read the
+    // unused entry 0); the sum takes the node's own terms in sub-edge order
+    FT xv[SMAX];
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      xv[j] = fl[ce[j] >> 1];
+    });
     double xflux = 0.0;
     static_for<SMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j < n) {
-        const double x = (double)fl[ce[j] >> 1];
+        const double x = (double)xv[j];
         if (ce[j] & 1) xflux = xflux - x;
         else xflux = xflux + x;
       }
@@ -475,12 +509,32 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
   a.wts[NI(g, i, 1)] = 0.0;
   for (int k = 1; k <= KBM1; ++k) {
     const long ko = (long)(k - 1) * g.n1;
+    // Slots in groups of 8: a group's loads are unconditional (slots past
+    // the count read element 0) and issue together, then its terms are
+    // summed in slot order.
     double x = 0.0;
-    static_for<SMAX>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      if (j < n) {
-        const double dij = dte[j] * DZ1(ce[j], k);
-        x = x + dij * (-a.u[ko + ce[j]] * sdy[j] + a.v[ko + ce[j]] * sdx[j]);
+    constexpr int CH = 8;
+    static_for<(SMAX + CH - 1) / CH>([&](auto cc) {
+      constexpr int c0 = decltype(cc)::value * CH;
+      if (c0 < n) {
+        double uu[CH], vv[CH], dz[CH];
+        static_for<CH>([&](auto jc) {
+          constexpr int j = c0 + decltype(jc)::value;
+          if constexpr (j < SMAX) {
+            uu[j - c0] = a.u[ko + ce[j]];
+            vv[j - c0] = a.v[ko + ce[j]];
+            dz[j - c0] = DZ1(ce[j], k);
+          }
+        });
+        static_for<CH>([&](auto jc) {
+          constexpr int j = c0 + decltype(jc)::value;
+          if constexpr (j < SMAX) {
+            if (j < n) {
+              const double dij = dte[j] * dz[j - c0];
+              x = x + dij * (-uu[j - c0] * sdy[j] + vv[j - c0] * sdx[j]);
+            }
+          }
+        });
       }
     });
     double xflux = obc ? 0.0 : x;
@@ -533,9 +587,17 @@ __global__ __launch_bounds__(256) void k_viscofh_r(DevGrid g, IntArgs a) {
   const double art1 = g.art1[i];
   for (int k = 1; k <= g.KBM1; ++k) {
     const long ko = (long)(k - 1) * g.n1;
+    // the ring's velocities are loaded unconditionally (slots past the ring
+    // read element 0) so that the loads issue together
+    double ur[RMAX], vr[RMAX];
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      ur[j] = a.u[ko + re[j]];
+      vr[j] = a.v[ko + re[j]];
+    });
     double pupx = 0.0, pupy = 0.0, pvpx = 0.0, pvpy = 0.0;
     {
-      const double uu = a.u[ko + re[0]], vv = a.v[ko + re[0]];
+      const double uu = ur[0], vv = vr[0];
       pupx = pupx + uu * ry[0]; pupy = pupy + uu * rx[0];
       pvpx = pvpx + vv * ry[0]; pvpy = pvpy + vv * rx[0];
       if (bnd) {
@@ -546,7 +608,7 @@ __global__ __launch_bounds__(256) void k_viscofh_r(DevGrid g, IntArgs a) {
     static_for<RMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (j >= 1 && j <= nt - 2) {
-        const double uu = a.u[ko + re[j]], vv = a.v[ko + re[j]];
+        const double uu = ur[j], vv = vr[j];
         pupx = pupx + uu * ry[j]; pupy = pupy + uu * rx[j];
         pvpx = pvpx + vv * ry[j]; pvpy = pvpy + vv * rx[j];
       }
