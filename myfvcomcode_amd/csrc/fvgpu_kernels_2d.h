@@ -217,25 +217,34 @@ __device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const voi
   const EdgeFlux4T<FT>* ef = (const EdgeFlux4T<FT>*)ef_;
   xf = 0.0; yf = 0.0; px = 0.0; py = 0.0;
   const double ep = g.epor[i], iu = (double)g.iucp[i], gdi = g.grav_e[i] * d1[i];
+  // the three edges' loads first, then the sums with the side chosen by a
+  // select: no branch separates the loads, so they issue together
+  int ent[3];
+  EdgeFlux4T<FT> r[3];
+  double isbc_tmp[3], el[3], dlx[3], dly[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) ent[q] = g.ed_ent[3 * i + q];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const int ent = g.ed_ent[3 * i + q], e = ent >> 1;
-    const EdgeFlux4T<FT> r = ef[e];
-    const double xadv = (double)r.xadv, yadv = (double)r.yadv, fx = (double)r.fx, fy = (double)r.fy;
-    const double isbc_tmp = (double)g.isbc[e];
-    const double gd = gdi * elij[e];
-    const double dlx = g.dltxc[e], dly = g.dltyc[e];
-    if ((ent & 1) == 0) {
-      xf = xf + (xadv + fx * ep) * (1.0 - isbc_tmp) * iu;
-      yf = yf + (yadv + fy * ep) * (1.0 - isbc_tmp) * iu;
-      px = px - gd * dly;
-      py = py + gd * dlx;
-    } else {
-      xf = xf - (xadv + fx * ep) * (1.0 - isbc_tmp) * iu;
-      yf = yf - (yadv + fy * ep) * (1.0 - isbc_tmp) * iu;
-      px = px + gd * dly;
-      py = py - gd * dlx;
-    }
+    const int e = ent[q] >> 1;
+    r[q] = ef[e];
+    isbc_tmp[q] = (double)g.isbc[e];
+    el[q] = elij[e];
+    dlx[q] = g.dltxc[e];
+    dly[q] = g.dltyc[e];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double xadv = (double)r[q].xadv, yadv = (double)r[q].yadv, fx = (double)r[q].fx, fy = (double)r[q].fy;
+    const double gd = gdi * el[q];
+    const double tx = (xadv + fx * ep) * (1.0 - isbc_tmp[q]) * iu;
+    const double ty = (yadv + fy * ep) * (1.0 - isbc_tmp[q]) * iu;
+    const double gy = gd * dly[q], gx = gd * dlx[q];
+    const bool ib = (ent[q] & 1) != 0;
+    xf = ib ? xf - tx : xf + tx;
+    yf = ib ? yf - ty : yf + ty;
+    px = ib ? px + gy : px - gy;
+    py = ib ? py - gx : py + gx;
   }
   if (g.isbce[i] == 2) {
     xf = (xf + fluxobn[i] * ua[i]) * (double)g.iucp[i];

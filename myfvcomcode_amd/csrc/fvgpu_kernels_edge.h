@@ -103,17 +103,17 @@ __global__ __launch_bounds__(256) void k_adv3d_gather(DevGrid g, IntArgs a) {
   for (int k = 1; k <= g.KBM1; ++k) {
     const EdgeFlux4T<FT>* fk = (const EdgeFlux4T<FT>*)a.ef3 + (long)(k - 1) * es;
     double xf = 0.0, yf = 0.0;
+    // loads first, side by select: no branch separates the three loads
+    EdgeFlux4T<FT> r[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r[q] = fk[ee[q] >> 1];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const EdgeFlux4T<FT> r = fk[ee[q] >> 1];
-      const double xadv = (double)r.xadv, yadv = (double)r.yadv, fx = (double)r.fx, fy = (double)r.fy;
-      if ((ee[q] & 1) == 0) {
-        xf = xf + xadv * tp[q] + fx * ep;
-        yf = yf + yadv * tp[q] + fy * ep;
-      } else {
-        xf = xf - xadv * tp[q] - fx * ep;
-        yf = yf - yadv * tp[q] - fy * ep;
-      }
+      const double xadv = (double)r[q].xadv, yadv = (double)r[q].yadv, fx = (double)r[q].fx, fy = (double)r[q].fy;
+      const double ax = xadv * tp[q], ay = yadv * tp[q], bx = fx * ep, by = fy * ep;
+      const bool ib = (ee[q] & 1) != 0;
+      xf = ib ? (xf - ax) - bx : (xf + ax) + bx;
+      yf = ib ? (yf - ay) - by : (yf + ay) + by;
     }
     if (obc) { xf = 0.0; yf = 0.0; }
     for (int p = r0; p < r1; ++p) {
@@ -166,43 +166,48 @@ __global__ __launch_bounds__(256) void k_advuv_gather(DevGrid g, IntArgs a) {
     const FT* vk_ = (const FT*)a.efvisc + (long)(k - 1) * es;
     const double dzk = DZ1(i, k);
     double xf = 0.0, yf = 0.0, px = 0.0, py = 0.0, vm = 0.0;
+    // loads first, side by select: no branch separates the six loads
+    EdgeFlux4T<FT> r[3];
+    FT vis[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const int e = ee[q] >> 1;
-      const EdgeFlux4T<FT> r = fk[e];
-      const double xadv = (double)r.xadv, yadv = (double)r.yadv, fx = (double)r.fx, fy = (double)r.fy;
-      vm = vm + (double)vk_[e];
+      r[q] = fk[ee[q] >> 1];
+      vis[q] = vk_[ee[q] >> 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double xadv = (double)r[q].xadv, yadv = (double)r[q].yadv, fx = (double)r[q].fx, fy = (double)r[q].fy;
+      vm = vm + (double)vis[q];
       const double gd = ge * dt1 * dzk * elij[q];
-      if ((ee[q] & 1) == 0) {
-        xf = xf + xadv * tp[q] + fx * ep;
-        yf = yf + yadv * tp[q] + fy * ep;
-        px = px - gd * dly[q];
-        py = py + gd * dlx[q];
-      } else {
-        xf = xf - xadv * tp[q] - fx * ep;
-        yf = yf - yadv * tp[q] - fy * ep;
-        px = px + gd * dly[q];
-        py = py - gd * dlx[q];
-      }
+      const double ax = xadv * tp[q], ay = yadv * tp[q], bx = fx * ep, by = fy * ep;
+      const double gy = gd * dly[q], gx = gd * dlx[q];
+      const bool ib = (ee[q] & 1) != 0;
+      xf = ib ? (xf - ax) - bx : (xf + ax) + bx;
+      yf = ib ? (yf - ay) - by : (yf + ay) + by;
+      px = ib ? px + gy : px - gy;
+      py = ib ? py - gx : py + gx;
     }
     xf = xf + px;
     yf = yf + py;
     // vertical advection (adv_uv_edge_gcn.F:927-946)
     const double uk = U[ko + i], vk = V[ko + i];
+    // the layer's neighbours are loaded unconditionally (layer 1 reads
+    // itself as its upper neighbour; level KB exists below KBM1) so that
+    // they issue with the edge loads
+    const long kp = ko + g.n1, km = k != 1 ? ko - g.n1 : ko;
+    const double dzp = DZ1(i, k + 1), dzm = DZ1(i, k != 1 ? k - 1 : k);
+    const double wk = W[ko + i], wp = W[kp + i];
+    const double up = U[kp + i], vp = V[kp + i], um = U[km + i], vmm = V[km + i];
     double xfluxv, yfluxv;
     if (k == 1) {
-      const long kp = ko + g.n1; const double dzp = DZ1(i, k + 1);
-      xfluxv = -W[kp + i] * (uk * dzp + U[kp + i] * dzk) / (dzk + dzp);
-      yfluxv = -W[kp + i] * (vk * dzp + V[kp + i] * dzk) / (dzk + dzp);
+      xfluxv = -wp * (uk * dzp + up * dzk) / (dzk + dzp);
+      yfluxv = -wp * (vk * dzp + vp * dzk) / (dzk + dzp);
     } else if (k == g.KBM1) {
-      const long km = ko - g.n1; const double dzm = DZ1(i, k - 1);
-      xfluxv = W[ko + i] * (uk * dzm + U[km + i] * dzk) / (dzk + dzm);
-      yfluxv = W[ko + i] * (vk * dzm + V[km + i] * dzk) / (dzk + dzm);
+      xfluxv = wk * (uk * dzm + um * dzk) / (dzk + dzm);
+      yfluxv = wk * (vk * dzm + vmm * dzk) / (dzk + dzm);
     } else {
-      const long kp = ko + g.n1, km = ko - g.n1;
-      const double dzp = DZ1(i, k + 1), dzm = DZ1(i, k - 1);
-      xfluxv = W[ko + i] * (uk * dzm + U[km + i] * dzk) / (dzk + dzm) - W[kp + i] * (uk * dzp + U[kp + i] * dzk) / (dzk + dzp);
-      yfluxv = W[ko + i] * (vk * dzm + V[km + i] * dzk) / (dzk + dzm) - W[kp + i] * (vk * dzp + V[kp + i] * dzk) / (dzk + dzp);
+      xfluxv = wk * (uk * dzm + um * dzk) / (dzk + dzm) - wp * (uk * dzp + up * dzk) / (dzk + dzp);
+      yfluxv = wk * (vk * dzm + vmm * dzk) / (dzk + dzm) - wp * (vk * dzp + vp * dzk) / (dzk + dzp);
     }
     xf = xf + xfluxv * art + a.drhox[ko + i] - cor * vk * dt1 * dzk * art;
     yf = yf + yfluxv * art + a.drhoy[ko + i] + cor * uk * dt1 * dzk * art;
