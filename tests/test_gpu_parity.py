@@ -183,3 +183,22 @@ def test_river_forcing_changed_between_steps(built):
     assert not bad, bad
     with pytest.raises(RuntimeError):
         dev.set_forcing("wind", np.zeros(2 * (m.n + 1)))    # the channel has no wind
+
+
+def test_high_degree_node_parity(built):
+    """A node with 12 triangles around it (the reference's limit): the
+    widest register lists (rings of 16, sub-edge lists of 24) against the
+    oracle, eager and as one graph per step."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    from helpers import wheel_mesh
+    m = wheel_mesh()
+    for graph in (False, True):
+        print("case: wheel mesh (node of degree 12), 10 INTERNAL_STEPs, graph=%s" % graph, flush=True)
+        dev, orc = F.DeviceModel(m), Oracle(m)
+        dev.run(10, graph=graph)
+        orc.run(10)
+        rep = {}
+        worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+        assert not bad, bad
+        del dev

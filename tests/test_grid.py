@@ -194,3 +194,19 @@ def test_estuary_mesh_metrics(built):
     art, art1 = m.field("art")[1:], m.field("art1")[1:]
     assert abs(art.sum() - art1.sum()) < 1e-9 * art.sum()
     assert m.field("ntsn").max() <= 13
+
+
+def test_wheel_mesh_reaches_the_widest_lists(built):
+    """The high-degree test mesh really has a node with 12 triangles (ring of
+    13 entries, 24 control-volume sub-edges), so the device parity test on it
+    runs the RMAX = 16 and SMAX = 24 kernel variants; the oracle steps it."""
+    from helpers import wheel_mesh
+    from oracle import Oracle
+    m = wheel_mesh()
+    assert m.field("ntsn")[1:].max() == 13 and m.field("ntve")[1:].max() == 12
+    niec = m.field("niec").reshape(2, -1)[:, 1:]
+    cnt = np.bincount(niec.ravel(), minlength=m.m + 1)
+    assert cnt.max() == 24
+    o = Oracle(m)
+    o.run(5)
+    assert np.abs(o.get("ua")).max() > 1e-3 and np.all(np.isfinite(o.get("t1")))
