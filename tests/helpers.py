@@ -38,12 +38,13 @@ def compare(dev, orc, model, names, tol, report=None):
     return worst, bad
 
 
-def wheel_mesh(n=12, kb=6):
+def wheel_mesh(n=12, kb=6, **pkw):
     """A closed basin whose centre node has n triangles around it (12 is the
     reference's limit): the node's ring holds 13 entries and its control
     volume 2n sub-edges, so the device takes its widest register lists
     (rings of 16, sub-edge lists of 24).  Two rings of nodes at 2 and 4 km,
-    depth 15-20 m, a cosine tilt of the surface and a stratified T."""
+    depth 15-20 m, a cosine tilt of the surface and a stratified T; keyword
+    arguments override the case parameters."""
     import myfvcomcode_amd as F
     R = 2000.0
     pts = [(0.0, 0.0)]
@@ -67,7 +68,10 @@ def wheel_mesh(n=12, kb=6):
         ccw = (x2 - x1) * (y3 - y1) - (y2 - y1) * (x3 - x1) > 0
         rows.append((t[0], t[2], t[1]) if ccw else t)      # clockwise
     h = 15.0 + 5.0 * np.hypot(xy[:, 0], xy[:, 1]) / (2 * R)
-    m = F.Model.from_mesh(xy[:, 0], xy[:, 1], np.array(rows), h, kb, [], F.Model.case_defaults(0).p)
+    p = F.Model.case_defaults(0).p
+    for k, v in pkw.items():
+        setattr(p, k, v)
+    m = F.Model.from_mesh(xy[:, 0], xy[:, 1], np.array(rows), h, kb, [], p)
     x = m.field("vx")
     m.field("el")[1:] = 0.05 * np.cos(np.pi * x[1:] / (4 * R))
     m.field3("t1")[:-1, 1:] = 15.0 + np.linspace(3.0, 0.0, kb - 1)[:, None]

@@ -60,3 +60,31 @@ def test_mixed_conserves_closed_basin(built):
     assert abs(d1.volume - d0.volume) <= 1e-13 * d0.volume
     assert abs(d1.salt - d0.salt) <= 1e-12 * abs(d0.salt)
     assert abs(d1.heat - d0.heat) <= 1e-12 * abs(d0.heat)
+
+
+def test_mixed_high_degree_node(built):
+    """The mixed form on the mesh with a node of degree 12 (the widest
+    register lists in FP32): closed-basin budgets to round-off and the same
+    bounds against FP64 as above after 30 steps."""
+    import myfvcomcode_amd as F
+    from helpers import wheel_mesh
+    m = wheel_mesh(scalar_positivity=0, noflux_bot=0)
+    print("case: wheel mesh (node of degree 12), 30 steps, mixed vs fp64 and budgets", flush=True)
+    d64 = F.DeviceModel(m, precision="fp64")
+    dmx = F.DeviceModel(m, precision="mixed")
+    b0 = dmx.diagnostics()
+    d64.run(30, graph=True)
+    dmx.run(30, graph=False)
+    b1 = dmx.diagnostics()
+    assert abs(b1.volume - b0.volume) <= 1e-13 * b0.volume
+    assert abs(b1.salt - b0.salt) <= 1e-12 * abs(b0.salt)
+    assert abs(b1.heat - b0.heat) <= 1e-12 * abs(b0.heat)
+    rep = {}
+    for name, bound in BOUNDS.items():
+        a = interior(name, dmx.get(name), m)
+        b = interior(name, d64.get(name), m)
+        assert np.isfinite(a).all(), name
+        rep[name] = rms(a, b)
+    print(rep, flush=True)
+    for name, bound in BOUNDS.items():
+        assert rep[name] <= bound, (name, rep[name], bound)
