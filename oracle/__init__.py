@@ -51,7 +51,10 @@ class Oracle:
 
     def __init__(self, model):
         self.model = model  # keep the grid alive
-        self._o = C.c_void_p(lib().orc_create(model.ptr))
+        ptr = lib().orc_create(model.ptr)
+        if not ptr:
+            raise ValueError("the oracle takes 2 <= KB <= 64 (KB = %d)" % model.kb)
+        self._o = C.c_void_p(ptr)
 
     def internal_step(self):
         lib().orc_internal_step(self._o)

@@ -57,6 +57,11 @@ typedef struct orc {
   int depth_fail;
 } orc;
 
+/* Column work arrays hold levels 0..KB; KB is limited to ORC_MAXKB, the
+ * device library's own limit. */
+#define ORC_MAXKB 64
+#define ORC_COL (ORC_MAXKB + 2)
+
 static const double ALPHA_RK[5] = {0.0, 0.2500, 1.0 / 3.0, 0.5000, 1.0};
 static const double ONE_THIRD = 1.0 / 3.0;
 static const double GRAV = 9.81;
@@ -98,6 +103,7 @@ static double* zal(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double))
 
 /* ----------------------------------------------------------------- setup */
 orc* orc_create(const fv_model* g) {
+  if (g->kb < 2 || g->kb > ORC_MAXKB) return NULL;
   orc* o = (orc*)calloc(1, sizeof(orc));
   o->g = g;
   o->p = g->p;
@@ -610,7 +616,7 @@ static void baropg(orc* o) {
   memset(o->drhoy, 0, sizeof(double) * N1 * KB);
   for (int k = 1; k <= KB; ++k) { EL3(o->rmean, 0, k) = 0.0; EL3(o->rho, 0, k) = 0.0; }
   /* per element: RIJK(3,K) prefix sums DRIJK1 and DRIJK2 */
-  double rijk[3][64], drijk1[3][64], drijk2[64];
+  double rijk[3][ORC_COL], drijk1[3][ORC_COL], drijk2[ORC_COL];
   for (int i = 1; i <= N; ++i) {
     for (int j = 1; j <= 3; ++j) {
       int jj1 = j + 1 - ((j + 1) / 4) * 3, jj2 = j + 2 - ((j + 2) / 4) * 3;
@@ -837,7 +843,7 @@ static void adv_uv_edge_gcn(orc* o) {
 static void vdif_uv(orc* o) {
   const int N = o->N, KB = o->KB, KBM1 = o->KBM1, KBM2 = o->KBM2;
   const double dti = o->dti, umol = o->p.umol;
-  double a[64], c[64], vhu[64], vhv[64], vhpu[64], vhpv[64];
+  double a[ORC_COL], c[ORC_COL], vhu[ORC_COL], vhv[ORC_COL], vhpu[ORC_COL], vhpv[ORC_COL];
   for (int i = 1; i <= N; ++i) {
     if (!(o->d1[i] > 0.0)) continue;
     double d1 = o->d1[i];
@@ -1033,7 +1039,7 @@ static void vdif_q(orc* o) {
   double* Q2 = o->q2; double* Q2L = o->q2l; double* Q2F = o->q2f; double* Q2LF = o->q2lf;
   double* L = o->l; double* KM = o->km; double* KH = o->kh; double* KQ = o->kq;
   const double* DZ = G->dz; const double* DZZ = G->dzz; const double* Z = G->z;
-  double gh[64], sm[64], sh[64], prod[64], dtef[64], kn[64], boygr[64], a[64], c[64], vhp[64], vh[64], stf[64];
+  double gh[ORC_COL], sm[ORC_COL], sh[ORC_COL], prod[ORC_COL], dtef[ORC_COL], kn[ORC_COL], boygr[ORC_COL], a[ORC_COL], c[ORC_COL], vhp[ORC_COL], vh[ORC_COL], stf[ORC_COL];
   for (int k = 2; k <= KBM1; ++k)
     for (int i = 1; i <= M; ++i)
       if (ND(Q2F, i, k) <= SMALL || ND(Q2LF, i, k) <= SMALL) { ND(Q2F, i, k) = SMALL; ND(Q2LF, i, k) = SMALL; }
@@ -1246,7 +1252,7 @@ static void vdif_ts(orc* o, int ncon, double* f) {
   const int M = o->M, KB = o->KB, KBM1 = o->KBM1, KBM2 = o->KBM2;
   const double dti = o->dti, umolpr = o->p.umol * 1.E0;
   const double* DZ = G->dz; const double* DZZ = G->dzz;
-  double af[64], cf[64], vhf[64], vhpf[64], ffc[64];
+  double af[ORC_COL], cf[ORC_COL], vhf[ORC_COL], vhpf[ORC_COL], ffc[ORC_COL];
   for (int i = 1; i <= M; ++i) {
     const double d = o->d[i];
     double khbottom = 0.0;
@@ -1296,7 +1302,7 @@ static void bcond_ts(orc* o, int ncon) {
     }
   }
   if (!o->p.obc_on) return;
-  double ttmp[64];
+  double ttmp[ORC_COL];
   for (int i = 0; i < o->IOBCN; ++i) {
     int j = G->i_obc_n[i], j1 = G->next_obc[i];
     double t2d = 0.0, t2d_next = 0.0, xflux2d = 0.0;

@@ -202,3 +202,15 @@ def test_high_degree_node_parity(built):
         worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
         assert not bad, bad
         del dev
+
+
+@pytest.mark.parametrize("kb", [4, 64])
+def test_layer_count_extremes_parity(built, kb):
+    """KB = 4 (two interior Q levels) and KB = 64 (the device's limit: the
+    largest column register arrays) against the oracle."""
+    m, dev, orc = _pair(0, nx=21, ny=7, kb=kb)
+    print("case: 21x7 estuary channel, KB=%d, 3 INTERNAL_STEPs (graph)" % kb, flush=True)
+    dev.run(3, graph=True)
+    orc.run(3)
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0)
+    assert not bad, bad

@@ -207,3 +207,22 @@ def test_detmath_accuracy(built, name, ref, lo, hi, maxulp):
         if b != 0:
             worst = max(worst, abs(a - b) / math.ulp(b))
     assert worst <= maxulp
+
+
+@pytest.mark.parametrize("kb", [4, 64])
+def test_oracle_layer_count_extremes(built, kb):
+    """The fewest levels the grid accepts (KB = 4) and the most the device
+    takes (KB = 64): the oracle steps both (its column work arrays hold
+    levels 0..KB; an AddressSanitizer build of it runs these cases clean)."""
+    m = F.Model.build(0, nx=21, ny=7, kb=kb, river=0)
+    o = Oracle(m)
+    o.run(3)
+    for name in ("q2", "q2l", "km", "t1", "s1", "u", "el"):
+        assert np.all(np.isfinite(o.get(name))), name
+    assert np.abs(o.get("ua")).max() > 1e-4
+
+
+def test_oracle_rejects_kb_beyond_limit(built):
+    m = F.Model.build(0, nx=21, ny=7, kb=65)
+    with pytest.raises(ValueError):
+        Oracle(m)
