@@ -74,11 +74,18 @@ __global__ __launch_bounds__(256) void k_closure_features(DevGrid g, IntArgs a, 
   wus = wus / ntd;
   wvs = wvs / ntd;
   const float ustar = (float)sqrt(sqrt(wus * wus + wvs * wvs));
+  // ring values loaded unconditionally (slots past the ring read element
+  // 0) so that the loads issue together; the sum takes the ring in order
   auto ring_avg = [&](const double* f, long eo) {
+    double fv[RMAX];
+    static_for<RMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      fv[j] = f[eo + re[j]];
+    });
     double s = 0.0;
     static_for<RMAX>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      if (j < nt) s = s + f[eo + re[j]];
+      if (j < nt) s = s + fv[j];
     });
     return s / ntd;
   };
