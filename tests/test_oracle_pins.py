@@ -676,3 +676,57 @@ def test_adv_uv_momentum_terms(built):
         ev = D / D1 * (vk - dti * f * U0 - dti * ge[e] * gy) - dti * D * U0 * cx / D1 - dti * vk * dw
         assert np.max(np.abs(uf[k, e] - eu)) < 1e-12, k
         assert np.max(np.abs(vf[k, e] - ev)) < 1e-12, k
+
+
+def test_bcond_ts_river_values_and_local_bounds(built):
+    """BCOND_TS (bcond_ts.F:60-95, 96-160): with the river's T and S given,
+    every layer of the river node takes them (both tracers in one call, as
+    the reference does); at an outflow OBC node the Blumberg-Kantha value is
+    kept where it lies inside the range of the node's ring and column
+    neighbours and clipped into that range where a large inflowing flux
+    drives it out (the FCT principle: no new extrema)."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8)
+    assert m.hdr.p.river_ts_specified == 1 and m.hdr.numqbc == 1
+    kb, m1, nob = m.kb, m.m + 1, m.hdr.iobcn
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    o = Oracle(m)
+    j = int(m.field("inodeq")[0])
+    t1 = np.zeros((kb, m1))
+    t1[:-1, 1:] = 12.0 + np.random.default_rng(6).uniform(-0.5, 0.5, (kb - 1, m1 - 1))
+    o.put("t1", t1.ravel())
+    o.put("tf1", t1.ravel())
+    o.put("sf1", t1.ravel())
+    uo = np.full(nob, 10.0)
+    o.put("uard_obcn", uo)
+    xf = np.zeros((kb, nob))
+    xf[:-1, 0] = -5e4            # OBC node 0: a large inflow of tracer content
+    o.put("xflux_obc", xf.ravel())
+    o.routine("bcond_ts", 1)
+    tf = o.get("tf1").reshape(kb, m1)
+    sf = o.get("sf1").reshape(kb, m1)
+    assert np.all(tf[:-1, j] == m.field("tdis")[0]) and np.all(sf[:-1, j] == m.field("sdis")[0])
+    obc, nxt, dln = m.field("i_obc_n"), m.field("next_obc"), m.field("dltn_obc")
+    DZ, art1, h, g = m.field("dz").reshape(kb, m1), m.field("art1"), m.field("h"), m.field("grav_n")
+    d, dt = o.get("d"), o.get("dt")
+    nbsn, off, ntsn = m.field("nbsn"), m.field("nbsn_off"), m.field("ntsn")
+    kept = clipped = 0
+    for q, jo in enumerate(obc):
+        j1 = nxt[q]
+        t2d = np.sum(t1[:-1, jo] * DZ[:-1, jo])
+        t2n = np.sum(t1[:-1, j1] * DZ[:-1, j1])
+        mean = (t2d * dt[jo] - (np.sum(xf[:-1, q]) + t2d * uo[q]) * dti / art1[jo]) / d[jo]
+        cc = np.sqrt(g[jo] * h[jo]) * dti / dln[q]
+        raw = mean + (cc * (t1[:-1, j1] - t2n) + (t1[:-1, jo] - t2d) * (1 - dti / 10800.0)) / (1 + cc)
+        ring = list(nbsn[off[jo]:off[jo] + ntsn[jo]])
+        for k in range(kb - 1):
+            col = t1[max(k - 1, 0):min(k + 2, kb - 1), jo]
+            lo = min(t1[k, ring].min(), col.min())
+            hi = max(t1[k, ring].max(), col.max())
+            assert lo <= tf[k, jo] <= hi
+            if t1[k, ring].min() < raw[k] < t1[k, ring].max():
+                assert abs(tf[k, jo] - raw[k]) < 1e-12
+                kept += 1
+            elif raw[k] > hi:
+                assert tf[k, jo] < raw[k]
+                clipped += 1
+    assert kept > 0 and clipped > 0
