@@ -17,6 +17,7 @@ does not come from the code under test:
   ADVAVE_EDGE_GCN  pressure terms of a linear elevation; closing momentum
                 sums of divergence-free linear velocities
   ADVECTION_EDGE_GCN  the same, layer by layer
+  ADV_Q         horizontal and vertical advection of linear fields
   ADV_UV_EDGE_GCN  UF/VF term by term: advection, pressure, Coriolis, omega
   BAROPG        hydrostatic baroclinic force on a flat bottom (baropg.F)
   VDIF_UV       column momentum budget with surface and implicit bottom
@@ -730,3 +731,49 @@ def test_bcond_ts_river_values_and_local_bounds(built):
                 assert tf[k, jo] < raw[k]
                 clipped += 1
     assert kept > 0 and clipped > 0
+
+
+def test_adv_q_linear_fields(built):
+    """ADV_Q (adv_q.F:35-418) on a flat bottom with DT = D and constant
+    horizontal diffusivity: a field linear in x carried by a uniform U0
+    gives exactly Q2F = Q2 - DTI U0 dQ/dx at nodes whose ring is interior
+    (the face reconstruction is exact, the CV fluxes close, the midpoint
+    rule is exact); a field linear in the level index under a uniform omega
+    W0 gives Q2F = Q2 + DTI W0 (dQ/dk) / (D dsigma) (the central vertical
+    flux is exact for it)."""
+    c = F.Model.build(0, nx=21, ny=7, kb=8, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    p = F.Model.case_defaults(0).p
+    p.horizontal_closure = 0
+    m = F.Model.from_mesh(x, y, nv, np.full(c.m, 12.0), 8, [], p)
+    m.cold_start()
+    m.field("nn_hvc")[:] = 3.0
+    kb, m1, n1 = m.kb, m.m + 1, m.n + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    o = Oracle(m)
+    vx, D, U0, ax = m.field("vx"), 12.0, 0.3, 2e-8
+    q = np.zeros((kb, m1))
+    q[:, 1:] = 3e-4 + ax * vx[1:]
+    u = np.zeros((kb, n1))
+    u[:-1, 1:] = U0
+    for nm, arr in (("q2", q.ravel()), ("u", u.ravel()), ("v", np.zeros(kb * n1)), ("wts", np.zeros(kb * m1)),
+                    ("d", np.full(m1, D)), ("dt", np.full(m1, D)), ("dt1", np.full(n1, D))):
+        o.put(nm, arr)
+    o.routine("adv_q")
+    qf = o.get("q2f").reshape(kb, m1)
+    isonb = m.field("isonb")
+    nbsn, off, ntsn = m.field("nbsn"), m.field("nbsn_off"), m.field("ntsn")
+    deep = np.array([i for i in range(1, m1) if isonb[i] == 0 and
+                     all(isonb[j] == 0 for j in nbsn[off[i]:off[i] + ntsn[i]])])
+    assert len(deep) > 30
+    assert np.max(np.abs(qf[1:-1, deep] - (q[1:-1, deep] - dti * U0 * ax))) < 1e-16
+    # vertical
+    W0, beta = 1e-4, 1e-5
+    q[:, 1:] = (3e-4 + beta * np.arange(kb))[:, None]
+    for nm, arr in (("q2", q.ravel()), ("u", np.zeros(kb * n1)), ("wts", np.full(kb * m1, W0))):
+        o.put(nm, arr)
+    o.routine("adv_q")
+    qf = o.get("q2f").reshape(kb, m1)
+    dsig = m.field("dz").reshape(kb, m1)[0, 1]
+    assert np.max(np.abs(qf[1:-1, 1:] - (q[1:-1, 1:] + dti * W0 * beta / (D * dsig)))) < 1e-13
