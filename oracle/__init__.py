@@ -103,7 +103,8 @@ class Oracle:
         """Run one routine of the step on the current state (viscof_h,
         wreal, vdif_q, bcond_ts with NCON = arg, adjust2d3d of type arg,
         bottom_roughness, baropg, vdif_uv, advave (ADVUA/ADVVA), advection
-        (ADVX/ADVY), adv_uv (UF/VF), adv_q (Q2F from Q2), bcond_gcn2 on
+        (ADVX/ADVY), adv_uv (UF/VF), adv_q (Q2F from Q2), adv_ts (TF1 or SF1
+        for NCON = arg), bcond_gcn2 on
         UAF/VAF, bcond_gcn5 on U/V)."""
         if lib().orc_routine(self._o, name.encode(), int(arg)) != 0:
             raise KeyError(name)

@@ -1496,6 +1496,7 @@ int orc_routine(orc* o, const char* name, int arg) {
   else if (strcmp(name, "advection") == 0) advection_edge_gcn(o, o->advx, o->advy);
   else if (strcmp(name, "adv_uv") == 0) adv_uv_edge_gcn(o);
   else if (strcmp(name, "adv_q") == 0) adv_q(o, o->q2, o->q2f);
+  else if (strcmp(name, "adv_ts") == 0) adv_ts(o, arg);
   else if (strcmp(name, "bcond_gcn2") == 0)
     for (int i = 1; i <= o->N; ++i) wall_project(o, i, &o->uaf[i], &o->vaf[i]);
   else if (strcmp(name, "bcond_gcn5") == 0)

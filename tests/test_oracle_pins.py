@@ -17,7 +17,7 @@ does not come from the code under test:
   ADVAVE_EDGE_GCN  pressure terms of a linear elevation; closing momentum
                 sums of divergence-free linear velocities
   ADVECTION_EDGE_GCN  the same, layer by layer
-  ADV_Q         horizontal and vertical advection of linear fields
+  ADV_Q, ADV_T  advection of linear fields
   ADV_UV_EDGE_GCN  UF/VF term by term: advection, pressure, Coriolis, omega
   BAROPG        hydrostatic baroclinic force on a flat bottom (baropg.F)
   VDIF_UV       column momentum budget with surface and implicit bottom
@@ -777,3 +777,37 @@ def test_adv_q_linear_fields(built):
     qf = o.get("q2f").reshape(kb, m1)
     dsig = m.field("dz").reshape(kb, m1)[0, 1]
     assert np.max(np.abs(qf[1:-1, 1:] - (q[1:-1, 1:] + dti * W0 * beta / (D * dsig)))) < 1e-13
+
+
+def test_adv_t_linear_field(built):
+    """ADV_T (adv_t.F:34-1077) under the same flat-bottom set-up as ADV_Q:
+    a temperature linear in x carried by a uniform U0 gives exactly
+    TF1 = T1 - DTI U0 dT/dx at nodes with interior rings, every layer."""
+    c = F.Model.build(0, nx=21, ny=7, kb=8, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    p = F.Model.case_defaults(0).p
+    p.horizontal_closure = 0
+    m = F.Model.from_mesh(x, y, nv, np.full(c.m, 12.0), 8, [], p)
+    m.cold_start()
+    m.field("nn_hvc")[:] = 3.0
+    kb, m1, n1 = m.kb, m.m + 1, m.n + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    o = Oracle(m)
+    vx, D, U0, ax = m.field("vx"), 12.0, 0.3, 2e-5
+    t1 = np.zeros((kb, m1))
+    t1[:-1, 1:] = 10.0 + ax * vx[1:]
+    u = np.zeros((kb, n1))
+    u[:-1, 1:] = U0
+    for nm, arr in (("t1", t1.ravel()), ("tmean1", np.zeros(kb * m1)), ("u", u.ravel()),
+                    ("v", np.zeros(kb * n1)), ("wts", np.zeros(kb * m1)), ("d", np.full(m1, D)),
+                    ("dt", np.full(m1, D)), ("dtfa", np.full(m1, D)), ("dt1", np.full(n1, D))):
+        o.put(nm, arr)
+    o.routine("adv_ts", 1)
+    tf = o.get("tf1").reshape(kb, m1)
+    isonb = m.field("isonb")
+    nbsn, off, ntsn = m.field("nbsn"), m.field("nbsn_off"), m.field("ntsn")
+    deep = np.array([i for i in range(1, m1) if isonb[i] == 0 and
+                     all(isonb[j] == 0 for j in nbsn[off[i]:off[i] + ntsn[i]])])
+    assert len(deep) > 30
+    assert np.max(np.abs(tf[:-1, deep] - (t1[:-1, deep] - dti * U0 * ax))) < 1e-12
