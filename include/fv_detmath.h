@@ -59,6 +59,7 @@ FV_HD double fv_cos(double x) {
   const double PIO2_MD = 6.07710050630396597660e-11;
   const double PIO2_LO = 2.02226624879595063154e-21;
   const double TWO_OVER_PI = 6.3661977236758138e-01;
+  if (!(x - x == 0.0)) return x - x;  /* NaN or infinite argument: NaN */
   double n = fv_rint(x * TWO_OVER_PI);
   double r = ((x - n * PIO2_HI) - n * PIO2_MD) - n * PIO2_LO;
   long q = (long)n;
@@ -80,6 +81,7 @@ FV_HD double fv_cos(double x) {
 }
 
 FV_HD double fv_expm1(double y) {
+  if (y != y) return y;             /* NaN: no integer conversion of it below */
   if (y > 709.0) return 1.0 / 0.0;
   if (y < -40.0) return -1.0;
   const double LN2_HI = 6.93147180369123816490e-01; /* 33 bits */

@@ -67,10 +67,10 @@ class Oracle:
 
     def _len(self, name):
         m1, n1, kb = self.model.m + 1, self.model.n + 1, self.model.kb
-        node2 = {"el", "et", "d", "dt", "elf", "dtfa", "egf"}
+        node2 = {"el", "et", "d", "dt", "elf", "dtfa", "egf", "elrk"}
         elem2 = {"ua", "va", "el1", "et1", "d1", "dt1", "uard", "vard", "adx2d", "ady2d", "drx2d",
                  "dry2d", "advua", "advva", "pstx", "psty", "cbc", "wubot", "wvbot", "wusurf",
-                 "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf"}
+                 "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf", "uark", "vark", "elrk1", "elf1"}
         elem3 = {"u", "v", "w", "ww", "uf", "vf", "advx", "advy", "drhox", "drhoy", "rho", "rmean",
                  "t", "s", "km1", "viscofm"}
         if name == "ramp":
@@ -104,8 +104,10 @@ class Oracle:
         wreal, vdif_q, bcond_ts with NCON = arg, adjust2d3d of type arg,
         bottom_roughness, baropg, vdif_uv, advave (ADVUA/ADVVA), advection
         (ADVX/ADVY), adv_uv (UF/VF), adv_q (Q2F from Q2), adv_ts (TF1 or SF1
-        for NCON = arg), bcond_gcn2 on
-        UAF/VAF, bcond_gcn5 on U/V)."""
+        for NCON = arg), extuv (EXTUV_EDGE of RK stage arg), vertvl,
+        fct_ts (FCT_T for arg 1, FCT_S for 2), fct_q (FCT_Q2 for arg 1,
+        FCT_Q2L for 2), adjust_ts, bcond_gcn2 on UAF/VAF, bcond_gcn5 on
+        U/V)."""
         if lib().orc_routine(self._o, name.encode(), int(arg)) != 0:
             raise KeyError(name)
 

@@ -178,7 +178,8 @@ static double* orc_lookup(orc* o, const char* name, size_t* len) {
       {"pstx", o->pstx, o->n1}, {"psty", o->psty, o->n1}, {"cbc", o->cbc, o->n1},
       {"wubot", o->wubot, o->n1}, {"wvbot", o->wvbot, o->n1}, {"wusurf", o->wusurf, o->n1},
       {"wvsurf", o->wvsurf, o->n1}, {"wusurf2", o->wusurf2, o->n1}, {"wvsurf2", o->wvsurf2, o->n1},
-      {"uaf", o->uaf, o->n1}, {"vaf", o->vaf, o->n1},
+      {"uaf", o->uaf, o->n1}, {"vaf", o->vaf, o->n1}, {"uark", o->uark, o->n1}, {"vark", o->vark, o->n1},
+      {"elrk1", o->elrk1, o->n1}, {"elf1", o->elf1, o->n1}, {"elrk", o->elrk, o->m1},
       {"u", o->u, nk}, {"v", o->v, nk}, {"w", o->w, nk}, {"ww", o->ww, nk}, {"uf", o->uf, nk},
       {"vf", o->vf, nk}, {"advx", o->advx, nk}, {"advy", o->advy, nk}, {"drhox", o->drhox, nk},
       {"drhoy", o->drhoy, nk}, {"rho", o->rho, nk}, {"rmean", o->rmean, nk}, {"t", o->t, nk},
@@ -1346,6 +1347,22 @@ static void bcond_ts(orc* o, int ncon) {
   for (int k = 1; k <= KBM1; ++k) { EL3(o->t, 0, k) = 0.0; EL3(o->s, 0, k) = 0.0; }
 }
 
+/* ADJUST_TS, FVCOM_source/adjust_ts.F:150-172, called when RIVER_TS_SETTING
+ * is 'calculated' (internal_step.F:1374-1376): a river node's T1/S1 on every
+ * layer is raised to the river's TDIS/SDIS, then N2E3D of both. */
+static void adjust_ts(orc* o) {
+  if (o->NUMQBC <= 0) return;
+  for (int i = 0; i < o->NUMQBC; ++i) {
+    int j = G->inodeq[i];
+    for (int k = 1; k <= o->KBM1; ++k) {
+      ND(o->t1, j, k) = fmax(ND(o->t1, j, k), G->tdis[i]);
+      ND(o->s1, j, k) = fmax(ND(o->s1, j, k), G->sdis[i]);
+    }
+  }
+  n2e3d(o, o->t1, o->t);
+  n2e3d(o, o->s1, o->s);
+}
+
 /* DENS2, FVCOM_source/eqs_of_state.F:254-403 */
 static void dens2(orc* o) {
   for (int k = 1; k <= o->KB; ++k)
@@ -1432,6 +1449,7 @@ void orc_internal_step(orc* o) {
     memcpy(o->s1, o->sf1, sizeof(double) * M1 * KB);
     n2e3d(o, o->s1, o->s);
   }
+  if (!o->p.river_ts_specified) adjust_ts(o);
   if (!o->p.barotropic) dens2(o);
   memcpy(o->u, o->uf, sizeof(double) * N1 * KB);
   memcpy(o->v, o->vf, sizeof(double) * N1 * KB);
@@ -1497,6 +1515,11 @@ int orc_routine(orc* o, const char* name, int arg) {
   else if (strcmp(name, "adv_uv") == 0) adv_uv_edge_gcn(o);
   else if (strcmp(name, "adv_q") == 0) adv_q(o, o->q2, o->q2f);
   else if (strcmp(name, "adv_ts") == 0) adv_ts(o, arg);
+  else if (strcmp(name, "extuv") == 0) extuv_edge(o, arg);
+  else if (strcmp(name, "vertvl") == 0) vertvl_edge(o);
+  else if (strcmp(name, "fct_ts") == 0) { if (arg == 1) fct_ts(o, o->t1, o->tf1); else fct_ts(o, o->s1, o->sf1); }
+  else if (strcmp(name, "fct_q") == 0) { if (arg == 1) fct_q(o, o->q2, o->q2f); else fct_q(o, o->q2l, o->q2lf); }
+  else if (strcmp(name, "adjust_ts") == 0) adjust_ts(o);
   else if (strcmp(name, "bcond_gcn2") == 0)
     for (int i = 1; i <= o->N; ++i) wall_project(o, i, &o->uaf[i], &o->vaf[i]);
   else if (strcmp(name, "bcond_gcn5") == 0)
