@@ -372,6 +372,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   } else if (s_on) {
     enqueue_ts_pass<FT>(h, a, 2, fct);
   }
+  if (!g.river_ts_specified && g.NUMQBC > 0) LAUNCH("adjust_ts", k_adjust_ts, 1, 256, g, a);
   if (!h->md->p.barotropic) {
     TLAUNCH("dens2", k_dens2, mk, g, a);
     TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.rho1, a.rho, g.KB);

@@ -5,6 +5,33 @@
 #pragma once
 #include "fvgpu_kernels_3d.h"
 
+// ADJUST_TS (adjust_ts.F:150-172), run when RIVER_TS_SETTING is
+// 'calculated' (internal_step.F:1374-1376), in one workgroup: T1/S1 of
+// every river node raised to its river's TDIS/SDIS on layers 1..KBM1, one
+// thread per layer walking the rivers in order (two rivers at one node see
+// each other's result, as in the reference's loop); then N2E3D of T1 and S1
+// for the elements around the river nodes, the only elements whose three
+// nodes changed (the others' T/S already hold the same values).
+__global__ void k_adjust_ts(DevGrid g, IntArgs a) {
+  for (int k = threadIdx.x + 1; k <= g.KBM1; k += blockDim.x)
+    for (int r = 0; r < g.NUMQBC; ++r) {
+      const long p = NI(g, g.inodeq[r], k);
+      a.t1[p] = fmax(a.t1[p], g.tdis[r]);
+      a.s1[p] = fmax(a.s1[p], g.sdis[r]);
+    }
+  __syncthreads();
+  const double one_third = 1.0 / 3.0;
+  for (int r = 0; r < g.NUMQBC; ++r) {
+    const int j = g.inodeq[r], off = g.nbve_off[j], nt = g.ntve[j];
+    for (int q = threadIdx.x; q < nt * g.KB; q += blockDim.x) {
+      const int i = g.nbve[off + q % nt], k = q / nt + 1;
+      const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
+      a.t[EI(g, i, k)] = one_third * (a.t1[NI(g, j1, k)] + a.t1[NI(g, j2, k)] + a.t1[NI(g, j3, k)]);
+      a.s[EI(g, i, k)] = one_third * (a.s1[NI(g, j1, k)] + a.s1[NI(g, j2, k)] + a.s1[NI(g, j3, k)]);
+    }
+  }
+}
+
 // BCOND_TS(NCON) (bcond_ts.F:28-268) in two kernels.  Part 1, one thread
 // per OBC node: the column sums of the node (T2D, the interior neighbour's
 // T2D_NEXT, the OBC advective flux XFLUX2D) in layer order, into

@@ -204,13 +204,96 @@ def test_high_degree_node_parity(built):
         del dev
 
 
-@pytest.mark.parametrize("kb", [4, 64])
+@pytest.mark.parametrize("kb", [4, 9, 17, 21, 33, 41, 49, 64])
 def test_layer_count_extremes_parity(built, kb):
-    """KB = 4 (two interior Q levels) and KB = 64 (the device's limit: the
-    largest column register arrays) against the oracle."""
+    """Every register-array bound of the column kernels against the oracle:
+    KB = 4 (two interior Q levels), 9, 17, 21 (the ensemble member's 20
+    layers), 33, 41 (the 1M-element case's 40 layers), 49, and KB = 64 (the
+    device's limit: the largest column register arrays)."""
     m, dev, orc = _pair(0, nx=21, ny=7, kb=kb)
     print("case: 21x7 estuary channel, KB=%d, 3 INTERNAL_STEPs (graph)" % kb, flush=True)
     dev.run(3, graph=True)
     orc.run(3)
     worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0)
     assert not bad, bad
+
+
+@pytest.mark.parametrize("kb", [21, 41])
+def test_jittered_mesh_real_layer_counts(built, kb):
+    """The seamount generator (jitter, seamounts, wind, M2/S2/K1) at
+    101x51 nodes with the layer counts the benchmark and the ensemble member
+    run: KB = 41 (40 layers, the KBM1 = 40 column kernels) and KB = 21."""
+    m, dev, orc = _pair(1, nx=101, ny=51, kb=kb)
+    n = 4
+    print("case: jittered seamount mesh 101x51 KB=%d, %d steps (graph)" % (kb, n), flush=True)
+    dev.run(n, graph=True)
+    orc.run(n)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    for k, v in sorted(rep.items(), key=lambda x: -x[1])[:6]:
+        print("  %-8s %.3e" % (k, v))
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_calculated_river_parity(built, graph):
+    """RIVER_TS_SETTING 'calculated': the river's T/S enter through the
+    advective flux and ADJUST_TS raises the river node to them after the
+    tracer passes (adjust_ts.F); a warm river into the channel."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    m = F.Model.build(0, river_t=24.0, river_s=3.0)
+    m.hdr.p.river_ts_specified = 0
+    dev, orc = F.DeviceModel(m), Oracle(m)
+    n = 6
+    print("case: 81x21 estuary channel, calculated river T/S, %d steps, graph=%s" % (n, graph), flush=True)
+    dev.run(n, graph=graph)
+    orc.run(n)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    assert not bad, bad
+    j = int(m.field("inodeq")[0])
+    t1 = dev.get("t1").reshape(m.kb, -1)
+    assert np.all(t1[:-1, j] >= 24.0)
+
+
+def test_nonfinite_state_is_reported(built):
+    """A NaN written into UA at one element: the next internal step spreads
+    it and fvgpu_diagnostics reports a non-finite state, as the oracle's
+    state shows; the run itself does not stop (the reference checks after
+    the step, SHUTDOWN_CHECK)."""
+    m, dev, orc = _pair(0, nx=21, ny=7, kb=6)
+    print("case: 21x7 channel, NaN put into UA, 1 INTERNAL_STEP", flush=True)
+    assert dev.diagnostics().nonfinite == 0
+    ua = dev.get("ua")
+    ua[m.n // 2] = np.nan
+    dev.put("ua", ua)
+    orc.put("ua", ua)
+    dev.run(1, graph=False)
+    orc.run(1)
+    d = dev.diagnostics()
+    assert d.nonfinite == 1
+    assert not np.all(np.isfinite(orc.get("ua")))
+    # the NaN reached the same entries on both sides
+    for name in ("ua", "el", "u", "t1"):
+        a, b = dev.get(name), orc.get(name)
+        assert np.array_equal(np.isnan(a), np.isnan(b)), name
+
+
+def test_depth_below_min_depth_is_reported(built):
+    """An elevation that leaves a node's depth below MIN_DEPTH: DEPTH_CHECK
+    (depth_check.F) flags it in the external step, on the device as in the
+    oracle."""
+    m, dev, orc = _pair(0, nx=21, ny=7, kb=6)
+    print("case: 21x7 channel, EL set to leave D < MIN_DEPTH at one node, 1 INTERNAL_STEP", flush=True)
+    assert dev.diagnostics().depth_fail == 0
+    h = m.field("h")
+    j = m.m // 2
+    el = dev.get("el")
+    el[j] = -h[j] + 0.2 * m.hdr.p.min_depth
+    dev.put("el", el)
+    orc.put("el", el)
+    dev.run(1, graph=False)
+    orc.run(1)
+    assert dev.diagnostics().depth_fail == 1
+    assert orc.depth_fail
