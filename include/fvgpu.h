@@ -220,6 +220,15 @@ int  fv_model_from_mesh(int32_t m, int32_t n, const double* x, const double* y,
                         const fv_params* p, fv_model** out);
 void fv_model_free(fv_model* mdl);
 int64_t fv_extent_len(const fv_model* mdl, int32_t extent);
+/* Reverse Cuthill-McKee numbering of a mesh (north-star locality
+ * requirement; the reference keeps the grid file's numbering): nodes by
+ * RCM of the node graph from a pseudo-peripheral node per component,
+ * elements by their lowest/middle/highest new node index.  node_new[j-1]
+ * and elem_new[e-1] receive the 1-based new index of old node j and old
+ * element e; a host renumbers its mesh with them before
+ * fv_model_from_mesh, so that the gather kernels read neighbouring
+ * entities from neighbouring addresses.                                   */
+int  fv_rcm_order(int32_t m, int32_t n, const int32_t* nv_rows, int32_t* node_new, int32_t* elem_new);
 /* Look up one array by its ALL_VARS name (lower case).  *dtype: 0 double,
  * 1 int32.  Returns non-zero if the name is unknown.                     */
 int  fv_model_field(const fv_model* mdl, const char* name, void** ptr,
@@ -299,6 +308,16 @@ int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
  * increasing order of A(p,i)*X5(i,j).                                    */
 int  fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
                            const int64_t* elem_off, int64_t m, int64_t n, const double* x5);
+/* fvgpu_ensemble_update without any device allocation: X5 and the member
+ * offsets are staged in a fixed 33,280-byte device array of the library
+ * (one per GPU, looked up by fvgpu_create) instead of a buffer the handle
+ * allocates on its first update; the same kernel and results.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  nrens <= 64, as for fvgpu_ensemble_update.  Every call on
+ * one GPU shares the staging array: calls that overlap on one GPU race.   */
+int  fvgpu_ensemble_update_noalloc(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                                   const int64_t* elem_off, int64_t m, int64_t n, const double* x5);
 /* Time-varying forcing between internal steps, taking effect from the
  * next step.  The reference interpolates its forcing files in time inside
  * the step (UPDATE_RIVERS at IntTime, bcond_gcn.F:633-641; UPDATE_WIND and

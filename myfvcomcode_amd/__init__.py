@@ -90,6 +90,7 @@ def grid_lib():
         L.fv_model_set_tide.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_set_river.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.fv_model_set_sponge.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fv_rcm_order.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_union.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
         L.fv_extent_len.restype = C.c_int64
         L.fv_extent_len.argtypes = [C.c_void_p, C.c_int32]
@@ -123,6 +124,8 @@ def gpu_lib():
         L.fvgpu_set_forcing.argtypes = [vp, C.c_char_p, C.c_void_p, C.c_int64]
         L.fvgpu_ensemble_update.argtypes = [vp, C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                                             C.c_int64, C.c_void_p]
+        L.fvgpu_ensemble_update_noalloc.argtypes = [vp, C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                                    C.c_int64, C.c_int64, C.c_void_p]
         L.fvgpu_set_concurrency.argtypes = [vp, C.c_int32]
         L.fvgpu_autotune.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int64]
         fp = C.POINTER(C.c_float)
@@ -150,6 +153,33 @@ def _check(lib, rc, what):
     if rc != 0:
         msg = lib.fv_last_error()
         raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else "?"))
+
+
+def rcm_order(m, nv_rows):
+    """Reverse Cuthill-McKee numbering (fv_rcm_order) of a mesh of m nodes
+    and 1-based triangles nv_rows (N,3): (node_new, elem_new), the 1-based
+    new index of each old node and element."""
+    L = grid_lib()
+    nv = np.ascontiguousarray(nv_rows, np.int32)
+    node_new = np.zeros(m, np.int32)
+    elem_new = np.zeros(nv.shape[0], np.int32)
+    _check(L, L.fv_rcm_order(int(m), nv.shape[0], nv.ctypes.data, node_new.ctypes.data, elem_new.ctypes.data),
+           "fv_rcm_order")
+    return node_new, elem_new
+
+
+def renumber_mesh(x, y, nv_rows, h, obc_nodes, node_new, elem_new):
+    """The mesh in the numbering node_new / elem_new (1-based new index of
+    each old one): (x, y, nv_rows, h, obc_nodes) ready for
+    Model.from_mesh.  The open-boundary list keeps its order."""
+    m = len(x)
+    old_of_new = np.empty(m, np.int64)
+    old_of_new[node_new - 1] = np.arange(m)
+    nv = np.asarray(nv_rows)
+    nv_new = np.empty_like(nv)
+    nv_new[elem_new - 1] = node_new[nv - 1]
+    return (np.asarray(x)[old_of_new], np.asarray(y)[old_of_new], nv_new, np.asarray(h)[old_of_new],
+            node_new[np.asarray(obc_nodes, np.int64) - 1] if len(obc_nodes) else np.asarray(obc_nodes, np.int32))
 
 
 class Model:

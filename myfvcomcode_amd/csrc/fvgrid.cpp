@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <array>
 #include <type_traits>
 
 namespace {
@@ -626,6 +627,116 @@ extern "C" int fv_model_from_mesh(int32_t m, int32_t n, const double* x, const d
   rc = fv_model_cold_start(md);
   if (rc) { fv_model_free(md); return rc; }
   *out = md;
+  return 0;
+}
+
+namespace {
+
+// Breadth-first search of the node graph from `root` over the nodes whose
+// level is unset (-1): returns them in visiting order with their levels
+// set; the caller clears the levels again.
+std::vector<int> bfs_levels(int root, const std::vector<int>& off, const std::vector<int>& adj,
+                            std::vector<int>& level) {
+  std::vector<int> order;
+  order.push_back(root);
+  level[root] = 0;
+  for (size_t h = 0; h < order.size(); ++h) {
+    const int u = order[h];
+    for (int p = off[u]; p < off[u + 1]; ++p) {
+      const int v = adj[p];
+      if (level[v] < 0) {
+        level[v] = level[u] + 1;
+        order.push_back(v);
+      }
+    }
+  }
+  return order;
+}
+
+}  // namespace
+
+// Reverse Cuthill-McKee numbering of a triangle mesh's nodes (the node
+// graph of the triangles' edges), each connected component started from a
+// pseudo-peripheral node (George and Liu: repeated breadth-first searches
+// from a node of least degree in the last level until the depth stops
+// growing), neighbours visited by increasing degree, then the order
+// reversed.  Elements are numbered by their lowest, middle and highest new
+// node index.  Outputs are 1-based "new index of old index" maps:
+// node_new[old-1] and elem_new[old-1].
+extern "C" int fv_rcm_order(int32_t m, int32_t n, const int32_t* nv_rows, int32_t* node_new, int32_t* elem_new) {
+  if (m < 1 || n < 1 || !nv_rows || !node_new || !elem_new) return fail("fv_rcm_order: bad arguments");
+  for (int64_t q = 0; q < 3LL * n; ++q)
+    if (nv_rows[q] < 1 || nv_rows[q] > m) return fail("fv_rcm_order: node index out of range");
+  // node adjacency (CSR, 0-based), duplicates removed
+  std::vector<std::vector<int>> nb(m);
+  for (int e = 0; e < n; ++e)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        if (a != b) nb[nv_rows[3 * e + a] - 1].push_back(nv_rows[3 * e + b] - 1);
+  std::vector<int> off(m + 1, 0), adj, deg(m);
+  for (int u = 0; u < m; ++u) {
+    std::sort(nb[u].begin(), nb[u].end());
+    nb[u].erase(std::unique(nb[u].begin(), nb[u].end()), nb[u].end());
+    deg[u] = (int)nb[u].size();
+    off[u + 1] = off[u] + deg[u];
+  }
+  adj.reserve(off[m]);
+  for (int u = 0; u < m; ++u) {
+    std::sort(nb[u].begin(), nb[u].end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+    adj.insert(adj.end(), nb[u].begin(), nb[u].end());
+  }
+  std::vector<int> level(m, -1), order;
+  std::vector<char> done(m, 0);
+  order.reserve(m);
+  for (int seed = 0; seed < m; ++seed) {
+    if (done[seed]) continue;
+    // least-degree node of this component
+    std::vector<int> comp = bfs_levels(seed, off, adj, level);
+    int root = comp[0];
+    for (int u : comp) {
+      level[u] = -1;
+      if (deg[u] < deg[root] || (deg[u] == deg[root] && u < root)) root = u;
+    }
+    // pseudo-peripheral node: from the least-degree node of the last level
+    // while the number of levels grows
+    for (int it = 0; it < 16; ++it) {
+      std::vector<int> o = bfs_levels(root, off, adj, level);
+      const int d0 = level[o.back()];
+      int cand = -1;
+      for (int u : o)
+        if (level[u] == d0 && (cand < 0 || deg[u] < deg[cand] || (deg[u] == deg[cand] && u < cand))) cand = u;
+      for (int u : o) level[u] = -1;
+      std::vector<int> o2 = bfs_levels(cand, off, adj, level);
+      const int d1 = level[o2.back()];
+      for (int u : o2) level[u] = -1;
+      if (d1 <= d0) break;
+      root = cand;
+    }
+    // Cuthill-McKee from root (neighbours already sorted by degree)
+    const size_t start = order.size();
+    order.push_back(root);
+    done[root] = 1;
+    for (size_t h = start; h < order.size(); ++h) {
+      const int u = order[h];
+      for (int p = off[u]; p < off[u + 1]; ++p)
+        if (!done[adj[p]]) { done[adj[p]] = 1; order.push_back(adj[p]); }
+    }
+  }
+  if ((int)order.size() != m) return fail("fv_rcm_order: ordering incomplete");
+  std::reverse(order.begin(), order.end());
+  for (int k = 0; k < m; ++k) node_new[order[k]] = k + 1;
+  // elements by (lowest, middle, highest) new node index
+  std::vector<int> eo(n);
+  std::vector<std::array<int, 3>> key(n);
+  for (int e = 0; e < n; ++e) {
+    std::array<int, 3> t = {node_new[nv_rows[3 * e] - 1], node_new[nv_rows[3 * e + 1] - 1],
+                            node_new[nv_rows[3 * e + 2] - 1]};
+    std::sort(t.begin(), t.end());
+    key[e] = t;
+    eo[e] = e;
+  }
+  std::stable_sort(eo.begin(), eo.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (int k = 0; k < n; ++k) elem_new[eo[k]] = k + 1;
   return 0;
 }
 
