@@ -210,24 +210,66 @@ def time_variants(F, model, steps, warmup, fp64_step_s, dti, units):
     return out
 
 
-def cpu_baseline(steps, kb):
-    """Serial FP64 oracle on the 251x126-node sample of the same generator."""
-    import myfvcomcode_amd as F
-    from oracle import Oracle
-    m = F.Model.build(1, nx=251, ny=126, kb=kb)
-    o = Oracle(m)
-    o.internal_step()                 # first step outside the clock (page-in)
-    t0 = time.perf_counter()
-    o.run(steps)
-    wall = time.perf_counter() - t0
-    n, kbm1 = m.n, m.kb - 1
-    rate = n * kbm1 * steps / wall
-    dti = m.hdr.p.dte * m.hdr.p.isplit
-    sdph_1m = rate * dti / (24.0 * 1000000 * 40)
-    return {"value": sdph_1m, "unit": "sim-days/wall-hour (scaled to 1M x 40)", "cores": 1, "kind": "port",
-            "elem_layer_steps_per_s": rate,
-            "sample": "oracle (serial FP64 C restatement, -O3) on the same jittered-mesh generator at 251x126 nodes "
-                      "(%d elements x %d layers), %d internal steps, %.1f s" % (n, kbm1, steps, wall)}
+def _cpu_ranks(nranks, steps, nx, ny, kb):
+    """Start `nranks` oracle ranks (oracle/cpu_rank.py) as child processes,
+    release them together once all have built their sample, and return
+    their timings."""
+    import subprocess
+    script = os.path.join(ROOT, "oracle", "cpu_rank.py")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = [subprocess.Popen([sys.executable, script, str(nx), str(ny), str(kb), str(steps)],
+                              stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True, env=env)
+             for _ in range(nranks)]
+    for p in procs:
+        if p.stdout.readline().strip() != "ready":
+            raise RuntimeError("bench: a CPU baseline rank failed to start")
+    for p in procs:
+        p.stdin.write("go\n")
+        p.stdin.flush()
+    out = []
+    for p in procs:
+        line = p.stdout.readline()
+        if p.wait(timeout=600) != 0 or not line:
+            raise RuntimeError("bench: a CPU baseline rank failed")
+        out.append(json.loads(line))
+    return out
+
+
+def cpu_baseline(steps, kb, nx=251, ny=126):
+    """The FP64 oracle (the serial C restatement, built here with -O3
+    -march=native) on the 251x126-node sample of the same generator
+    (62,500 elements, 1/16 of the 1M-element mesh): one core, median of 3
+    runs; and one rank per host core (at most 16, the host share of one
+    GPU) each stepping its own sample at once, the stand-in for the
+    reference's MPI decomposition of the 1M-element mesh into 62,500-element
+    subdomains (no halo exchange, which favours the CPU).  Rates are scaled
+    to the metric's 1M elements x 40 layers."""
+    import oracle
+    import statistics
+    oracle.build(native=True)
+    single = [_cpu_ranks(1, steps, nx, ny, kb)[0] for _ in range(3)]
+    walls = sorted(r["wall_s"] for r in single)
+    r0 = single[0]
+    units = r0["elements"] * r0["layers"] * r0["steps"]
+    dti = r0["dti"]
+    rate1 = units / statistics.median(walls)
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    nranks = max(1, min(16, ncores))
+    many = _cpu_ranks(nranks, steps, nx, ny, kb)
+    rate_all = sum(units / r["wall_s"] for r in many)
+
+    def sdph(rate):
+        return rate * dti / (24.0 * 1000000 * 40)
+
+    return {"value": sdph(rate_all), "unit": "sim-days/wall-hour (scaled to 1M x 40)", "cores": nranks,
+            "kind": "port", "elem_layer_steps_per_s": rate_all,
+            "single_core": {"value": sdph(rate1), "elem_layer_steps_per_s": rate1, "cores": 1,
+                            "wall_s_runs": walls},
+            "sample": "oracle (serial FP64 C restatement, -O3 -march=native, built on this host) on the same "
+                      "jittered-mesh generator at %dx%d nodes (%d elements x %d layers), %d internal steps after "
+                      "one untimed step; one core: median of 3 runs; all cores: %d ranks each stepping its own "
+                      "sample at once (stand-in for the reference's MPI domain decomposition, halo exchange "
+                      "not included)" % (nx, ny, r0["elements"], r0["layers"], steps, nranks)}
 
 
 if __name__ == "__main__":

@@ -13,21 +13,25 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "libfvoracle.so")
-_lib = None
+_SO_NATIVE = os.path.join(_HERE, "_build", "libfvoracle_native.so")
+_libs = {}
 
 
-def build(verbose=False):
-    subprocess.run(["make", "-s", "-C", _HERE], check=True,
+def build(verbose=False, native=False):
+    """Build the oracle; native=True builds the copy tuned for this host
+    (-march=native) that the CPU baseline times."""
+    subprocess.run(["make", "-s", "-C", _HERE] + (["native"] if native else []), check=True,
                    stdout=None if verbose else subprocess.DEVNULL)
-    return _SO
+    return _SO_NATIVE if native else _SO
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_SO):
-            build()
-        L = C.CDLL(_SO)
+def lib(native=False):
+    L = _libs.get(native)
+    if L is None:
+        so = _SO_NATIVE if native else _SO
+        if not os.path.exists(so):
+            build(native=native)
+        L = C.CDLL(so)
         vp = C.c_void_p
         L.orc_create.restype = vp
         L.orc_create.argtypes = [vp]
@@ -42,28 +46,29 @@ def lib():
         L.orc_depth_fail.argtypes = [vp]
         L.orc_routine.argtypes = [vp, C.c_char_p, C.c_int]
         L.orc_budgets.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        _lib = L
-    return _lib
+        _libs[native] = L
+    return L
 
 
 class Oracle:
     """FP64 CPU state of one model, stepped by the oracle."""
 
-    def __init__(self, model):
+    def __init__(self, model, native=False):
         self.model = model  # keep the grid alive
-        ptr = lib().orc_create(model.ptr)
+        self._L = lib(native)
+        ptr = self._L.orc_create(model.ptr)
         if not ptr:
             raise ValueError("the oracle takes 2 <= KB <= 64 (KB = %d)" % model.kb)
         self._o = C.c_void_p(ptr)
 
     def internal_step(self):
-        lib().orc_internal_step(self._o)
+        self._L.orc_internal_step(self._o)
 
     def run(self, nsteps):
-        lib().orc_run(self._o, int(nsteps))
+        self._L.orc_run(self._o, int(nsteps))
 
     def external_step(self, iext=1):
-        lib().orc_external_only(self._o, int(iext))
+        self._L.orc_external_only(self._o, int(iext))
 
     def _len(self, name):
         m1, n1, kb = self.model.m + 1, self.model.n + 1, self.model.kb
@@ -90,13 +95,13 @@ class Oracle:
     def get(self, name):
         n = self._len(name)
         out = np.empty(n, np.float64)
-        if lib().orc_get(self._o, name.encode(), out.ctypes.data, n) != 0:
+        if self._L.orc_get(self._o, name.encode(), out.ctypes.data, n) != 0:
             raise KeyError(name)
         return out
 
     def put(self, name, values):
         v = np.ascontiguousarray(values, np.float64)
-        if lib().orc_put(self._o, name.encode(), v.ctypes.data, len(v)) != 0:
+        if self._L.orc_put(self._o, name.encode(), v.ctypes.data, len(v)) != 0:
             raise KeyError(name)
 
     def routine(self, name, arg=0):
@@ -108,25 +113,25 @@ class Oracle:
         fct_ts (FCT_T for arg 1, FCT_S for 2), fct_q (FCT_Q2 for arg 1,
         FCT_Q2L for 2), adjust_ts, bcond_gcn2 on UAF/VAF, bcond_gcn5 on
         U/V)."""
-        if lib().orc_routine(self._o, name.encode(), int(arg)) != 0:
+        if self._L.orc_routine(self._o, name.encode(), int(arg)) != 0:
             raise KeyError(name)
 
     @property
     def iint(self):
-        return lib().orc_iint(self._o)
+        return self._L.orc_iint(self._o)
 
     @property
     def depth_fail(self):
-        return bool(lib().orc_depth_fail(self._o))
+        return bool(self._L.orc_depth_fail(self._o))
 
     def budgets(self):
         v, s, h = C.c_double(), C.c_double(), C.c_double()
-        lib().orc_budgets(self._o, C.byref(v), C.byref(s), C.byref(h))
+        self._L.orc_budgets(self._o, C.byref(v), C.byref(s), C.byref(h))
         return v.value, s.value, h.value
 
     def close(self):
         if self._o is not None and self._o.value:
-            lib().orc_destroy(self._o)
+            self._L.orc_destroy(self._o)
             self._o = None
 
     def __del__(self):
