@@ -53,7 +53,9 @@ FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 
 
 def build_fortran(verbose=False):
-    """The Fortran driver (fvcom_gpu) over the ISO_C_BINDING module."""
+    """The Fortran drivers (fvcom_gpu: the seeded cases; fvcom_mesh: a
+    host's own mesh through FVGPU_SETUP_MESH) over the ISO_C_BINDING
+    module."""
     fdir = os.path.join(HERE, "fortran")
     mdir = os.path.join(LIB, "_fmod")
     os.makedirs(mdir, exist_ok=True)
@@ -63,6 +65,10 @@ def build_fortran(verbose=False):
     _run([FC, "-O2", "-J", mdir, os.path.join(fdir, "fvcom_driver.f90"), os.path.join(mdir, "fvgpu_mod.o"),
           "-L" + LIB, "-lfvgpu", "-lfvgrid", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"], verbose)
     os.replace(out + ".tmp", out)
+    out2 = os.path.join(LIB, "fvcom_mesh")
+    _run([FC, "-O2", "-J", mdir, os.path.join(fdir, "fvcom_mesh_driver.f90"), os.path.join(mdir, "fvgpu_mod.o"),
+          "-L" + LIB, "-lfvgpu", "-lfvgrid", "-Wl,-rpath,$ORIGIN", "-o", out2 + ".tmp"], verbose)
+    os.replace(out2 + ".tmp", out2)
     return out
 
 

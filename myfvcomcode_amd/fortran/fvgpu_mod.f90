@@ -244,13 +244,17 @@ contains
   ! The host model's own mesh (after GRID_METRICS/STARTUP, fvcom.F:629-753):
   ! nodes VX/VY/H (1:M), element triples NV(1:N,1:3) clockwise, the OBC node
   ! list; cold start on the device model, then upload.  precision: 0 FP64,
-  ! 1 mixed.  Forcing (fv_model_set_tide/_river/_sponge) goes between
-  ! fv_model_from_mesh and fvgpu_create, so this variant sets none.
-  subroutine fvgpu_setup_mesh(m, n, vx, vy, nv, h, kb, obc_nodes, p, precision)
+  ! 1 mixed.  Optionally a spectral tide at the OBC nodes (periods in s,
+  ! amplitudes in m, phases in degrees, the same at every OBC node;
+  ! BCOND_ASL, mod_obcs.F:552-582), set before the cold start.
+  subroutine fvgpu_setup_mesh(m, n, vx, vy, nv, h, kb, obc_nodes, p, precision, tide_period, tide_amp, &
+       tide_phase)
     integer, intent(in) :: m, n, kb, precision
     real(c_double), intent(in) :: vx(m), vy(m), h(m)
     integer, intent(in) :: nv(n, 3), obc_nodes(:)
     type(fv_params), intent(inout) :: p
+    real(c_double), intent(in), optional :: tide_period(:), tide_amp(:), tide_phase(:)
+    real(c_double), allocatable :: tp(:), ta(:), tf(:)
     integer(c_int32_t), allocatable :: rows(:), obc(:)
     real(c_double), allocatable :: xx(:), yy(:), hh(:)
     integer :: i
@@ -264,6 +268,12 @@ contains
     xx = vx; yy = vy; hh = h
     call fvgpu_check(fv_model_from_mesh(int(m, c_int32_t), int(n, c_int32_t), xx, yy, rows, hh, &
          int(kb, c_int32_t), int(size(obc_nodes), c_int32_t), obc, p, fvgpu_model), "fv_model_from_mesh")
+    if (present(tide_period) .and. present(tide_amp) .and. present(tide_phase)) then
+      allocate(tp(size(tide_period)), ta(size(tide_period)), tf(size(tide_period)))
+      tp = tide_period; ta = tide_amp; tf = tide_phase
+      call fvgpu_check(fv_model_set_tide(fvgpu_model, int(size(tp), c_int32_t), tp, ta, tf), &
+           "fv_model_set_tide")
+    end if
     call fvgpu_check(fv_model_cold_start(fvgpu_model), "fv_model_cold_start")
     call fvgpu_check(fvgpu_create(fvgpu_model, int(precision, c_int32_t), fvgpu_h), "fvgpu_create")
   end subroutine

@@ -84,3 +84,37 @@ def test_union_gpu_parity(built):
     for name in FIELDS:
         a, b = dev.get(name), orc.get(name)
         assert np.array_equal(a, b), (name, float(np.abs(a - b).max()))
+
+
+@pytest.mark.gpu
+def test_mixed_union_member_shape(built):
+    """The mixed-precision form on a union of ensemble-member instances at
+    the member's 20 layers (KB = 21) plus two smaller instances of the same
+    generator: every instance of the union advances bit for bit as its own
+    mixed-precision run, and stays within the mixed form's bounds of the
+    FP64 union."""
+    ps = [F.Model.build(3, nx=41, ny=21, kb=21, seed=21, n_seamounts=1),
+          F.Model.build(3, nx=41, ny=21, kb=21, seed=22, n_seamounts=1),
+          F.Model.build(3, nx=25, ny=13, kb=21, seed=23, n_seamounts=0),
+          F.Model.build(3, nx=17, ny=9, kb=21, seed=24, n_seamounts=0)]
+    u, offs = F.Model.union(ps)
+    n = 6
+    print("case: mixed union of 4 member-generator instances KB=21, %d steps (graph)" % n, flush=True)
+    dmx = F.DeviceModel(u, precision="mixed")
+    dmx.run(n, graph=True)
+    d64 = F.DeviceModel(u, precision="fp64")
+    d64.run(n, graph=True)
+    assert not dmx.diagnostics().nonfinite
+    for p, off in zip(ps, offs):
+        alone = F.DeviceModel(p, precision="mixed")
+        alone.run(n, graph=True)
+        for name in FIELDS:
+            a = instance_slice(dmx.get(name), u, off, u.kb)
+            b = own(alone.get(name), p, u.kb)
+            assert np.array_equal(a, b), (name, off)
+        del alone
+    for name, bound in (("el", 1e-6), ("ua", 1e-5), ("u", 1e-4), ("t1", 1e-4), ("s1", 1e-4)):
+        a, b = dmx.get(name), d64.get(name)
+        err = float(np.sqrt(np.mean((a - b) ** 2)))
+        print("  mixed vs fp64 %-3s rms %.3e" % (name, err), flush=True)
+        assert err <= bound, (name, err)
