@@ -142,8 +142,13 @@ enum fv_extent {
   X(double,  angleq,    FVX_RIV)  X(double,  qdis0,     FVX_RIV)            \
   X(double,  tdis,      FVX_RIV)  X(double,  sdis,      FVX_RIV)            \
   X(double,  vqdist,    FVX_RIVxK)                                          \
-  /* constant wind speed at elements (WIND_TYPE = 'speed') */               \
-  X(double,  uuwind,    FVX_N1)   X(double,  vvwind,    FVX_N1)
+  /* wind speed at elements (WIND_TYPE = 'speed'): UUWIND/VVWIND at       \
+   * IntTime of the coming internal step, UUWIND0/VVWIND0 at its start     \
+   * (IntTime - DTI); the external steps take the linear interpolation at  \
+   * ExtTime (UPDATE_WIND, bcond_gcn.F:878-895), the internal surface     \
+   * stress the IntTime value (bcond_gcn.F:661-678)                        */ \
+  X(double,  uuwind,    FVX_N1)   X(double,  vvwind,    FVX_N1)             \
+  X(double,  uuwind0,   FVX_N1)   X(double,  vvwind0,   FVX_N1)
 
 /* Prognostic / persistent state carried between internal steps. */
 #define FV_STATE_FIELDS(X)                                                  \
@@ -322,11 +327,16 @@ int  fvgpu_ensemble_update_noalloc(fvgpu_handle* h, const char* name, int32_t nr
  * next step.  The reference interpolates its forcing files in time inside
  * the step (UPDATE_RIVERS at IntTime, bcond_gcn.F:633-641; UPDATE_WIND and
  * ASIMPLE_DRAG at IntTime and at each ExtTime, bcond_gcn.F:661-665,
- * 878-882); here the caller supplies the values for the coming internal
- * step, held over its external steps:
- *   "wind"  UUWIND(0:N) then VVWIND(0:N), element wind speed (m/s); the
- *           stress follows PSIMPLE_DRAG (mod_bulk.F:58-90).  The model must
- *           have been created with wind on.
+ * 878-882); here the caller supplies the values at the coming step's
+ * IntTime:
+ *   "wind"  UUWIND(0:N) then VVWIND(0:N), element wind speed (m/s) at the
+ *           end of the coming internal step; the wind set before (or the
+ *           model's, at first) becomes the wind at its start.  The
+ *           internal surface stress uses the end value; each external step
+ *           uses the speed interpolated linearly in time to its ExtTime
+ *           (fraction IEXT/ISPLIT of the step), converted by PSIMPLE_DRAG
+ *           (mod_bulk.F:58-90).  The model must have been created with
+ *           wind on.
  *   "qdis"  river discharge per river (m^3/s, before RAMP)
  *   "tdis", "sdis"  river temperature / salinity per river               */
 int  fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int64_t len);

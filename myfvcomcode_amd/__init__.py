@@ -419,7 +419,10 @@ class DeviceModel:
 
     def set_forcing(self, name, values):
         """Replace a forcing from the next step on: "wind" (UUWIND then
-        VVWIND over elements 0..N), "qdis", "tdis" or "sdis" (per river)."""
+        VVWIND over elements 0..N, the wind at the end of the coming step;
+        the previous one becomes its start and each external step takes
+        the wind interpolated to its ExtTime), "qdis", "tdis" or "sdis"
+        (per river)."""
         L = gpu_lib()
         v = np.ascontiguousarray(values, np.float64)
         _check(L, L.fvgpu_set_forcing(self._h, name.encode(), v.ctypes.data, len(v)), "set_forcing " + name)

@@ -50,8 +50,33 @@ struct ExtArgs {
   double* elij;          // per-edge mean elevation of the stage input
   double* phi;           // per sub-edge EXTEL terms (read for the stage input,
                          // written by k_ext_momentum for the next stage)
+  const double *ws2x, *ws2y;   // wind stress of the external step (before RAMP*0.001)
   DevClock* clk;
 };
+
+// PSIMPLE_DRAG (mod_bulk.F:58-90): wind stress of a wind speed (Large and
+// Pond drag coefficient, 1.2 kg/m^3 air), on the host and the device.
+__host__ __device__ __forceinline__ void psimple_drag(double tx, double ty, double& sx, double& sy) {
+  const double wds = sqrt(tx * tx + ty * ty);
+  double cd = (double)1.2E-3f;
+  if (wds >= 11.0) cd = (0.49 + 0.065 * wds) * 1.E-3;
+  if (wds >= 25.0) cd = (0.49 + 0.065 * 25.0) * 1.E-3;
+  sx = 1.2 * cd * tx * wds;
+  sy = 1.2 * cd * ty * wds;
+}
+
+// BCOND_GCN(9) wind of EXTERNAL_STEP iext (bcond_gcn.F:878-895): the speed
+// interpolated linearly in time between the internal step's start (u0, v0)
+// and end (u1, v1) to ExtTime, fraction iext/ISPLIT, then its stress.
+__global__ void k_ext_wind(DevGrid g, const double* u0, const double* v0, const double* u1, const double* v1,
+                           int iext, double* sx, double* sy) {
+  const int i = (int)xcd_thread() + 1;
+  if (i > g.N) return;
+  const double frac = (double)iext / (double)g.isplit;
+  const double uu = u0[i] + frac * (u1[i] - u0[i]);
+  const double vv = v0[i] + frac * (v1[i] - v0[i]);
+  psimple_drag(uu, vv, sx[i], sy[i]);
+}
 
 // Wall projection of a velocity pair (BCOND_GCN cases 2/3/5).
 __device__ __forceinline__ void wall_project(const DevGrid& g, int i, double& u, double& v) {
@@ -297,8 +322,8 @@ __global__ void k_ext_momentum(DevGrid g, ExtArgs a) {
   // EXTUV_EDGE
   const double ua = a.cur.ua[i], va = a.cur.va[i], d1 = a.cur.d1[i], art = g.art[i], cor = g.cor[i];
   const double ramp = a.clk->ramp_ext;
-  const double wus2 = g.wind_on ? g.wsx[i] * ramp * 0.001 : 0.0;
-  const double wvs2 = g.wind_on ? g.wsy[i] * ramp * 0.001 : 0.0;
+  const double wus2 = g.wind_on ? a.ws2x[i] * ramp * 0.001 : 0.0;
+  const double wvs2 = g.wind_on ? a.ws2y[i] * ramp * 0.001 : 0.0;
   const double resx = a.adx2d[i] + xf + a.drx2d[i] + px - cor * va * d1 * art - (wus2 + a.wubot[i]) * art;
   const double resy = a.ady2d[i] + yf + a.dry2d[i] + py + cor * ua * d1 * art - (wvs2 + a.wvbot[i]) * art;
   const double h1 = g.h1[i], elrk1 = a.base.el1[i];

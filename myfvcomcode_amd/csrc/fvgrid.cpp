@@ -914,6 +914,8 @@ extern "C" int fv_model_build(const fv_case* c, fv_model** out) {
     for (int i = 1; i <= N; ++i) {
       md->uuwind[i] = c->wind_mean * std::cos(dir) + c->wind_std * fu(md->xc[i], md->yc[i]);
       md->vvwind[i] = c->wind_mean * std::sin(dir) + c->wind_std * fv(md->xc[i], md->yc[i]);
+      md->uuwind0[i] = md->uuwind[i];   // held constant: the same at the start of every step
+      md->vvwind0[i] = md->vvwind[i];
     }
   }
   // initial state

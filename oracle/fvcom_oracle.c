@@ -278,12 +278,18 @@ static void bcond_forcing_int(orc* o) {
   }
 }
 
-/* BCOND_GCN(9,0): external-mode surface forcing, bcond_gcn.F:851-905 */
+/* BCOND_GCN(9,0): external-mode surface forcing, bcond_gcn.F:851-905.  The
+ * wind speed at ExtTime is interpolated linearly in time between its values
+ * at the start (UUWIND0) and the end (UUWIND) of the internal step, as
+ * UPDATE_WIND interpolates its records (bcond_gcn.F:878-885). */
 static void bcond_forcing_ext(orc* o) {
   if (o->p.wind_on) {
+    const double frac = (double)o->iext / (double)o->p.isplit;
     for (int i = 1; i <= o->N; ++i) {
       double sx, sy;
-      simple_drag(G->uuwind[i], G->vvwind[i], &sx, &sy);
+      const double uu = G->uuwind0[i] + frac * (G->uuwind[i] - G->uuwind0[i]);
+      const double vv = G->vvwind0[i] + frac * (G->vvwind[i] - G->vvwind0[i]);
+      simple_drag(uu, vv, &sx, &sy);
       o->wusurf2[i] = sx * o->ramp * 0.001;
       o->wvsurf2[i] = sy * o->ramp * 0.001;
     }

@@ -138,7 +138,9 @@ def test_state_written_between_steps(built, graph):
 
 def test_forcing_changed_between_steps(built):
     """Wind speed changed between internal steps through fvgpu_set_forcing
-    advances exactly as the oracle reading the changed wind."""
+    advances exactly as the oracle reading the changed wind: the new value
+    is the wind at the end of the coming step, the previous one its start,
+    and each external step takes the wind interpolated to its ExtTime."""
     import myfvcomcode_amd as F
     from oracle import Oracle
     m = F.Model.build(1, nx=61, ny=31, kb=9, n_seamounts=2)
@@ -148,12 +150,16 @@ def test_forcing_changed_between_steps(built):
     dev.run(2, graph=True)
     orc.run(2)
     uu, vv = m.field("uuwind"), m.field("vvwind")
+    uu0, vv0 = m.field("uuwind0"), m.field("vvwind0")
     rng = np.random.default_rng(5)
-    uu[1:] = 1.5 * uu[1:] + rng.standard_normal(uu.size - 1)    # the oracle reads the model's arrays
-    vv[1:] = 0.5 * vv[1:] - 3.0
-    dev.set_forcing("wind", np.concatenate([uu, vv]))
-    dev.run(2, graph=True)
-    orc.run(2)
+    for step in range(3):
+        uu0[:] = uu                                                # the oracle reads the model's arrays
+        vv0[:] = vv
+        uu[1:] = 1.5 * uu[1:] + rng.standard_normal(uu.size - 1)
+        vv[1:] = 0.5 * vv[1:] - 3.0
+        dev.set_forcing("wind", np.concatenate([uu, vv]))
+        dev.run(1, graph=step > 0)
+        orc.run(1)
     rep = {}
     worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
     assert not bad, bad

@@ -526,3 +526,43 @@ def test_calculated_river_keeps_river_node_at_or_above_river_values(built):
     o.run(3)
     t1, s1 = o.get("t1").reshape(kb, m1), o.get("s1").reshape(kb, m1)
     assert np.all(t1[:-1, j] >= 25.0) and np.all(s1[:-1, j] >= 31.0)
+
+
+# ------------------------------------------------- wind at each ExtTime
+def _drag(u, v):
+    """Large and Pond drag law of PSIMPLE_DRAG (mod_bulk.F:58-90):
+    tau = rho_air Cd |w| w with Cd = 1.2e-3 below 11 m/s, (0.49 + 0.065|w|)
+    1e-3 up to 25 m/s, constant above; rho_air = 1.2."""
+    w = np.hypot(u, v)
+    cd = np.where(w >= 25.0, (0.49 + 0.065 * 25.0) * 1e-3,
+                  np.where(w >= 11.0, (0.49 + 0.065 * w) * 1e-3, float(np.float32(1.2e-3))))
+    return 1.2 * cd * u * w, 1.2 * cd * v * w
+
+
+def test_external_wind_interpolated_to_ext_time(built):
+    """With different winds at the start and end of the internal step, the
+    external step IEXT takes the wind at its ExtTime, the fraction
+    IEXT/ISPLIT of the way (UPDATE_WIND at ExtTime, bcond_gcn.F:878-895),
+    and the internal surface stress the wind at the end (IntTime)."""
+    m = F.Model.build(1, nx=31, ny=17, kb=5, n_seamounts=0)
+    assert m.hdr.p.wind_on
+    n1 = m.n + 1
+    uu, vv, uu0, vv0 = (m.field(x) for x in ("uuwind", "vvwind", "uuwind0", "vvwind0"))
+    rng = np.random.default_rng(2)
+    uu0[1:] = 5.0 + 3 * rng.standard_normal(n1 - 1)
+    vv0[1:] = -4.0 + 3 * rng.standard_normal(n1 - 1)
+    uu[1:] = 12.0 + 8 * rng.standard_normal(n1 - 1)     # across the 11 and 25 m/s branches
+    vv[1:] = 9.0 + 8 * rng.standard_normal(n1 - 1)
+    o = Oracle(m)
+    isplit = m.hdr.p.isplit
+    for iext in (1, 4, isplit):
+        o.external_step(iext)
+        ramp = o.get("ramp")[0]
+        f = iext / isplit
+        sx, sy = _drag(uu0[1:] + f * (uu[1:] - uu0[1:]), vv0[1:] + f * (vv[1:] - vv0[1:]))
+        np.testing.assert_allclose(o.get("wusurf2")[1:], sx * ramp * 0.001, rtol=1e-13, atol=1e-18)
+        np.testing.assert_allclose(o.get("wvsurf2")[1:], sy * ramp * 0.001, rtol=1e-13, atol=1e-18)
+    o.internal_step()
+    sx, sy = _drag(uu[1:], vv[1:])
+    ramp = o.get("ramp")[0]
+    np.testing.assert_allclose(o.get("wusurf")[1:], -sx * ramp * 0.001, rtol=1e-13, atol=1e-18)
