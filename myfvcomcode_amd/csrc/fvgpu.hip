@@ -21,8 +21,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 #include <algorithm>
@@ -42,6 +44,40 @@ struct KernelStat { double total_ms = 0; int64_t count = 0; };
 
 }  // namespace
 
+// Staging of fvgpu_ensemble_update_noalloc: X5 (FV_ENS_MAX x FV_ENS_MAX)
+// and the member offsets (FV_ENS_MAX), 33,280 bytes held on every GPU the
+// library runs on until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: host code uses only the device
+// address of the current GPU's copy, looked up once per device by
+// fvgpu_create (scratch_of).  Every call on one GPU shares its copy, so
+// overlapping calls on one GPU (from different handles' streams or host
+// threads) race.
+static __device__ __align__(256) double scratch[FV_ENS_MAX * FV_ENS_MAX + FV_ENS_MAX];
+#define FV_SCRATCH_DEVICES 64
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[FV_SCRATCH_DEVICES] = {};
+
+// Device address of the current GPU's `scratch` (looked up on first use
+// for that device, which must be current).  Null with the error set on
+// failure.
+double* scratch_of(int dev) {
+  if (dev < 0 || dev >= FV_SCRATCH_DEVICES) { fail("device index beyond the staging table"); return nullptr; }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 struct fvgpu_handle {
   const fv_model* md = nullptr;
   int precision = 0;
@@ -54,6 +90,13 @@ struct fvgpu_handle {
   hipStream_t stream2 = nullptr, cur = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int concurrent = 3;     // bit 0: 3-D chain pairs, bit 1: the 2-D stage pair on stream2
+  int two_phase_uv = 0;   // ADVECTION/ADV_UV as edge kernel + element gather (1) or element-centric (0)
+  int vertvl_column = 0;  // VERTVL one thread per node column (1) or FV_VV_CH lanes per column (0)
+  // wind speeds at the start (wu0, wv0) and end (wu1, wv1) of the coming
+  // internal step; once they differ, every external step forms its own
+  // stress (ws2x, ws2y), otherwise it uses the end stress g.wsx/g.wsy
+  double *wu0 = nullptr, *wv0 = nullptr, *wu1 = nullptr, *wv1 = nullptr, *ws2x = nullptr, *ws2y = nullptr;
+  int wind_varies = 0;
   DevGrid g{};
   std::vector<void*> allocs;
   // device state
@@ -162,6 +205,12 @@ int flush_profile(fvgpu_handle* h) {
     Launch l_(h, NAME);                                                           \
     hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), 0, h->cur, __VA_ARGS__);    \
   } while (0)
+// The same with SHMEM bytes of dynamic LDS per workgroup.
+#define LAUNCH_SH(NAME, KERNEL, GRID, BLOCK, SHMEM, ...)                                  \
+  do {                                                                                    \
+    Launch l_(h, NAME);                                                                   \
+    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), (SHMEM), h->cur, __VA_ARGS__);     \
+  } while (0)
 
 inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -222,6 +271,8 @@ ExtArgs make_ext_args(fvgpu_handle* h, int K, const Ext2DState& cur, const Ext2D
   a.ef = h->ia.ef2;
   a.elij = h->ia.elij2;
   a.phi = h->phi;
+  a.ws2x = h->wind_varies ? h->ws2x : h->g.wsx;
+  a.ws2y = h->wind_varies ? h->ws2y : h->g.wsy;
   a.clk = h->clk;
   return a;
 }
@@ -235,6 +286,8 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext, bool fresh) {
   const DevGrid& g = h->g;
   LAUNCH("ext_clock", k_ext_clock, 1, 64, h->clk, iext);
   if (fresh) TLAUNCH("ext_phi", k_ext_phi, g.N, g, h->ext[0].ua, h->ext[0].va, h->ext[0].d1, h->phi);
+  if (h->wind_varies)
+    TLAUNCH("ext_wind", k_ext_wind, g.N, g, h->wu0, h->wv0, h->wu1, h->wv1, iext, h->ws2x, h->ws2y);
   // stage buffers: 1: A->B, 2: B->C, 3: C->B, 4: B->A (in place on the RK base)
   const Ext2DState* cur[5] = {nullptr, &h->ext[0], &h->ext[1], &h->ext[2], &h->ext[1]};
   const Ext2DState* nxt[5] = {nullptr, &h->ext[1], &h->ext[2], &h->ext[1], &h->ext[0]};
@@ -300,8 +353,12 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
   TLAUNCH("adjust1_brough", k_adjust1_brough, g.N + 1, g, a);
-  TLAUNCH("uvedge_adv", (k_uvedge<0, FT>), g.NE + 1, g, a, a.u, a.v);
-  TLAUNCH("adv3d", k_adv3d_gather<FT>, g.N + 1, g, a);
+  if (h->two_phase_uv) {
+    TLAUNCH("uvedge_adv", (k_uvedge<0, FT>), g.NE + 1, g, a, a.u, a.v);
+    TLAUNCH("adv3d", k_adv3d_gather<FT>, g.N + 1, g, a);
+  } else {
+    TLAUNCH("adv3d_elem", (k_uv_elem<0, FT>), g.N + 1, g, a);
+  }
   TLAUNCH("baropg", k_baropg, g.N + 1, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, g.M + 1, g, a);
   TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
@@ -310,13 +367,23 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext, iext == 1);
   TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
-#define VERTVL_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
-  FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
+  // VERTVL: FV_VV_CH lanes per node column, 128-lane workgroups (32 nodes,
+  // at most 32 x 63 x 16 B = 31.5 KB of LDS for the layer terms)
+#define VERTVL_CALL(SM)                                                                                \
+  LAUNCH_SH("vertvl", k_vertvl_c<SM>, (unsigned)((g.M + 1 + 128 / FV_VV_CH - 1) / (128 / FV_VV_CH)), 128, \
+            (size_t)(128 / FV_VV_CH) * g.KBM1 * 2 * sizeof(double), g, a)
+#define VERTVL_R_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
+  if (h->vertvl_column) FV_SMAX_DISPATCH(g.cv_max, VERTVL_R_CALL);
+  else FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
   TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
-  TLAUNCH("uvedge_uv", (k_uvedge<1, FT>), g.NE + 1, g, a, a.u, a.v);
-  TLAUNCH("advuv", k_advuv_gather<FT>, g.N + 1, g, a);
+  if (h->two_phase_uv) {
+    TLAUNCH("uvedge_uv", (k_uvedge<1, FT>), g.NE + 1, g, a, a.u, a.v);
+    TLAUNCH("advuv", k_advuv_gather<FT>, g.N + 1, g, a);
+  } else {
+    TLAUNCH("advuv_elem", (k_uv_elem<1, FT>), g.N + 1, g, a);
+  }
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   TLAUNCH("uv_finish", k_uv_finish, (long)g.N * g.KBM1, g, a);
@@ -395,15 +462,7 @@ void enqueue_internal_step(fvgpu_handle* h) {
 // Wind stress of element wind speeds (PSIMPLE_DRAG, mod_bulk.F:58-90),
 // elements 1..n1-1.
 void wind_stress(const double* uu, const double* vv, size_t n1, double* wx, double* wy) {
-  for (size_t i = 1; i < n1; ++i) {
-    const double tx = uu[i], ty = vv[i];
-    const double wds = std::sqrt(tx * tx + ty * ty);
-    double cd = (double)1.2E-3f;
-    if (wds >= 11.0) cd = (0.49 + 0.065 * wds) * 1.E-3;
-    if (wds >= 25.0) cd = (0.49 + 0.065 * 25.0) * 1.E-3;
-    wx[i] = 1.2 * cd * tx * wds;
-    wy[i] = 1.2 * cd * ty * wds;
-  }
+  for (size_t i = 1; i < n1; ++i) psimple_drag(uu[i], vv[i], wx[i], wy[i]);
 }
 
 // ------------------------------------------------------ derived structures
@@ -593,6 +652,18 @@ int upload_grid(fvgpu_handle* h) {
   if (p.wind_on) wind_stress(md->uuwind, md->vvwind, n1, wx.data(), wy.data());
   if (dupload(h, &g.wsx, wx.data(), n1)) return 1;
   if (dupload(h, &g.wsy, wy.data(), n1)) return 1;
+  if (p.wind_on) {
+    const double* w0s[4] = {md->uuwind0, md->vvwind0, md->uuwind, md->vvwind};
+    double** dst[4] = {&h->wu0, &h->wv0, &h->wu1, &h->wv1};
+    for (int q = 0; q < 4; ++q) {
+      const double* src = nullptr;
+      if (dupload(h, &src, w0s[q], n1)) return 1;
+      *dst[q] = (double*)src;
+    }
+    if (dalloc(h, &h->ws2x, n1) || dalloc(h, &h->ws2y, n1)) return 1;
+    h->wind_varies = std::memcmp(md->uuwind0, md->uuwind, n1 * sizeof(double)) != 0 ||
+                     std::memcmp(md->vvwind0, md->vvwind, n1 * sizeof(double)) != 0;
+  }
   // sigma: one column when every column is identical
   const bool un = columns_uniform(md->z, m1, KB, 0) && columns_uniform(md->dz, m1, KB, 0) &&
                   columns_uniform(md->dzz, m1, KB, 0) && columns_uniform(md->zz, m1, KB, 0);
@@ -758,6 +829,9 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   h->md = md;
   h->precision = precision;
   HIPCHK(hipGetDevice(&h->device));
+  if (!scratch_of(h->device)) { delete h; return 1; }
+  if (const char* v = std::getenv("FVGPU_TWO_PHASE_UV")) h->two_phase_uv = std::atoi(v) != 0;
+  if (const char* v = std::getenv("FVGPU_VERTVL_COLUMN")) h->vertvl_column = std::atoi(v) != 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
   h->cur = h->stream;
   if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
@@ -835,8 +909,12 @@ int fvgpu_put_state(fvgpu_handle* h, const char* name, const double* host, int64
   return 0;
 }
 
-int fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
-                          const int64_t* elem_off, int64_t m, int64_t n, const double* x5) {
+namespace {
+// fvgpu_ensemble_update with its staging (X5, then the member offsets) in
+// `stage`, or, when `stage` is null, in a buffer the handle allocates on
+// first use.
+int ens_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off, const int64_t* elem_off,
+               int64_t m, int64_t n, const double* x5, double* stage) {
   const DevGrid& g = h->g;
   if (nrens < 2 || nrens > FV_ENS_MAX) return fail("ensemble size must be 2..64");
   size_t l = 0;
@@ -856,14 +934,17 @@ int fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, cons
   if (len < 1) return fail("ensemble update: empty member slice");
   for (int i = 0; i < nrens; ++i)
     if (off[i] < 0 || (size_t)(off[i] + len + 1) > ls) return fail("ensemble update: member slice outside the array");
-  if (!h->ens_buf) {
-    void* q = nullptr;
-    HIPCHK(hipMalloc(&q, (FV_ENS_MAX * FV_ENS_MAX + FV_ENS_MAX) * sizeof(double)));
-    h->allocs.push_back(q);
-    h->ens_buf = (double*)q;
+  if (!stage) {
+    if (!h->ens_buf) {
+      void* q = nullptr;
+      HIPCHK(hipMalloc(&q, (FV_ENS_MAX * FV_ENS_MAX + FV_ENS_MAX) * sizeof(double)));
+      h->allocs.push_back(q);
+      h->ens_buf = (double*)q;
+    }
+    stage = h->ens_buf;
   }
-  double* dx5 = h->ens_buf;
-  long* doff = (long*)(h->ens_buf + FV_ENS_MAX * FV_ENS_MAX);
+  double* dx5 = stage;
+  long* doff = (long*)(stage + FV_ENS_MAX * FV_ENS_MAX);
   std::vector<long> offl(off, off + nrens);
   HIPCHK(hipMemcpyAsync(dx5, x5, (size_t)nrens * nrens * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(doff, offl.data(), (size_t)nrens * sizeof(long), hipMemcpyHostToDevice, h->stream));
@@ -874,6 +955,24 @@ int fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, cons
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
+}
+}  // namespace
+
+int fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                          const int64_t* elem_off, int64_t m, int64_t n, const double* x5) {
+  return ens_update(h, name, nrens, node_off, elem_off, m, n, x5, nullptr);
+}
+
+int fvgpu_ensemble_update_noalloc(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                                  const int64_t* elem_off, int64_t m, int64_t n, const double* x5) {
+  if (nrens > FV_ENS_MAX) return fail("ensemble size beyond the fixed staging of 64 members");
+  double* stage = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (h->device >= 0 && h->device < FV_SCRATCH_DEVICES) stage = g_scratch_ptr[h->device];
+  }
+  if (!stage) return fail("no staging address for this device (the handle was not created on it)");
+  return ens_update(h, name, nrens, node_off, elem_off, m, n, x5, stage);
 }
 
 int fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int64_t len) {
@@ -887,6 +986,23 @@ int fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int
     wind_stress(host, host + n1, n1, wx.data(), wy.data());
     HIPCHK(hipMemcpyAsync((void*)g.wsx, wx.data(), n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync((void*)g.wsy, wy.data(), n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    // the previous end speeds become the start of the coming step
+    std::vector<double> old_u(n1), old_v(n1);
+    HIPCHK(hipMemcpyAsync(old_u.data(), h->wu1, n1 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(old_v.data(), h->wv1, n1 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpyAsync(h->wu0, old_u.data(), n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->wv0, old_v.data(), n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->wu1, host, n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->wv1, host + n1, n1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const bool varies = std::memcmp(old_u.data(), host, n1 * sizeof(double)) != 0 ||
+                        std::memcmp(old_v.data(), host + n1, n1 * sizeof(double)) != 0;
+    if (varies && !h->wind_varies) {
+      // the captured step gains the per-external-step stress kernel
+      h->wind_varies = 1;
+      if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+      if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+    }
   } else if (n == "qdis" || n == "tdis" || n == "sdis") {
     if ((size_t)len != nr) return fail(n + ": expected one value per river (NUMQBC)");
     const double* dst = n == "qdis" ? g.qdis0 : n == "tdis" ? g.tdis : g.sdis;
@@ -1299,7 +1415,8 @@ int fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode) {
 
 int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
   static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
-                               "cvflux_q", "adv_ts", "adv_q", "uvedge_adv", "uvedge_uv", "adv3d", "advuv", "vertvl",
+                               "cvflux_q", "adv_ts", "adv_q", "uvedge_adv", "uvedge_uv", "adv3d", "advuv",
+                               "adv3d_elem", "advuv_elem", "vertvl",
                                "viscofh", "vdif_q", "vdif_ts", "n2e3d", "adjust1_brough", "adjust2", "baropg",
                                "wreal", "uv_finish", "copy", "closure_features", "closure_apply"};
   static const int cands[] = {64, 128, 256};

@@ -560,6 +560,112 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
   a.dtfa[i] = dtfa;
 }
 
+// VERTVL_EDGE with each node's column split over FV_VV_CH lanes: lane c of
+// a node forms the layer terms of its chunk of layers (the CV divergence
+// over the node's sub-edges in sub-edge order, as in k_vertvl_r) and parks
+// them in LDS; then every lane of the node runs the column integration
+// w(k+1) = (w(k) + XFLUX/ART1) + DZ*(D-DT)/DTI over all layers in order
+// from LDS (the same operations in the same order as one thread would, so
+// the same bits), and writes WTS of its own chunk with the linear mass
+// correction applied, and lane 0 writes DTFA.  Four lanes per node carry
+// four times the loads in flight of one thread per column.
+#define FV_VV_CH 4
+template <int SMAX>
+__global__ __launch_bounds__(256) void k_vertvl_c(DevGrid g, IntArgs a) {
+  extern __shared__ double vv_sh[];      // [nodes of the block][KBM1][2]: XFLUX/ART1, DZ*(D-DT)/DTI
+  const int npb = (int)blockDim.x / FV_VV_CH;
+  const int nl = threadIdx.x / FV_VV_CH, c = threadIdx.x % FV_VV_CH;
+  const int i = xcd_block() * npb + nl;
+  const int KBM1 = g.KBM1;
+  const int L = (KBM1 + FV_VV_CH - 1) / FV_VV_CH;
+  const int k0 = c * L + 1, k1 = min(k0 + L - 1, KBM1);
+  double* sh = vv_sh + (long)nl * KBM1 * 2;
+  const bool act = i >= 1 && i <= g.M;
+  if (i == 0 && c == 0) {
+    for (int k = 1; k <= g.KB; ++k) a.wts[NI(g, 0, k)] = 0.0;
+    a.dtfa[0] = a.d[0];
+  }
+  bool obc = false;
+  double art1 = 1.0, dd = 0.0;
+  if (act) {
+    const int n = g.cv_cnt[i];
+    int ce[SMAX];
+    double dte[SMAX], sdx[SMAX], sdy[SMAX];
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const long q = (long)j * g.m1 + i;
+      ce[j] = j < n ? g.cvl_elem[q] : 0;
+      dte[j] = j < n ? a.dt1[ce[j]] : 0.0;
+      sdx[j] = j < n ? g.cvl_sdx[q] : 0.0;
+      sdy[j] = j < n ? g.cvl_sdy[q] : 0.0;
+    });
+    obc = g.isonb[i] == 2;
+    const int rv = g.node_riv[i];
+    art1 = g.art1[i];
+    dd = a.d[i] - a.dt[i];
+    for (int k = k0; k <= k1; ++k) {
+      const long ko = (long)(k - 1) * g.n1;
+      double x = 0.0;
+      constexpr int CHS = 8;
+      static_for<(SMAX + CHS - 1) / CHS>([&](auto cc) {
+        constexpr int c0 = decltype(cc)::value * CHS;
+        if (c0 < n) {
+          double uu[CHS], vv[CHS], dz[CHS];
+          static_for<CHS>([&](auto jc) {
+            constexpr int j = c0 + decltype(jc)::value;
+            if constexpr (j < SMAX) {
+              uu[j - c0] = a.u[ko + ce[j]];
+              vv[j - c0] = a.v[ko + ce[j]];
+              dz[j - c0] = DZ1(ce[j], k);
+            }
+          });
+          static_for<CHS>([&](auto jc) {
+            constexpr int j = c0 + decltype(jc)::value;
+            if constexpr (j < SMAX) {
+              if (j < n) {
+                const double dij = dte[j] * dz[j - c0];
+                x = x + dij * (-uu[j - c0] * sdy[j] + vv[j - c0] * sdx[j]);
+              }
+            }
+          });
+        }
+      });
+      double xflux = obc ? 0.0 : x;
+      if (rv >= 0)
+        for (int j = rv; j < g.NUMQBC; ++j)
+          if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+      sh[2 * (k - 1)] = xflux / art1;
+      sh[2 * (k - 1) + 1] = DZ(i, k) * dd / g.dti;
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  double w = 0.0;
+  for (int k = 1; k <= KBM1; ++k) w = (w + sh[2 * (k - 1)]) + sh[2 * (k - 1) + 1];
+  const double wbottom = 0.0;
+  const bool corr = fabs(w - wbottom) > 1.0E-8 && !obc;
+  const double wc = w - wbottom;
+  double wk = 0.0;
+  for (int k = 1; k <= k1; ++k) {
+    wk = (wk + sh[2 * (k - 1)]) + sh[2 * (k - 1) + 1];
+    if (k >= k0) {
+      double v = wk;
+      if (corr) v = v - (double)((float)k / (float)KBM1) * wc;   // WTS(k+1): (k+1-1)/KBM1
+      a.wts[NI(g, i, k + 1)] = v;
+    }
+  }
+  if (c == 0) {
+    a.wts[NI(g, i, 1)] = 0.0;
+    double dtfa = a.d[i];
+    if (corr) {
+      double tmp1 = a.elf[i] * (double)KBM1 - wc * g.dti / DZ(i, 1);
+      tmp1 = tmp1 / (double)KBM1;
+      dtfa = tmp1 + g.h[i];
+    }
+    a.dtfa[i] = dtfa;
+  }
+}
+
 // VISCOF_H (viscofh.F:33-127): nodal strain-rate magnitude from the NBVE
 // element ring, one thread per node with the ring (RMAX >= NTVE) held in
 // registers across the layers.  The first and last ring elements carry the

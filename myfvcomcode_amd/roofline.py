@@ -152,6 +152,34 @@ def advuv(s, real=R8):
     return reads + writes
 
 
+def _uv_elem(s, real, uv):
+    """Element-centric ADVECTION / ADV_UV: the element's three edges
+    evaluated by the element (no edge-layer buffer)."""
+    M, N, NE, KB, KBM1 = s["M"], s["N"], s["NE"], s["KB"], s["KBM1"]
+    reads = (NE * (5 * I4 + 4 * real)                  # IEC, IENODE, ISBC; XIJC, YIJC, DLTXC, DLTYC
+             + N * (3 * I4 + 12 * real)                # NBE; A1U, A2U, XC, YC, ART, CC_HVC
+             + N * (4 * I4 + 2 * real)                 # edge list, IUCP/ISBCE, river offset; EPOR, ...
+             + M * real                                # DT
+             + N * KBM1 * 2 * real)                    # U, V
+    if uv:
+        reads += (N * 5 * real + M * real               # DT1, D1, COR, GRAV_E, ART; EGF
+                  + N * KBM1 * 2 * real + N * KB * real)  # DRHOX, DRHOY; W
+        writes = N * KBM1 * 3 * real + N * real        # UF, VF, VISCOFM
+    else:
+        writes = 2 * N * real                          # ADX2D, ADY2D
+    return reads + writes
+
+
+def adv3d_elem(s, real=R8):
+    """ADVECTION_EDGE_GCN per element column, summed into ADX2D/ADY2D."""
+    return _uv_elem(s, real, False)
+
+
+def advuv_elem(s, real=R8):
+    """ADV_UV_EDGE_GCN per element column -> UF, VF, VISCOFM."""
+    return _uv_elem(s, real, True)
+
+
 def vertvl(s, real=R8):
     """VERTVL_EDGE: sub-edge divergence, column integral, correction."""
     M, N, NCV, KB, KBM1 = s["M"], s["N"], s["NCV"], s["KB"], s["KBM1"]
@@ -200,7 +228,8 @@ def n2e3d(s, real=R8):
 
 FAMILIES = {f.__name__: f for f in (
     ext_elevation, advave_edge, ext_momentum, cvrec_ts, cvrec_q, cvflux_ts, cvflux_q, adv_ts, adv_q,
-    uvedge_adv, uvedge_uv, adv3d, advuv, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts, n2e3d)}
+    uvedge_adv, uvedge_uv, adv3d, advuv, adv3d_elem, advuv_elem, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts,
+    n2e3d)}
 
 
 def bytes_per_launch(name, model, real=R8):
