@@ -313,6 +313,16 @@ int  fvgpu_diagnostics(fvgpu_handle* h, fv_diag* out);
  * increasing order of A(p,i)*X5(i,j).                                    */
 int  fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
                            const int64_t* elem_off, int64_t m, int64_t n, const double* x5);
+/* fvgpu_ensemble_update without any device allocation: X5 and the member
+ * offsets are staged in a fixed 33,280-byte device array of the library
+ * (one per GPU, looked up by fvgpu_create) instead of a buffer the handle
+ * allocates on its first update; the same kernel and results.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  nrens <= 64, as for fvgpu_ensemble_update.  Every call on
+ * one GPU shares the staging array: calls that overlap on one GPU race.   */
+int  fvgpu_ensemble_update_noalloc(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                                   const int64_t* elem_off, int64_t m, int64_t n, const double* x5);
 /* Time-varying forcing between internal steps, taking effect from the
  * next step.  The reference interpolates its forcing files in time inside
  * the step (UPDATE_RIVERS at IntTime, bcond_gcn.F:633-641; UPDATE_WIND and

@@ -124,6 +124,8 @@ def gpu_lib():
         L.fvgpu_set_forcing.argtypes = [vp, C.c_char_p, C.c_void_p, C.c_int64]
         L.fvgpu_ensemble_update.argtypes = [vp, C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                                             C.c_int64, C.c_void_p]
+        L.fvgpu_ensemble_update_noalloc.argtypes = [vp, C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                                    C.c_int64, C.c_int64, C.c_void_p]
         L.fvgpu_set_concurrency.argtypes = [vp, C.c_int32]
         L.fvgpu_autotune.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int64]
         fp = C.POINTER(C.c_float)

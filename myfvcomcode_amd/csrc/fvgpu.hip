@@ -44,6 +44,41 @@ struct KernelStat { double total_ms = 0; int64_t count = 0; };
 
 }  // namespace
 
+// Staging of fvgpu_ensemble_update_noalloc: X5 (FV_ENS_MAX x FV_ENS_MAX)
+// and the member offsets (FV_ENS_MAX), 33,280 bytes held on every GPU the
+// library runs on until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: host code uses only the device
+// address of the current GPU's copy, looked up once per device by
+// fvgpu_create (scratch_of).  Every call on one GPU shares its copy, so
+// overlapping calls on one GPU (from different handles' streams or host
+// threads) race.
+static __device__ __align__(256) double scratch[FV_ENS_MAX * FV_ENS_MAX + FV_ENS_MAX];
+#define FV_SCRATCH_DEVICES 64
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[FV_SCRATCH_DEVICES] = {};
+
+// Device address of the current GPU's `scratch` (looked up on first use
+// for that device, which must be current).  Null with the error set on
+// failure.
+double* scratch_of(int dev) {
+  if (dev < 0 || dev >= FV_SCRATCH_DEVICES) { fail("device index beyond the staging table"); return nullptr; }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+
 struct fvgpu_handle {
   const fv_model* md = nullptr;
   int precision = 0;
@@ -795,6 +830,7 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   h->md = md;
   h->precision = precision;
   HIPCHK(hipGetDevice(&h->device));
+  if (!scratch_of(h->device)) { delete h; return 1; }
   if (const char* v = std::getenv("FVGPU_TWO_PHASE_UV")) h->two_phase_uv = std::atoi(v) != 0;
   if (const char* v = std::getenv("FVGPU_VERTVL_COLUMN")) h->vertvl_column = std::atoi(v) != 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
@@ -927,6 +963,19 @@ int fvgpu_ensemble_update(fvgpu_handle* h, const char* name, int32_t nrens, cons
                           const int64_t* elem_off, int64_t m, int64_t n, const double* x5) {
   return ens_update(h, name, nrens, node_off, elem_off, m, n, x5, nullptr);
 }
+
+int fvgpu_ensemble_update_noalloc(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* node_off,
+                                  const int64_t* elem_off, int64_t m, int64_t n, const double* x5) {
+  if (nrens > FV_ENS_MAX) return fail("ensemble size beyond the fixed staging of 64 members");
+  double* stage = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (h->device >= 0 && h->device < FV_SCRATCH_DEVICES) stage = g_scratch_ptr[h->device];
+  }
+  if (!stage) return fail("no staging address for this device (the handle was not created on it)");
+  return ens_update(h, name, nrens, node_off, elem_off, m, n, x5, stage);
+}
+
 
 int fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int64_t len) {
   const DevGrid& g = h->g;

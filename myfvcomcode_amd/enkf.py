@@ -86,12 +86,15 @@ class Ensemble:
         k = 0 if layer is None else layer - 1
         return k * ls + base + local
 
-    def update(self, dev, name, X5):
+    def update(self, dev, name, X5, noalloc=False):
+        """A <- A*X5 for array `name` over every member, in place on the
+        device (fvgpu_ensemble_update, or fvgpu_ensemble_update_noalloc,
+        which stages X5 in the library's fixed device array)."""
         L = gpu_lib()
         x5 = np.asfortranarray(X5, np.float64)
-        _check(L, L.fvgpu_ensemble_update(dev._h, name.encode(), self.nens, self.node_off.ctypes.data,
-                                          self.elem_off.ctypes.data, self.m, self.n,
-                                          x5.ravel(order="F").ctypes.data), "ensemble update " + name)
+        fn = L.fvgpu_ensemble_update_noalloc if noalloc else L.fvgpu_ensemble_update
+        _check(L, fn(dev._h, name.encode(), self.nens, self.node_off.ctypes.data, self.elem_off.ctypes.data,
+                     self.m, self.n, x5.ravel(order="F").ctypes.data), "ensemble update " + name)
 
 
 def assimilate(dev, ens, obs, variables, truncation=0.99, seed=0):
