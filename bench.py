@@ -6,11 +6,14 @@ tide on the east boundary, constant seeded wind field, MY2.5 closure,
 DTE = 2 s, ISPLIT = 10 (DTI = 20 s), synthetic seeded data.
 
 A step is one INTERNAL_STEP (ISPLIT external steps included).  An untimed
-survey pass with HIP events around every launch gives each kernel family's
-average device time ("roofline_by_kernel") and the dominant family; the
-timed region then runs K steps with the state resident in HBM and HIP
-events around the dominant family's launches only, whose average device
-time is the "roofline" figure.  The CPU baseline is the serial FP64 oracle on a
+survey pass (eager launches, every chain on one stream) with HIP events
+around every launch gives each kernel family's own average device time
+("roofline_by_kernel") and the dominant family; the timed region then runs
+K steps launched eagerly with the independent chains side by side on the
+handle's two streams, the state resident in HBM, and HIP events around the
+dominant family's launches only, whose average device time is the
+"roofline" figure.  The same step replayed as one hipGraph per step is
+timed among the variants.  The CPU baseline is the serial FP64 oracle on a
 bounded sample of the same workload (the same generator at 251x126 nodes,
 62,500 elements x 40 layers), scaled to the metric's unit.
 
@@ -71,14 +74,15 @@ def main():
     dev.synchronize()
     print("bench: warmup done", file=sys.stderr, flush=True)
 
-    # Every chain runs on the handle's main stream (concurrency 0) so that no
-    # family's event time includes another kernel running beside it; the
-    # library's default of independent chains side by side is timed with the
-    # variants.  A survey pass (not timed) records HIP events around every
-    # launch: each family's average device time and the dominant family.
-    # The timed region then records events around the dominant family's
-    # launches only -- events around all ~300 launches of a step would add
-    # about 5 % to it.
+    # The survey pass (not timed) runs every chain on the handle's main
+    # stream (concurrency 0), eagerly, with HIP events around every launch,
+    # so that no family's event time includes another kernel running beside
+    # it: each family's average device time and the dominant family.  The
+    # timed region then runs the library's default (a hipGraph per step,
+    # chains side by side) with events around the dominant family's launches
+    # only -- events around all ~300 launches of a step would add about 5 %
+    # to it.  The dominant family (the 2-D momentum stage) runs on the main
+    # stream with nothing beside it, so its event time is its own.
     dev.set_concurrency(0)
     survey_steps = max(1, min(10, args.steps))
     dev.synchronize()
@@ -102,8 +106,11 @@ def main():
     # dominant kernel = the modelled family with the largest device time per step
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
 
-    # timed region: K internal steps, events around the dominant family
-    dev.synchronize()
+    # timed region: K internal steps, launched eagerly with the independent
+    # chains side by side on the handle's two streams, events around the
+    # dominant family (HIP events cannot time launches inside a replayed
+    # graph here; the graph replay of the same step is timed as a variant)
+    dev.set_concurrency(3)
     dev.set_profiling(True, kernel=dom)
     dev.synchronize()
     t0 = time.perf_counter()
@@ -163,7 +170,8 @@ def main():
         "data": "synthetic (seeded jittered mesh, seamounts, M2/S2/K1 tide, Gaussian random wind)",
         "config": {"workload": "FVCOM INTERNAL_STEP, %dx%d-node jittered mesh, %d elements x %d layers, "
                                "DTE=2 s ISPLIT=10, MY2.5, plain FP64" % (args.nx, args.ny, n, kbm1),
-                   "elements": n, "layers": kbm1, "dti_s": dti, "mode": "eager, one stream"},
+                   "elements": n, "layers": kbm1, "dti_s": dti,
+                   "mode": "eager launches, independent chains on two streams"},
         "elem_layer_steps_per_s": rate,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": pk["achieved_GBps"], "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": pk["frac"], "traffic": traffic,
@@ -182,21 +190,22 @@ def main():
 
 
 def time_variants(F, model, steps, warmup, fp64_step_s, dti, units):
-    """FP64 with every independent pair of chains side by side, the
-    mixed-precision form, and mixed precision with the learned closure, on
-    the same workload (graph replays, same step count)."""
+    """FP64 as one hipGraph replay per step, the mixed-precision form, and
+    mixed precision with the learned closure (graph replays, chains side by
+    side), on the same workload and step count."""
     from myfvcomcode_amd.closure import load_params
     out = {}
-    for name, prec, closure in (("fp64_graph_all_concurrent", "fp64", False), ("mixed", "mixed", False),
+    for name, prec, closure in (("fp64_graph", "fp64", False), ("mixed", "mixed", False),
                                 ("mixed_learned_closure", "mixed", True)):
         dev = F.DeviceModel(model, precision=prec)
+        graph = True
         dev.set_concurrency(3)
         if closure:
             dev.set_closure(1, load_params())
-        dev.run(warmup, graph=True)
+        dev.run(warmup, graph=graph)
         dev.synchronize()
         t0 = time.perf_counter()
-        dev.run(steps, graph=True)
+        dev.run(steps, graph=graph)
         dev.synchronize()
         w = (time.perf_counter() - t0) / steps
         if dev.diagnostics().nonfinite:

@@ -92,7 +92,7 @@ struct fvgpu_handle {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int concurrent = 3;     // bit 0: 3-D chain pairs, bit 1: the 2-D stage pair on stream2
   int two_phase_uv = 0;   // ADVECTION/ADV_UV as edge kernel + element gather (1) or element-centric (0)
-  int vertvl_column = 0;  // VERTVL one thread per node column (1) or FV_VV_CH lanes per column (0)
+  int two_phase_cv = 0;   // CV records then face fluxes in two launches (1) or one fused launch (0)
   // wind speeds at the start (wu0, wv0) and end (wu1, wv1) of the coming
   // internal step; once they differ, every external step forms its own
   // stress (ws2x, ws2y), otherwise it uses the end stress g.wsx/g.wsy
@@ -201,16 +201,17 @@ int flush_profile(fvgpu_handle* h) {
   return 0;
 }
 
+// The captured steps are stale (the step's launches or their options
+// changed): the next graph run captures again.
+void drop_graphs(fvgpu_handle* h) {
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+}
+
 #define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...)                                   \
   do {                                                                            \
     Launch l_(h, NAME);                                                           \
     hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), 0, h->cur, __VA_ARGS__);    \
-  } while (0)
-// The same with SHMEM bytes of dynamic LDS per workgroup.
-#define LAUNCH_SH(NAME, KERNEL, GRID, BLOCK, SHMEM, ...)                                  \
-  do {                                                                                    \
-    Launch l_(h, NAME);                                                                   \
-    hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), (SHMEM), h->cur, __VA_ARGS__);     \
   } while (0)
 
 inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
@@ -316,9 +317,14 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
   double* ff = which == 1 ? a.tf1 : a.sf1;
   const double* fm = which == 1 ? a.tmean1 : a.smean1;
   const double* fdis = which == 1 ? g.tdis : g.sdis;
+  if (h->two_phase_cv) {
 #define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
-  FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
-  TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.N + 1, g, a);
+    FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
+    TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.N + 1, g, a);
+  } else {
+#define CVRF_TS_CALL(RM) LAUNCH("cvrecflux_ts", (k_cvrecflux<0, RM, FT>), g.cvb_nblk, 256, g, a, f1, fm, 0.0)
+    FV_RMAX_DISPATCH(g.ring_max, CVRF_TS_CALL);
+  }
 #define ADV_TS_CALL10(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, 10, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
 #define ADV_TS_CALL16(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, 16, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
   if (g.ring_max <= 10) FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL10);
@@ -338,9 +344,14 @@ template <typename FT>
 void enqueue_adv_q(fvgpu_handle* h, IntArgs& a, double* q, double* qf, int fct) {
   const DevGrid& g = h->g;
   const double qmean1 = (double)1.E-8f;
+  if (h->two_phase_cv) {
 #define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
-  FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
-  TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.N + 1, g, a);
+    FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
+    TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.N + 1, g, a);
+  } else {
+#define CVRF_Q_CALL(RM) LAUNCH("cvrecflux_q", (k_cvrecflux<1, RM, FT>), g.cvb_nblk, 256, g, a, q, nullptr, qmean1)
+    FV_RMAX_DISPATCH(g.ring_max, CVRF_Q_CALL);
+  }
 #define ADV_Q_CALL10(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, 10, FT>), g.M + 1, g, a, q, qf, fct)
 #define ADV_Q_CALL16(SM) TLAUNCH("adv_q", (k_adv_q_gather<SM, 16, FT>), g.M + 1, g, a, q, qf, fct)
   if (g.ring_max <= 10) FV_SMAX_DISPATCH(g.cv_max, ADV_Q_CALL10);
@@ -368,14 +379,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext, iext == 1);
   TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
-  // VERTVL: FV_VV_CH lanes per node column, 128-lane workgroups (32 nodes,
-  // at most 32 x 63 x 16 B = 31.5 KB of LDS for the layer terms)
-#define VERTVL_CALL(SM)                                                                                \
-  LAUNCH_SH("vertvl", k_vertvl_c<SM>, (unsigned)((g.M + 1 + 128 / FV_VV_CH - 1) / (128 / FV_VV_CH)), 128, \
-            (size_t)(128 / FV_VV_CH) * g.KBM1 * 2 * sizeof(double), g, a)
-#define VERTVL_R_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
-  if (h->vertvl_column) FV_SMAX_DISPATCH(g.cv_max, VERTVL_R_CALL);
-  else FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
+#define VERTVL_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
+  FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
   TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
@@ -577,6 +582,57 @@ int build_lists(fvgpu_handle* h) {
   if (dupload(h, &g.node_obc, nobc.data(), nobc.size())) return 1;
   if (dupload(h, &g.node_riv, nriv.data(), nriv.size())) return 1;
   if (dupload(h, &g.fct_skip, skip.data(), skip.size())) return 1;
+  // element blocks of the fused record + face-flux kernels: the largest
+  // block size (at most 256 elements) whose blocks all use at most 256
+  // distinct nodes
+  {
+    int eb = 256;
+    std::vector<int> bnn, bnodes, loc;
+    for (;; eb -= 8) {
+      if (eb < 8) return fail("no element block size keeps the fused kernels' node sets within 256");
+      const int nblk = (N + eb - 1) / eb;
+      bnn.assign(nblk, 0);
+      bnodes.assign((size_t)nblk * 256, 0);
+      loc.assign((size_t)N + 1, 0);
+      bool ok = true;
+      std::vector<int> mark_blk(M + 1, -1), mark_loc(M + 1, 0);   // block that listed the node, its slot
+      for (int b = 0; b < nblk && ok; ++b) {
+        int nn = 0;
+        const int e0 = b * eb + 1, e1 = std::min(N, (b + 1) * eb);
+        for (int e = e0; e <= e1 && ok; ++e) {
+          int packed = 0;
+          for (int j = 0; j < 3 && ok; ++j) {
+            const int nd = md->nv[(size_t)j * (N + 1) + e];
+            if (mark_blk[nd] != b) {
+              if (nn >= 256) { ok = false; break; }
+              bnodes[(size_t)b * 256 + nn] = nd;
+              mark_blk[nd] = b;
+              mark_loc[nd] = nn++;
+            }
+            packed |= mark_loc[nd] << (8 * j);
+          }
+          loc[e] = packed;
+        }
+        bnn[b] = nn;
+      }
+      if (ok) { g.cvb_nblk = nblk; break; }
+    }
+    g.cvb_eb = eb;
+    std::vector<int> keep(M + 1, 0);
+    for (int i = 0; i < md->iobcn; ++i) {
+      const int j = md->i_obc_n[i];
+      keep[j] = 1;
+      for (int q = off[j]; q < off[j + 1]; ++q) {
+        const int c = ent[q] >> 1;
+        keep[md->niec[c]] = 1;
+        keep[md->niec[(size_t)(NCV + 1) + c]] = 1;
+      }
+    }
+    if (dupload(h, &g.cvb_nn, bnn.data(), bnn.size())) return 1;
+    if (dupload(h, &g.cvb_nodes, bnodes.data(), bnodes.size())) return 1;
+    if (dupload(h, &g.cvb_loc, loc.data(), loc.size())) return 1;
+    if (dupload(h, &g.rec_keep, keep.data(), keep.size())) return 1;
+  }
   // element -> rivers (NBVE(J,1) then NBVE(J,NTVE(J)) of the river node)
   std::vector<std::vector<int>> re(N + 1);
   for (int k = 0; k < md->numqbc; ++k) {
@@ -831,8 +887,12 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   h->precision = precision;
   HIPCHK(hipGetDevice(&h->device));
   if (!scratch_of(h->device)) { delete h; return 1; }
-  if (const char* v = std::getenv("FVGPU_TWO_PHASE_UV")) h->two_phase_uv = std::atoi(v) != 0;
-  if (const char* v = std::getenv("FVGPU_VERTVL_COLUMN")) h->vertvl_column = std::atoi(v) != 0;
+  // FP64: element-centric ADVECTION/ADV_UV and records + face fluxes in
+  // one launch; the FP32 flux form keeps the two-launch forms of both
+  // (measured faster for it: its higher occupancy there outweighs the
+  // intermediate buffers' traffic)
+  h->two_phase_uv = precision == FV_MIXED ? 1 : 0;
+  h->two_phase_cv = precision == FV_MIXED ? 1 : 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
   h->cur = h->stream;
   if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
@@ -1002,8 +1062,7 @@ int fvgpu_set_forcing(fvgpu_handle* h, const char* name, const double* host, int
     if (varies && !h->wind_varies) {
       // the captured step gains the per-external-step stress kernel
       h->wind_varies = 1;
-      if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-      if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+      drop_graphs(h);
     }
   } else if (n == "qdis" || n == "tdis" || n == "sdis") {
     if ((size_t)len != nr) return fail(n + ": expected one value per river (NUMQBC)");
@@ -1046,7 +1105,9 @@ int fvgpu_set_profiling(fvgpu_handle* h, int32_t on) {
 }
 
 int fvgpu_profile_only(fvgpu_handle* h, const char* name) {
-  h->profile_only = name ? name : "";
+  const std::string n = name ? name : "";
+  if (n != h->profile_only) drop_graphs(h);
+  h->profile_only = n;
   return 0;
 }
 
@@ -1083,8 +1144,7 @@ void fvgpu_destroy(fvgpu_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->stream2) hipStreamSynchronize(h->stream2);
-  if (h->gexec) hipGraphExecDestroy(h->gexec);
-  if (h->graph) hipGraphDestroy(h->graph);
+  drop_graphs(h);
   for (hipEvent_t e : h->event_pool) hipEventDestroy(e);
   for (void* p : h->allocs) hipFree(p);
   if (h->ev_fork) hipEventDestroy(h->ev_fork);
@@ -1204,8 +1264,7 @@ int fvgpu_set_closure(fvgpu_handle* h, int32_t mode, const float* params) {
     HIPCHK(hipFree(dp));
   }
   h->closure_mode = mode;
-  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }   // the step changed
-  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  drop_graphs(h);   // the step changed
   return 0;
 }
 
@@ -1402,23 +1461,21 @@ extern "C" {
 int fvgpu_set_block(fvgpu_handle* h, const char* family, int32_t block) {
   if (block != 64 && block != 128 && block != 256) return fail("workgroup size must be 64, 128 or 256");
   h->tune[family] = block;
-  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  drop_graphs(h);
   return 0;
 }
 
 int fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode) {
   if (mode < 0 || mode > 3) return fail("concurrency must be 0..3");
   h->concurrent = mode;
-  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  drop_graphs(h);
   return 0;
 }
 
 int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
   static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
                                "cvflux_q", "adv_ts", "adv_q", "uvedge_adv", "uvedge_uv", "adv3d", "advuv",
-                               "adv3d_elem", "advuv_elem", "vertvl",
+                               "adv3d_elem", "advuv_elem", "cvrecflux_ts", "cvrecflux_q", "vertvl",
                                "viscofh", "vdif_q", "vdif_ts", "n2e3d", "adjust1_brough", "adjust2", "baropg",
                                "wreal", "uv_finish", "copy", "closure_features", "closure_apply"};
   static const int cands[] = {64, 128, 256};
@@ -1479,8 +1536,7 @@ int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_
   HIPCHK(hipMemcpyAsync(h->clk, arena + off, sizeof(DevClock), hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipFree(arena));
-  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  drop_graphs(h);
   if (report && report_len > 0) {
     std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
     report[report_len - 1] = 0;

@@ -65,6 +65,14 @@ struct DevGrid {
   const int *node_riv;             // node -> river index or -1
   const int *re_off, *re_riv;      // element -> rivers whose NBVE(1) / NBVE(NTVE) it is
   const int *fct_skip;             // node skipped by FCT (OBC or river node)
+  // Element blocks of the fused record + face-flux kernels: block b holds
+  // elements b*cvb_eb+1 .. and the cvb_nn[b] (<= 256) distinct nodes they
+  // use, cvb_nodes[256*b + t]; cvb_loc[e] packs the local indices of
+  // NV(e,1..3) in bits 0-7, 8-15, 16-23.  rec_keep[i] = 1 for the nodes
+  // whose records other kernels read at every layer (the OBC nodes and the
+  // other ends of their sub-edges).
+  int cvb_eb, cvb_nblk;
+  const int *cvb_nn, *cvb_nodes, *cvb_loc, *rec_keep;
   // OBC / tide / rivers
   const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc;
   const double *fluxf_obc, *dltn_obc, *emean, *apt, *phai, *period;

@@ -30,73 +30,102 @@
 // record is stored in the flux type FT.
 // kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
 // DZ-weighted interface viscosity (adv_q.F) times NN_HVC.
+// Ring of node i (RMAX >= NTSN): node indices and the DLTXTRIE/DLTYTRIE
+// factors of its ring segments, in registers across the layers.
+template <int RMAX>
+struct RingGeo {
+  int rn[RMAX];
+  double rdx[RMAX], rdy[RMAX];
+  int nt;
+};
+
+template <int RMAX>
+__device__ __forceinline__ void ring_geo_load(const DevGrid& g, int i, RingGeo<RMAX>& r) {
+  const int o = g.nbsn_off[i], nt = g.ntsn[i];
+  r.nt = nt;
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    r.rn[j] = j < nt ? g.nbsn[o + j] : 0;
+    r.rdx[j] = j < nt - 1 ? g.dltxtrie[o + j] : 0.0;
+    r.rdy[j] = j < nt - 1 ? g.dltytrie[o + j] : 0.0;
+  });
+}
+
+// The record of node i at layer k: ring gradient of f and of f - mean
+// (adv_t.F:219-376; adv_q.F:185-230 with the constant mean QMEAN1), the
+// ring bounds with the node (adv_t.F:469-480) and the viscosity factor.
+// kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
+// DZ-weighted interface viscosity (adv_q.F) times NN_HVC.  The sums are
+// FP64; the record is stored in the flux type FT.
+template <int KIND, int RMAX, typename FT>
+__device__ __forceinline__ CvRecT<FT> cv_record(const DevGrid& g, const IntArgs& a, int i, const RingGeo<RMAX>& rg,
+                                                double art2, double nn, const double* f, const double* mean,
+                                                double cmean, int k) {
+  const long ko = (long)(k - 1) * g.m1;
+  const int nt = rg.nt;
+  // the ring's values are loaded unconditionally (slots past the ring
+  // read node 0) so that the loads issue together; the sums take the
+  // ring's own terms in ring order
+  double fv[RMAX], mv[RMAX];
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    fv[j] = f[ko + rg.rn[j]];
+    mv[j] = mean ? mean[ko + rg.rn[j]] : cmean;
+  });
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+  double f1 = fv[0];
+  double m1v = mv[0];
+  double lo = f1, hi = f1;
+  static_for<RMAX - 1>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < nt - 1) {
+      const double f2 = fv[j + 1];
+      const double m2v = mv[j + 1];
+      const double ffd = 0.5 * (f1 + f2 - m1v - m2v);
+      const double ff1 = 0.5 * (f1 + f2);
+      s1 = s1 + ff1 * rg.rdy[j];
+      s2 = s2 + ff1 * rg.rdx[j];
+      s3 = s3 + ffd * rg.rdy[j];
+      s4 = s4 + ffd * rg.rdx[j];
+      if (j >= 1) { lo = fmin(lo, f1); hi = fmax(hi, f1); }
+      f1 = f2;
+      m1v = m2v;
+    }
+  });
+  const double c = f[ko + i];
+  double vhn;
+  if (KIND == 0) {
+    vhn = a.viscofh[ko + i] * nn;
+  } else {
+    const double vfa = (a.viscofh[ko + i] * DZ(i, k - 1) + a.viscofh[ko - g.m1 + i] * DZ(i, k)) /
+                       (DZ(i, k) + DZ(i, k - 1));
+    vhn = vfa * nn;
+  }
+  CvRecT<FT> r;
+  r.f = (FT)c;
+  r.gx = (FT)(s1 / art2);
+  r.gy = (FT)(s2 / art2);
+  r.fmin = (FT)fmin(lo, c);
+  r.fmax = (FT)fmax(hi, c);
+  r.gxd = (FT)(s3 / art2);
+  r.gyd = (FT)(s4 / art2);
+  r.vhn = (FT)vhn;
+  return r;
+}
+
+// Ring gradient / bounds of field f at node i for the layers [k0, k1], one
+// thread per node with its ring in registers.
 template <int KIND, int RMAX, typename FT>
 __global__ __launch_bounds__(256) void k_cvrec(DevGrid g, IntArgs a, const double* f, const double* mean,
                                                double cmean, int k0, int k1) {
   const int i = (int)xcd_thread();
   if (i < 1 || i > g.M) return;
   CvRecT<FT>* rec = (CvRecT<FT>*)a.rec;
-  const int o = g.nbsn_off[i], nt = g.ntsn[i];
-  int rn[RMAX];
-  double rdx[RMAX], rdy[RMAX];
-  static_for<RMAX>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    rn[j] = j < nt ? g.nbsn[o + j] : 0;
-    rdx[j] = j < nt - 1 ? g.dltxtrie[o + j] : 0.0;
-    rdy[j] = j < nt - 1 ? g.dltytrie[o + j] : 0.0;
-  });
+  RingGeo<RMAX> rg;
+  ring_geo_load<RMAX>(g, i, rg);
   const double art2 = g.art2[i], nn = g.nn_hvc[i];
-  for (int k = k0; k <= k1; ++k) {
-    const long ko = (long)(k - 1) * g.m1;
-    // the ring's values are loaded unconditionally (slots past the ring
-    // read node 0) so that the loads issue together; the sums take the
-    // ring's own terms in ring order
-    double fv[RMAX], mv[RMAX];
-    static_for<RMAX>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      fv[j] = f[ko + rn[j]];
-      mv[j] = mean ? mean[ko + rn[j]] : cmean;
-    });
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-    double f1 = fv[0];
-    double m1v = mv[0];
-    double lo = f1, hi = f1;
-    static_for<RMAX - 1>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      if (j < nt - 1) {
-        const double f2 = fv[j + 1];
-        const double m2v = mv[j + 1];
-        const double ffd = 0.5 * (f1 + f2 - m1v - m2v);
-        const double ff1 = 0.5 * (f1 + f2);
-        s1 = s1 + ff1 * rdy[j];
-        s2 = s2 + ff1 * rdx[j];
-        s3 = s3 + ffd * rdy[j];
-        s4 = s4 + ffd * rdx[j];
-        if (j >= 1) { lo = fmin(lo, f1); hi = fmax(hi, f1); }
-        f1 = f2;
-        m1v = m2v;
-      }
-    });
-    const double c = f[ko + i];
-    double vhn;
-    if (KIND == 0) {
-      vhn = a.viscofh[ko + i] * nn;
-    } else {
-      const double vfa = (a.viscofh[ko + i] * DZ(i, k - 1) + a.viscofh[ko - g.m1 + i] * DZ(i, k)) /
-                         (DZ(i, k) + DZ(i, k - 1));
-      vhn = vfa * nn;
-    }
-    CvRecT<FT> r;
-    r.f = (FT)c;
-    r.gx = (FT)(s1 / art2);
-    r.gy = (FT)(s2 / art2);
-    r.fmin = (FT)fmin(lo, c);
-    r.fmax = (FT)fmax(hi, c);
-    r.gxd = (FT)(s3 / art2);
-    r.gyd = (FT)(s4 / art2);
-    r.vhn = (FT)vhn;
-    rec[ko + i] = r;
-  }
+  for (int k = k0; k <= k1; ++k)
+    rec[(long)(k - 1) * g.m1 + i] = cv_record<KIND, RMAX, FT>(g, a, i, rg, art2, nn, f, mean, cmean, k);
 }
 
 // Min and max of field f over the NBSN ring of node i at layer offset ko,
@@ -196,6 +225,32 @@ __device__ __forceinline__ FT face_value(const CvRecT<FT>& r, FT x, FT y) {
   return v;
 }
 
+// The three face fluxes of one element at one layer (adv_t.F:440-500,
+// adv_q.F:200-330) from the records r[3] of its nodes NV(e,1..3), the
+// normal velocity (un = V*DLTXE - U*DLTYE) from the element velocity (uk,
+// vk) and the layer thickness dtij; viscosity divided by HPRNU when DIVP.
+template <typename FT, bool DIVP>
+__device__ __forceinline__ void cv_faces(const ElemFaces<FT>& f, const CvRecT<FT> (&r)[3], FT uk, FT vk, FT dtij,
+                                         FT fact, FT rhprnu, FT* out) {
+  const FT half = (FT)0.5, one = (FT)1.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const CvRecT<FT>& ra = r[j];
+    const CvRecT<FT>& rb = r[(j + 1) % 3];
+    const FT fa = face_value(ra, f.xa[j], f.ya[j]), fb = face_value(rb, f.xb[j], f.yb[j]);
+    const FT fij1 = f.swap[j] ? fb : fa, fij2 = f.swap[j] ? fa : fb;
+    const FT un = vk * f.dltx[j] - uk * f.dlty[j];
+    FT viscof = fact * half * (ra.vhn + rb.vhn) + f.visc0[j];
+    if (DIVP && rhprnu != one) viscof = viscof / rhprnu;   // x / 1 is x: skipped when HPRNU = 1
+    const FT txx = half * (ra.gxd + rb.gxd) * viscof;
+    const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
+    const FT fxx = -dtij * txx * f.dlty[j];
+    const FT fyy = dtij * tyy * f.dltx[j];
+    const FT sgn = fsign1(un);
+    out[j] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
+  }
+}
+
 // Phase 2 of ADV_T/ADV_S: the three face fluxes of one element per layer
 // (adv_t.F:440-500), with the geometry held in registers across layers.
 // The two nodes of a face enter the diffusive terms through sums, which are
@@ -219,22 +274,7 @@ __global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
     CvRecT<FT> r[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) r[q] = rec[ko + nd[q]];
-    FT* out = cvflux + (long)(k - 1) * cs + p0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const CvRecT<FT>& ra = r[j];
-      const CvRecT<FT>& rb = r[(j + 1) % 3];
-      const FT fa = face_value(ra, f.xa[j], f.ya[j]), fb = face_value(rb, f.xb[j], f.yb[j]);
-      const FT fij1 = f.swap[j] ? fb : fa, fij2 = f.swap[j] ? fa : fb;
-      const FT un = vk * f.dltx[j] - uk * f.dlty[j];
-      const FT viscof = fact * half * (ra.vhn + rb.vhn) + f.visc0[j];
-      const FT txx = half * (ra.gxd + rb.gxd) * viscof;
-      const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
-      const FT fxx = -dtij * txx * f.dlty[j];
-      const FT fyy = dtij * tyy * f.dltx[j];
-      const FT sgn = fsign1(un);
-      out[j] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
-    }
+    cv_faces<FT, false>(f, r, uk, vk, dtij, fact, one, cvflux + (long)(k - 1) * cs + p0);
   }
 }
 
@@ -389,23 +429,72 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
     um = uk;
     vm = vk;
     const FT dtij = (FT)(dt1 * dzz1);
-    FT* out = cvflux + (long)(k - 1) * cs + p0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const CvRecT<FT>& ra = r[j];
-      const CvRecT<FT>& rb = r[(j + 1) % 3];
-      const FT fa = face_value(ra, f.xa[j], f.ya[j]), fb = face_value(rb, f.xb[j], f.yb[j]);
-      const FT fij1 = f.swap[j] ? fb : fa, fij2 = f.swap[j] ? fa : fb;
-      const FT un = vq * f.dltx[j] - uq * f.dlty[j];
-      FT viscof = fact * half * (ra.vhn + rb.vhn) + f.visc0[j];
-      if (rhprnu != one) viscof = viscof / rhprnu;   // x / 1 is x: skipped when HPRNU = 1
-      const FT txx = half * (ra.gxd + rb.gxd) * viscof;
-      const FT tyy = half * (ra.gyd + rb.gyd) * viscof;
-      const FT fxx = -dtij * txx * f.dlty[j];
-      const FT fyy = dtij * tyy * f.dltx[j];
-      const FT sgn = fsign1(un);
-      out[j] = -un * dtij * ((one + sgn) * fij2 + (one - sgn) * fij1) * half + fxx + fyy;
+    cv_faces<FT, true>(f, r, uq, vq, dtij, fact, rhprnu, cvflux + (long)(k - 1) * cs + p0);
+  }
+}
+
+// ADV_T/ADV_S (KIND 0, layers 1..KBM1) or ADV_Q (KIND 1, interfaces
+// 2..KBM1) phases 1 and 2 in one launch: a workgroup takes a block of
+// consecutive elements and the distinct nodes they use (at most 256,
+// cvb_*), and per layer forms the nodes' records into LDS (one thread per
+// node, ring in registers: cv_record, the same arithmetic as k_cvrec) and
+// then the elements' three face fluxes from them (one thread per element:
+// cv_faces, as k_cvflux_*).  A node shared by two blocks is formed by both;
+// the records cross HBM only for the nodes whose records other kernels read
+// (rec_keep: OBC nodes and their sub-edge neighbours) and, for the tracer,
+// at the bottom layer (the bottom gradients of VDIF_TS).
+template <int KIND, int RMAX, typename FT>
+__global__ __launch_bounds__(256) void k_cvrecflux(DevGrid g, IntArgs a, const double* fld, const double* mean,
+                                                   double cmean) {
+  __shared__ CvRecT<FT> srec[256];
+  const int b = xcd_block(), t = threadIdx.x;
+  const int nn = g.cvb_nn[b];
+  CvRecT<FT>* rec = (CvRecT<FT>*)a.rec;
+  FT* cvflux = (FT*)a.cvflux;
+  // node part: local node t of the block
+  const bool nact = t < nn;
+  const int i = nact ? g.cvb_nodes[256L * b + t] : 1;
+  RingGeo<RMAX> rg;
+  ring_geo_load<RMAX>(g, i, rg);
+  const double art2 = g.art2[i], nnv = g.nn_hvc[i];
+  const bool keep = nact && g.rec_keep[i] != 0;
+  // element part: element b*EB + t + 1
+  const int e = b * g.cvb_eb + t + 1;
+  const bool eact = t < g.cvb_eb && e <= g.N;
+  const int ee = eact ? e : 1;
+  const ElemFaces<FT> f = elem_faces<FT>(g, ee);
+  const int pk = g.cvb_loc[ee];
+  const int l0 = pk & 255, l1 = (pk >> 8) & 255, l2 = (pk >> 16) & 255;
+  const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu, one = (FT)1.0;
+  const double dt1 = a.dt1[ee];
+  const long cs = (long)g.NCV + 1, p0 = 3L * (ee - 1) + 1;
+  double um = a.u[ee], vm = a.v[ee];   // layer 1 (ADV_Q's interface interpolation)
+  const int k0 = KIND == 0 ? 1 : 2;
+  for (int k = k0; k <= g.KBM1; ++k) {
+    if (nact) {
+      const CvRecT<FT> r = cv_record<KIND, RMAX, FT>(g, a, i, rg, art2, nnv, fld, mean, cmean, k);
+      srec[t] = r;
+      if (keep || (KIND == 0 && k == g.KBM1)) rec[(long)(k - 1) * g.m1 + i] = r;
     }
+    __syncthreads();
+    if (eact) {
+      const long eo = (long)(k - 1) * g.n1;
+      const CvRecT<FT> r[3] = {srec[l0], srec[l1], srec[l2]};
+      FT* out = cvflux + (long)(k - 1) * cs + p0;
+      if (KIND == 0) {
+        const FT dtij = (FT)(dt1 * DZ1(e, k));
+        cv_faces<FT, false>(f, r, (FT)a.u[eo + e], (FT)a.v[eo + e], dtij, fact, one, out);
+      } else {
+        const double uk = a.u[eo + e], vk = a.v[eo + e];
+        const double dz1k = DZ1(e, k), dz1m = DZ1(e, k - 1), dzz1 = DZZ1(e, k - 1);
+        const FT uq = (FT)((uk * dz1m + um * dz1k) / (dz1k + dz1m));
+        const FT vq = (FT)((vk * dz1m + vm * dz1k) / (dz1k + dz1m));
+        um = uk;
+        vm = vk;
+        cv_faces<FT, true>(f, r, uq, vq, (FT)(dt1 * dzz1), fact, rhprnu, out);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -558,112 +647,6 @@ __global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
     }
   }
   a.dtfa[i] = dtfa;
-}
-
-// VERTVL_EDGE with each node's column split over FV_VV_CH lanes: lane c of
-// a node forms the layer terms of its chunk of layers (the CV divergence
-// over the node's sub-edges in sub-edge order, as in k_vertvl_r) and parks
-// them in LDS; then every lane of the node runs the column integration
-// w(k+1) = (w(k) + XFLUX/ART1) + DZ*(D-DT)/DTI over all layers in order
-// from LDS (the same operations in the same order as one thread would, so
-// the same bits), and writes WTS of its own chunk with the linear mass
-// correction applied, and lane 0 writes DTFA.  Four lanes per node carry
-// four times the loads in flight of one thread per column.
-#define FV_VV_CH 4
-template <int SMAX>
-__global__ __launch_bounds__(256) void k_vertvl_c(DevGrid g, IntArgs a) {
-  extern __shared__ double vv_sh[];      // [nodes of the block][KBM1][2]: XFLUX/ART1, DZ*(D-DT)/DTI
-  const int npb = (int)blockDim.x / FV_VV_CH;
-  const int nl = threadIdx.x / FV_VV_CH, c = threadIdx.x % FV_VV_CH;
-  const int i = xcd_block() * npb + nl;
-  const int KBM1 = g.KBM1;
-  const int L = (KBM1 + FV_VV_CH - 1) / FV_VV_CH;
-  const int k0 = c * L + 1, k1 = min(k0 + L - 1, KBM1);
-  double* sh = vv_sh + (long)nl * KBM1 * 2;
-  const bool act = i >= 1 && i <= g.M;
-  if (i == 0 && c == 0) {
-    for (int k = 1; k <= g.KB; ++k) a.wts[NI(g, 0, k)] = 0.0;
-    a.dtfa[0] = a.d[0];
-  }
-  bool obc = false;
-  double art1 = 1.0, dd = 0.0;
-  if (act) {
-    const int n = g.cv_cnt[i];
-    int ce[SMAX];
-    double dte[SMAX], sdx[SMAX], sdy[SMAX];
-    static_for<SMAX>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      const long q = (long)j * g.m1 + i;
-      ce[j] = j < n ? g.cvl_elem[q] : 0;
-      dte[j] = j < n ? a.dt1[ce[j]] : 0.0;
-      sdx[j] = j < n ? g.cvl_sdx[q] : 0.0;
-      sdy[j] = j < n ? g.cvl_sdy[q] : 0.0;
-    });
-    obc = g.isonb[i] == 2;
-    const int rv = g.node_riv[i];
-    art1 = g.art1[i];
-    dd = a.d[i] - a.dt[i];
-    for (int k = k0; k <= k1; ++k) {
-      const long ko = (long)(k - 1) * g.n1;
-      double x = 0.0;
-      constexpr int CHS = 8;
-      static_for<(SMAX + CHS - 1) / CHS>([&](auto cc) {
-        constexpr int c0 = decltype(cc)::value * CHS;
-        if (c0 < n) {
-          double uu[CHS], vv[CHS], dz[CHS];
-          static_for<CHS>([&](auto jc) {
-            constexpr int j = c0 + decltype(jc)::value;
-            if constexpr (j < SMAX) {
-              uu[j - c0] = a.u[ko + ce[j]];
-              vv[j - c0] = a.v[ko + ce[j]];
-              dz[j - c0] = DZ1(ce[j], k);
-            }
-          });
-          static_for<CHS>([&](auto jc) {
-            constexpr int j = c0 + decltype(jc)::value;
-            if constexpr (j < SMAX) {
-              if (j < n) {
-                const double dij = dte[j] * dz[j - c0];
-                x = x + dij * (-uu[j - c0] * sdy[j] + vv[j - c0] * sdx[j]);
-              }
-            }
-          });
-        }
-      });
-      double xflux = obc ? 0.0 : x;
-      if (rv >= 0)
-        for (int j = rv; j < g.NUMQBC; ++j)
-          if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
-      sh[2 * (k - 1)] = xflux / art1;
-      sh[2 * (k - 1) + 1] = DZ(i, k) * dd / g.dti;
-    }
-  }
-  __syncthreads();
-  if (!act) return;
-  double w = 0.0;
-  for (int k = 1; k <= KBM1; ++k) w = (w + sh[2 * (k - 1)]) + sh[2 * (k - 1) + 1];
-  const double wbottom = 0.0;
-  const bool corr = fabs(w - wbottom) > 1.0E-8 && !obc;
-  const double wc = w - wbottom;
-  double wk = 0.0;
-  for (int k = 1; k <= k1; ++k) {
-    wk = (wk + sh[2 * (k - 1)]) + sh[2 * (k - 1) + 1];
-    if (k >= k0) {
-      double v = wk;
-      if (corr) v = v - (double)((float)k / (float)KBM1) * wc;   // WTS(k+1): (k+1-1)/KBM1
-      a.wts[NI(g, i, k + 1)] = v;
-    }
-  }
-  if (c == 0) {
-    a.wts[NI(g, i, 1)] = 0.0;
-    double dtfa = a.d[i];
-    if (corr) {
-      double tmp1 = a.elf[i] * (double)KBM1 - wc * g.dti / DZ(i, 1);
-      tmp1 = tmp1 / (double)KBM1;
-      dtfa = tmp1 + g.h[i];
-    }
-    a.dtfa[i] = dtfa;
-  }
 }
 
 // VISCOF_H (viscofh.F:33-127): nodal strain-rate magnitude from the NBVE

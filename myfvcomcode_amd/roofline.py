@@ -95,6 +95,31 @@ def cvflux_q(s, real=R8):
     return _cvflux(s, s["KBM1"] - 1, s["KBM1"], real)
 
 
+def _cvrecflux(s, layers, nfields, uv_layers, bottom_records, real):
+    """Records formed in LDS and face fluxes in one launch: the node fields
+    and ring geometry, the element geometry and velocities are read, the
+    face fluxes (and the bottom layer's records for the tracer) written;
+    no node-layer record crosses HBM."""
+    M, N, NCV, RSN = s["M"], s["N"], s["NCV"], s["RSN"]
+    reads = (RSN * (I4 + 2 * real) + M * (2 * I4 + 2 * real)   # ring lists and factors, NTSN, ART2, NN_HVC
+             + nfields * M * layers * real                     # field, mean, VISCOFH
+             + N * I4 + M * I4                                 # block node lists, element local indices
+             + NCV * (3 * I4 + 6 * real) + N * real            # sub-edge geometry, DT1
+             + N * uv_layers * 2 * real)                       # U, V
+    writes = NCV * layers * real + (M * 8 * real if bottom_records else 0)
+    return reads + writes
+
+
+def cvrecflux_ts(s, real=R8):
+    """ADV_T/ADV_S records and face fluxes in one launch."""
+    return _cvrecflux(s, s["KBM1"], 3, s["KBM1"], True, real)
+
+
+def cvrecflux_q(s, real=R8):
+    """ADV_Q records and face fluxes in one launch."""
+    return _cvrecflux(s, s["KBM1"] - 1, 2, s["KBM1"], False, real)
+
+
 def adv_ts(s, real=R8):
     """Ordered sub-edge sums + vertical flux + update (T or S)."""
     M, NCV, KB, KBM1 = s["M"], s["NCV"], s["KB"], s["KBM1"]
@@ -227,7 +252,8 @@ def n2e3d(s, real=R8):
 
 
 FAMILIES = {f.__name__: f for f in (
-    ext_elevation, advave_edge, ext_momentum, cvrec_ts, cvrec_q, cvflux_ts, cvflux_q, adv_ts, adv_q,
+    ext_elevation, advave_edge, ext_momentum, cvrec_ts, cvrec_q, cvflux_ts, cvflux_q, cvrecflux_ts, cvrecflux_q,
+    adv_ts, adv_q,
     uvedge_adv, uvedge_uv, adv3d, advuv, adv3d_elem, advuv_elem, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts,
     n2e3d)}
 

@@ -3,7 +3,7 @@ surface column, every so many internal steps, until the end or the first
 non-finite value.
 
     python -m myfvcomcode_amd.stability [--config 1] [--hours 6] [--every 30]
-        [--dte S] [--umol K] [--out FILE.json]
+        [--fine-after H --fine-every MIN] [--dte S] [--umol K] [--out FILE.json]
 
 Per report (one JSON line each): the largest surface speed and where it is,
 the internal step's horizontal advective Courant number there
@@ -72,6 +72,8 @@ def main(argv=None):
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--hours", type=float, default=6.0)
     ap.add_argument("--every", type=float, default=30.0, help="minutes between reports")
+    ap.add_argument("--fine-after", type=float, default=None, help="hours after which reports come every --fine-every")
+    ap.add_argument("--fine-every", type=float, default=None, help="minutes between reports after --fine-after")
     ap.add_argument("--dte", type=float, default=None, help="external step (s); DTI = ISPLIT * DTE")
     ap.add_argument("--umol", type=float, default=None, help="background vertical viscosity (m^2/s)")
     ap.add_argument("--out", default=None)
@@ -91,8 +93,11 @@ def main(argv=None):
     head = {"config": args.config, "elements": m.n, "layers": m.kb - 1, "dte": m.hdr.p.dte, "dti": dti,
             "umol": m.hdr.p.umol, "min_altitude_m": float(alt[1:].min())}
     print(json.dumps(head), flush=True)
+    fine_from = nsteps if args.fine_after is None else int(round(args.fine_after * 3600.0 / dti))
+    fine = chunk if args.fine_every is None else max(1, int(round(args.fine_every * 60.0 / dti)))
     while done < nsteps:
-        s = min(chunk, nsteps - done)
+        step = fine if done >= fine_from else min(chunk, fine_from - done)
+        s = min(step, nsteps - done)
         dev.run(s, graph=True)
         done += s
         r = report(dev, m, alt, nv)
