@@ -47,6 +47,8 @@ FAMILY_OF = [
     (r"k_adv3d_gather<", "adv3d"), (r"k_advuv_gather<", "advuv"), (r"k_vertvl_r<", "vertvl"),
     (r"k_viscofh_r<", "viscofh"), (r"k_vdif_q_r<", "vdif_q"), (r"k_vdif_uv_r<", "vdif_uv"),
     (r"k_vdif_ts_r<", "vdif_ts"), (r"k_n2e3d\b", "n2e3d"),
+    (r"k_cvrecflux<0,", "cvrecflux_ts"), (r"k_cvrecflux<1,", "cvrecflux_q"),
+    (r"k_uv_elem<0,", "adv3d_elem"), (r"k_uv_elem<1,", "advuv_elem"),
 ]
 
 
