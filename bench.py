@@ -2,8 +2,9 @@
 
 Workload: the 1001x501-node jittered mesh (1,000,000 elements x 40 sigma
 layers, KB=41), depth 5+195*(x/Lx)^1.5 with 20 seeded seamounts, M2/S2/K1
-tide on the east boundary, constant seeded wind field, MY2.5 closure,
-DTE = 2 s, ISPLIT = 10 (DTI = 20 s), synthetic seeded data.
+tide on the east boundary, a seeded Gaussian random wind field held
+constant, MY2.5 closure, DTE = 2 s, ISPLIT = 10 (DTI = 20 s), synthetic
+seeded data.
 
 A step is one INTERNAL_STEP (ISPLIT external steps included).  An untimed
 survey pass (eager launches, every chain on one stream) with HIP events
@@ -15,7 +16,8 @@ dominant family's launches only, whose average device time is the
 "roofline" figure.  The same step replayed as one hipGraph per step is
 timed among the variants.  The CPU baseline is the serial FP64 oracle on a
 bounded sample of the same workload (the same generator at 251x126 nodes,
-62,500 elements x 40 layers), scaled to the metric's unit.
+62,500 elements x 40 layers), on one core and as one rank per host core (at
+most 16), scaled to the metric's unit.
 
 Prints one JSON line.
 """
@@ -78,11 +80,11 @@ def main():
     # stream (concurrency 0), eagerly, with HIP events around every launch,
     # so that no family's event time includes another kernel running beside
     # it: each family's average device time and the dominant family.  The
-    # timed region then runs the library's default (a hipGraph per step,
-    # chains side by side) with events around the dominant family's launches
-    # only -- events around all ~300 launches of a step would add about 5 %
-    # to it.  The dominant family (the 2-D momentum stage) runs on the main
-    # stream with nothing beside it, so its event time is its own.
+    # timed region then launches eagerly with the chains side by side, with
+    # events around the dominant family's launches only -- events around all
+    # ~300 launches of a step would add about 5 % to it.  The dominant family
+    # (the 2-D momentum stage) runs on the main stream with nothing beside
+    # it, so its event time is its own.
     dev.set_concurrency(0)
     survey_steps = max(1, min(10, args.steps))
     dev.synchronize()

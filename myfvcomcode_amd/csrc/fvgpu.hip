@@ -28,6 +28,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <utility>
 
 namespace {
 
@@ -91,7 +92,6 @@ struct fvgpu_handle {
   hipStream_t stream2 = nullptr, cur = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int concurrent = 3;     // bit 0: 3-D chain pairs, bit 1: the 2-D stage pair on stream2
-  int two_phase_uv = 0;   // ADVECTION/ADV_UV as edge kernel + element gather (1) or element-centric (0)
   int two_phase_cv = 0;   // CV records then face fluxes in two launches (1) or one fused launch (0)
   // wind speeds at the start (wu0, wv0) and end (wu1, wv1) of the coming
   // internal step; once they differ, every external step forms its own
@@ -365,12 +365,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
   TLAUNCH("adjust1_brough", k_adjust1_brough, g.N + 1, g, a);
-  if (h->two_phase_uv) {
-    TLAUNCH("uvedge_adv", (k_uvedge<0, FT>), g.NE + 1, g, a, a.u, a.v);
-    TLAUNCH("adv3d", k_adv3d_gather<FT>, g.N + 1, g, a);
-  } else {
-    TLAUNCH("adv3d_elem", (k_uv_elem<0, FT>), g.N + 1, g, a);
-  }
+  LAUNCH("adv3d_blk", (k_uv_blk<0, FT>), g.ub_nblk, 256, g, a);
   TLAUNCH("baropg", k_baropg, g.N + 1, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, g.M + 1, g, a);
   TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
@@ -384,12 +379,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
-  if (h->two_phase_uv) {
-    TLAUNCH("uvedge_uv", (k_uvedge<1, FT>), g.NE + 1, g, a, a.u, a.v);
-    TLAUNCH("advuv", k_advuv_gather<FT>, g.N + 1, g, a);
-  } else {
-    TLAUNCH("advuv_elem", (k_uv_elem<1, FT>), g.N + 1, g, a);
-  }
+  LAUNCH("advuv_blk", (k_uv_blk<1, FT>), g.ub_nblk, 256, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   TLAUNCH("uv_finish", k_uv_finish, (long)g.N * g.KBM1, g, a);
@@ -582,42 +572,73 @@ int build_lists(fvgpu_handle* h) {
   if (dupload(h, &g.node_obc, nobc.data(), nobc.size())) return 1;
   if (dupload(h, &g.node_riv, nriv.data(), nriv.size())) return 1;
   if (dupload(h, &g.fct_skip, skip.data(), skip.size())) return 1;
-  // element blocks of the fused record + face-flux kernels: the largest
-  // block size (at most 256 elements) whose blocks all use at most 256
-  // distinct nodes
+  // element blocks of the fused record + face-flux kernels: recursive
+  // coordinate bisection of the element centroids (each range halved at the
+  // median of its longer extent) down to blocks of at most FV_CVB elements
+  // that use at most FV_CVB distinct nodes -- compact tiles, so that few of
+  // a block's nodes are formed again by a neighbouring block.  Blocks in the
+  // bisection's depth-first order (neighbouring tiles in neighbouring
+  // blocks), each block's elements ascending.
   {
-    int eb = 256;
-    std::vector<int> bnn, bnodes, loc;
-    for (;; eb -= 8) {
-      if (eb < 8) return fail("no element block size keeps the fused kernels' node sets within 256");
-      const int nblk = (N + eb - 1) / eb;
-      bnn.assign(nblk, 0);
-      bnodes.assign((size_t)nblk * 256, 0);
-      loc.assign((size_t)N + 1, 0);
-      bool ok = true;
-      std::vector<int> mark_blk(M + 1, -1), mark_loc(M + 1, 0);   // block that listed the node, its slot
-      for (int b = 0; b < nblk && ok; ++b) {
-        int nn = 0;
-        const int e0 = b * eb + 1, e1 = std::min(N, (b + 1) * eb);
-        for (int e = e0; e <= e1 && ok; ++e) {
-          int packed = 0;
-          for (int j = 0; j < 3 && ok; ++j) {
-            const int nd = md->nv[(size_t)j * (N + 1) + e];
-            if (mark_blk[nd] != b) {
-              if (nn >= 256) { ok = false; break; }
-              bnodes[(size_t)b * 256 + nn] = nd;
-              mark_blk[nd] = b;
-              mark_loc[nd] = nn++;
-            }
-            packed |= mark_loc[nd] << (8 * j);
-          }
-          loc[e] = packed;
+    std::vector<int> ids(N);
+    for (int e = 1; e <= N; ++e) ids[e - 1] = e;
+    std::vector<int> mark(M + 1, -1);
+    int stamp = 0;
+    auto distinct_nodes = [&](int lo, int hi) {
+      ++stamp;
+      int n = 0;
+      for (int q = lo; q < hi; ++q)
+        for (int j = 0; j < 3; ++j) {
+          const int nd = md->nv[(size_t)j * (N + 1) + ids[q]];
+          if (mark[nd] != stamp) { mark[nd] = stamp; ++n; }
         }
-        bnn[b] = nn;
+      return n;
+    };
+    std::vector<std::pair<int, int>> stack{{0, N}}, leaves;
+    while (!stack.empty()) {
+      const std::pair<int, int> r = stack.back();
+      stack.pop_back();
+      const int lo = r.first, hi = r.second, n = hi - lo;
+      if (n <= FV_CVB && distinct_nodes(lo, hi) <= FV_CVB) { leaves.push_back(r); continue; }
+      double x0 = md->xc[ids[lo]], x1 = x0, y0 = md->yc[ids[lo]], y1 = y0;
+      for (int q = lo + 1; q < hi; ++q) {
+        const double x = md->xc[ids[q]], y = md->yc[ids[q]];
+        x0 = std::min(x0, x); x1 = std::max(x1, x); y0 = std::min(y0, y); y1 = std::max(y1, y);
       }
-      if (ok) { g.cvb_nblk = nblk; break; }
+      const double* key = (x1 - x0) >= (y1 - y0) ? md->xc : md->yc;
+      const int mid = lo + (n + 1) / 2;
+      std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+                       [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+      stack.push_back({mid, hi});
+      stack.push_back({lo, mid});
     }
-    g.cvb_eb = eb;
+    const int nblk = (int)leaves.size();
+    std::vector<int> bnn(nblk, 0), bne(nblk, 0), bnodes((size_t)nblk * FV_CVB, 1), belems((size_t)nblk * FV_CVB, 1);
+    std::vector<int> loc((size_t)N + 1, 0), nloc(M + 1, 0);
+    for (int b = 0; b < nblk; ++b) {
+      const int lo = leaves[b].first, hi = leaves[b].second;
+      std::sort(ids.begin() + lo, ids.begin() + hi);
+      ++stamp;
+      int nn = 0;
+      for (int q = lo; q < hi; ++q) {
+        const int e = ids[q];
+        belems[(size_t)b * FV_CVB + (q - lo)] = e;
+        int packed = 0;
+        for (int j = 0; j < 3; ++j) {
+          const int nd = md->nv[(size_t)j * (N + 1) + e];
+          if (mark[nd] != stamp) {
+            mark[nd] = stamp;
+            nloc[nd] = nn;
+            bnodes[(size_t)b * FV_CVB + nn++] = nd;
+          }
+          packed |= nloc[nd] << (8 * j);
+        }
+        loc[e] = packed;
+      }
+      bnn[b] = nn;
+      bne[b] = hi - lo;
+    }
+    g.cvb_nblk = nblk;
     std::vector<int> keep(M + 1, 0);
     for (int i = 0; i < md->iobcn; ++i) {
       const int j = md->i_obc_n[i];
@@ -630,8 +651,73 @@ int build_lists(fvgpu_handle* h) {
     }
     if (dupload(h, &g.cvb_nn, bnn.data(), bnn.size())) return 1;
     if (dupload(h, &g.cvb_nodes, bnodes.data(), bnodes.size())) return 1;
+    if (dupload(h, &g.cvb_ne, bne.data(), bne.size())) return 1;
+    if (dupload(h, &g.cvb_elems, belems.data(), belems.size())) return 1;
     if (dupload(h, &g.cvb_loc, loc.data(), loc.size())) return 1;
     if (dupload(h, &g.rec_keep, keep.data(), keep.size())) return 1;
+  }
+  // element blocks of the blocked ADVECTION / ADV_UV kernels: the largest
+  // block size (at most 128 elements, two lanes each, a multiple of 8)
+  // whose blocks all need at most 256 records (their own elements and the
+  // other elements of their edges) and at most 256 edges
+  {
+    std::vector<int> bns, belems, bne, bedges, beloc, epos;
+    std::vector<int> rslot(N + 1, -1), rblk(N + 1, -1), eslot(NE + 1, -1), eblk(NE + 1, -1);
+    int eb = 128;
+    for (;;) {
+      if (eb < 8) return fail("no element block size keeps the blocked momentum kernels within 256 records");
+      const int nblk = (N + eb - 1) / eb;
+      bns.assign(nblk, 0); bne.assign(nblk, 0);
+      belems.assign((size_t)nblk * 256, 0); bedges.assign((size_t)nblk * 256, 1); beloc.assign((size_t)nblk * 256, 0);
+      epos.assign((size_t)N + 1, 0);
+      std::fill(rblk.begin(), rblk.end(), -1);
+      std::fill(eblk.begin(), eblk.end(), -1);
+      int worst = 0;
+      for (int b = 0; b < nblk; ++b) {
+        const int e0 = b * eb + 1, e1 = std::min(N, (b + 1) * eb);
+        int ns = 0, ne = 0;
+        auto slot_of = [&](int x) {
+          if (rblk[x] != b) {
+            if (ns < 256) belems[(size_t)b * 256 + ns] = x;
+            rblk[x] = b;
+            rslot[x] = ns++;
+          }
+          return rslot[x];
+        };
+        for (int e = e0; e <= e1; ++e) slot_of(e);   // own elements first, in order
+        for (int e = e0; e <= e1; ++e) {
+          int packed = 0;
+          for (int q = 0; q < 3; ++q) {
+            const int c = ed[3 * (size_t)e + q] >> 1;
+            if (eblk[c] != b) {
+              const int ia = md->iec[c], ib = md->iec[(size_t)(NE + 1) + c];
+              const int sa = slot_of(ia), sb = slot_of(ib);
+              if (ne < 256) {
+                bedges[(size_t)b * 256 + ne] = c;
+                beloc[(size_t)b * 256 + ne] = (sa & 255) | ((sb & 255) << 8);
+              }
+              eblk[c] = b;
+              eslot[c] = ne++;
+            }
+            packed |= (eslot[c] & 255) << (8 * q);
+          }
+          epos[e] = packed;
+        }
+        bns[b] = ns;
+        bne[b] = ne;
+        worst = std::max(worst, std::max(ns, ne));
+      }
+      if (worst <= 256) { g.ub_nblk = nblk; break; }
+      // shrink in proportion, to a multiple of 8, by at least 8
+      eb = std::min(eb - 8, (int)((long)eb * 256 / worst) / 8 * 8);
+    }
+    g.ub_eb = eb;
+    if (dupload(h, &g.ub_ns, bns.data(), bns.size())) return 1;
+    if (dupload(h, &g.ub_elems, belems.data(), belems.size())) return 1;
+    if (dupload(h, &g.ub_ne, bne.data(), bne.size())) return 1;
+    if (dupload(h, &g.ub_edges, bedges.data(), bedges.size())) return 1;
+    if (dupload(h, &g.ub_eloc, beloc.data(), beloc.size())) return 1;
+    if (dupload(h, &g.ub_epos, epos.data(), epos.size())) return 1;
   }
   // element -> rivers (NBVE(J,1) then NBVE(J,NTVE(J)) of the river node)
   std::vector<std::vector<int>> re(N + 1);
@@ -786,14 +872,12 @@ int upload_state(fvgpu_handle* h) {
   RW(gx, nullptr, mk);
   // flux buffers sized for FP64 (the mixed form uses the first half)
   {
-    EdgeFlux4T<double>* p2 = nullptr; EdgeFlux4T<double>* p3 = nullptr; CvRecT<double>* pr = nullptr;
-    double *pv = nullptr, *pc = nullptr;
+    EdgeFlux4T<double>* p2 = nullptr; CvRecT<double>* pr = nullptr;
+    double* pc = nullptr;
     if (dalloc(h, &p2, (size_t)g.NE + 1)) return 1;
-    if (dalloc(h, &p3, ((size_t)g.NE + 1) * g.KB)) return 1;
-    if (dalloc(h, &pv, ((size_t)g.NE + 1) * g.KB)) return 1;
     if (dalloc(h, &pr, mk)) return 1;
     if (dalloc(h, &pc, ((size_t)g.NCV + 1) * g.KB)) return 1;
-    a.ef2 = p2; a.ef3 = p3; a.efvisc = pv; a.rec = pr; a.cvflux = pc;
+    a.ef2 = p2; a.rec = pr; a.cvflux = pc;
   }
   RW(elij2, nullptr, (size_t)g.NE + 1);
   if (dalloc(h, &a.vhpair, mk)) return 1;
@@ -887,11 +971,9 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   h->precision = precision;
   HIPCHK(hipGetDevice(&h->device));
   if (!scratch_of(h->device)) { delete h; return 1; }
-  // FP64: element-centric ADVECTION/ADV_UV and records + face fluxes in
-  // one launch; the FP32 flux form keeps the two-launch forms of both
-  // (measured faster for it: its higher occupancy there outweighs the
-  // intermediate buffers' traffic)
-  h->two_phase_uv = precision == FV_MIXED ? 1 : 0;
+  // FP64: CV records + face fluxes in one launch; the FP32 flux form keeps
+  // the two launches (measured faster for it: its records are half the
+  // size, and the two kernels' higher occupancy outweighs their traffic)
   h->two_phase_cv = precision == FV_MIXED ? 1 : 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
   h->cur = h->stream;
@@ -1128,8 +1210,8 @@ int fvgpu_stream_copy(fvgpu_handle* h, int64_t n, int32_t reps, double* avg_us) 
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *avg_us = 1000.0 * ms / reps;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   HIPCHK(hipFree(a));
   HIPCHK(hipFree(b));
   return 0;
@@ -1142,15 +1224,15 @@ int fvgpu_synchronize(fvgpu_handle* h) {
 
 void fvgpu_destroy(fvgpu_handle* h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  if (h->stream2) hipStreamSynchronize(h->stream2);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   drop_graphs(h);
-  for (hipEvent_t e : h->event_pool) hipEventDestroy(e);
-  for (void* p : h->allocs) hipFree(p);
-  if (h->ev_fork) hipEventDestroy(h->ev_fork);
-  if (h->ev_join) hipEventDestroy(h->ev_join);
-  if (h->stream2) hipStreamDestroy(h->stream2);
-  if (h->stream) hipStreamDestroy(h->stream);
+  for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -1474,8 +1556,7 @@ int fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode) {
 
 int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
   static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
-                               "cvflux_q", "adv_ts", "adv_q", "uvedge_adv", "uvedge_uv", "adv3d", "advuv",
-                               "adv3d_elem", "advuv_elem", "cvrecflux_ts", "cvrecflux_q", "vertvl",
+                               "cvflux_q", "adv_ts", "adv_q", "vertvl",
                                "viscofh", "vdif_q", "vdif_ts", "n2e3d", "adjust1_brough", "adjust2", "baropg",
                                "wreal", "uv_finish", "copy", "closure_features", "closure_apply"};
   static const int cands[] = {64, 128, 256};

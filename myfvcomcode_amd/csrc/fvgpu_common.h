@@ -18,6 +18,7 @@
 #endif
 #define FV_MAX_CV 32   // control-volume sub-edges per node (twice the node degree)
 #define FV_MAX_RING 16 // NTSN / NTVE bound of the node rings held in registers
+#define FV_CVB 128     // elements and nodes per block of the fused CV kernels
 
 struct DevGrid {
   int M, N, NE, NCV, KB, KBM1, KBM2, IOBCN, NUMQBC, NTIDE;
@@ -66,13 +67,22 @@ struct DevGrid {
   const int *re_off, *re_riv;      // element -> rivers whose NBVE(1) / NBVE(NTVE) it is
   const int *fct_skip;             // node skipped by FCT (OBC or river node)
   // Element blocks of the fused record + face-flux kernels: block b holds
-  // elements b*cvb_eb+1 .. and the cvb_nn[b] (<= 256) distinct nodes they
-  // use, cvb_nodes[256*b + t]; cvb_loc[e] packs the local indices of
-  // NV(e,1..3) in bits 0-7, 8-15, 16-23.  rec_keep[i] = 1 for the nodes
-  // whose records other kernels read at every layer (the OBC nodes and the
-  // other ends of their sub-edges).
-  int cvb_eb, cvb_nblk;
-  const int *cvb_nn, *cvb_nodes, *cvb_loc, *rec_keep;
+  // the cvb_ne[b] (<= FV_CVB) elements cvb_elems[FV_CVB*b + t] and the
+  // cvb_nn[b] (<= FV_CVB) distinct nodes they use, cvb_nodes[FV_CVB*b + t];
+  // cvb_loc[e] packs the local indices of NV(e,1..3) in bits 0-7, 8-15,
+  // 16-23.  rec_keep[i] = 1 for the nodes whose records other kernels read
+  // at every layer (the OBC nodes and the other ends of their sub-edges).
+  int cvb_nblk;
+  const int *cvb_nn, *cvb_nodes, *cvb_ne, *cvb_elems, *cvb_loc, *rec_keep;
+  // Element blocks of the blocked ADVECTION / ADV_UV kernels: block b owns
+  // elements b*ub_eb+1 .. (ub_eb <= 128; record slots 0 ..), followed by
+  // the other elements of their edges: ub_ns[b] (<= 256) records, element
+  // ub_elems[256*b + t]; its ub_ne[b] (<= 256) edges ub_edges[256*b + t],
+  // with the record slots of IA and IB in bits 0-7 and 8-15 of
+  // ub_eloc[256*b + t]; ub_epos[e] packs the edge slots of element e's
+  // three edges (in ed_ent order) in bits 0-7, 8-15, 16-23.
+  int ub_eb, ub_nblk;
+  const int *ub_ns, *ub_elems, *ub_ne, *ub_edges, *ub_eloc, *ub_epos;
   // OBC / tide / rivers
   const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc;
   const double *fluxf_obc, *dltn_obc, *emean, *apt, *phai, *period;

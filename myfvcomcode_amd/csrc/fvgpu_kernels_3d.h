@@ -28,8 +28,6 @@ struct IntArgs {
   // flux buffers, of the flux type FT (double, or float in the mixed form)
   void* ef2;             // per-edge ADVAVE terms (2-D), EdgeFlux4T<FT>
   double* elij2;         // per-edge mean elevation (2-D)
-  void* ef3;             // per edge-layer momentum terms (3-D), (NE+1) x KB EdgeFlux4T<FT>
-  void* efvisc;          // per edge-layer horizontal viscosity of ADV_UV, (NE+1) x KB FT
   void* rec;             // node-layer records of the field being advected, CvRecT<FT>
   double2* vhpair;       // node-layer pair scratch: VDIF_Q's elimination coefficients
   double* obcsum;        // BCOND_TS column sums, 3 per OBC node

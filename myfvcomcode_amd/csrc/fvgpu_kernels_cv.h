@@ -57,19 +57,39 @@ __device__ __forceinline__ void ring_geo_load(const DevGrid& g, int i, RingGeo<R
 // kind 0: tracer, VHN = VISCOFH*NN_HVC; kind 1: turbulence, VHN = the
 // DZ-weighted interface viscosity (adv_q.F) times NN_HVC.  The sums are
 // FP64; the record is stored in the flux type FT.
-template <int KIND, int RMAX, typename FT>
-__device__ __forceinline__ CvRecT<FT> cv_record(const DevGrid& g, const IntArgs& a, int i, const RingGeo<RMAX>& rg,
-                                                double art2, double nn, const double* f, const double* mean,
-                                                double cmean, int k) {
+//
+// The field's values the record reads (the ring's and the node's own) are
+// loaded by cv_load, so that a caller can issue them a layer ahead.
+template <int RMAX>
+struct CvLoads {
+  double fv[RMAX];
+  double c;
+};
+
+template <int RMAX>
+__device__ __forceinline__ void cv_load(const DevGrid& g, int i, const RingGeo<RMAX>& rg, const double* f, int k,
+                                        CvLoads<RMAX>& L) {
   const long ko = (long)(k - 1) * g.m1;
-  const int nt = rg.nt;
   // the ring's values are loaded unconditionally (slots past the ring
-  // read node 0) so that the loads issue together; the sums take the
-  // ring's own terms in ring order
-  double fv[RMAX], mv[RMAX];
+  // read node 0) so that the loads issue together
   static_for<RMAX>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
-    fv[j] = f[ko + rg.rn[j]];
+    L.fv[j] = f[ko + rg.rn[j]];
+  });
+  L.c = f[ko + i];
+}
+
+template <int KIND, int RMAX, typename FT>
+__device__ __forceinline__ CvRecT<FT> cv_record_of(const DevGrid& g, const IntArgs& a, int i, const RingGeo<RMAX>& rg,
+                                                   double art2, double nn, const CvLoads<RMAX>& L,
+                                                   const double* mean, double cmean, int k) {
+  const long ko = (long)(k - 1) * g.m1;
+  const int nt = rg.nt;
+  const double* fv = L.fv;
+  // the sums take the ring's own terms in ring order
+  double mv[RMAX];
+  static_for<RMAX>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
     mv[j] = mean ? mean[ko + rg.rn[j]] : cmean;
   });
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
@@ -92,7 +112,7 @@ __device__ __forceinline__ CvRecT<FT> cv_record(const DevGrid& g, const IntArgs&
       m1v = m2v;
     }
   });
-  const double c = f[ko + i];
+  const double c = L.c;
   double vhn;
   if (KIND == 0) {
     vhn = a.viscofh[ko + i] * nn;
@@ -111,6 +131,15 @@ __device__ __forceinline__ CvRecT<FT> cv_record(const DevGrid& g, const IntArgs&
   r.gyd = (FT)(s4 / art2);
   r.vhn = (FT)vhn;
   return r;
+}
+
+template <int KIND, int RMAX, typename FT>
+__device__ __forceinline__ CvRecT<FT> cv_record(const DevGrid& g, const IntArgs& a, int i, const RingGeo<RMAX>& rg,
+                                                double art2, double nn, const double* f, const double* mean,
+                                                double cmean, int k) {
+  CvLoads<RMAX> L;
+  cv_load<RMAX>(g, i, rg, f, k, L);
+  return cv_record_of<KIND, RMAX, FT>(g, a, i, rg, art2, nn, L, mean, cmean, k);
 }
 
 // Ring gradient / bounds of field f at node i for the layers [k0, k1], one
@@ -433,68 +462,110 @@ __global__ __launch_bounds__(256) void k_cvflux_q(DevGrid g, IntArgs a) {
   }
 }
 
+// The block's records of one layer in LDS, one array per quantity.
+template <typename FT>
+struct CvRecLds {
+  FT f[FV_CVB], gx[FV_CVB], gy[FV_CVB], fmin[FV_CVB], fmax[FV_CVB], gxd[FV_CVB], gyd[FV_CVB], vhn[FV_CVB];
+  __device__ __forceinline__ void put(int s, const CvRecT<FT>& r) {
+    f[s] = r.f; gx[s] = r.gx; gy[s] = r.gy; fmin[s] = r.fmin;
+    fmax[s] = r.fmax; gxd[s] = r.gxd; gyd[s] = r.gyd; vhn[s] = r.vhn;
+  }
+  __device__ __forceinline__ CvRecT<FT> get(int s) const {
+    CvRecT<FT> r;
+    r.f = f[s]; r.gx = gx[s]; r.gy = gy[s]; r.fmin = fmin[s];
+    r.fmax = fmax[s]; r.gxd = gxd[s]; r.gyd = gyd[s]; r.vhn = vhn[s];
+    return r;
+  }
+};
+
 // ADV_T/ADV_S (KIND 0, layers 1..KBM1) or ADV_Q (KIND 1, interfaces
-// 2..KBM1) phases 1 and 2 in one launch: a workgroup takes a block of
-// consecutive elements and the distinct nodes they use (at most 256,
-// cvb_*), and per layer forms the nodes' records into LDS (one thread per
-// node, ring in registers: cv_record, the same arithmetic as k_cvrec) and
-// then the elements' three face fluxes from them (one thread per element:
-// cv_faces, as k_cvflux_*).  A node shared by two blocks is formed by both;
-// the records cross HBM only for the nodes whose records other kernels read
-// (rec_keep: OBC nodes and their sub-edge neighbours) and, for the tracer,
-// at the bottom layer (the bottom gradients of VDIF_TS).
+// 2..KBM1) phases 1 and 2 in one launch.  A workgroup of 256 lanes takes a
+// block of at most FV_CVB = 128 elements and the at most 128 distinct nodes
+// they use (cvb_*: compact tiles of the mesh).  Its first two waves form
+// the nodes' records (one lane per node, ring in registers: cv_record, the
+// same arithmetic as k_cvrec) of layer k into one of two LDS buffers while
+// its last two waves evaluate the elements' three face fluxes (one lane per
+// element: cv_faces, as k_cvflux_*) of layer k-1 from the other buffer; one
+// barrier per layer hands the buffers over.  The two roles hold only their
+// own registers and run side by side.  A node shared by two blocks is
+// formed by both; the records cross HBM only for the nodes whose records
+// other kernels read (rec_keep: OBC nodes and their sub-edge neighbours)
+// and, for the tracer, at the bottom layer (the bottom gradients of
+// VDIF_TS).
 template <int KIND, int RMAX, typename FT>
 __global__ __launch_bounds__(256) void k_cvrecflux(DevGrid g, IntArgs a, const double* fld, const double* mean,
                                                    double cmean) {
-  __shared__ CvRecT<FT> srec[256];
+  __shared__ CvRecLds<FT> srec[2];
   const int b = xcd_block(), t = threadIdx.x;
-  const int nn = g.cvb_nn[b];
-  CvRecT<FT>* rec = (CvRecT<FT>*)a.rec;
-  FT* cvflux = (FT*)a.cvflux;
-  // node part: local node t of the block
-  const bool nact = t < nn;
-  const int i = nact ? g.cvb_nodes[256L * b + t] : 1;
-  RingGeo<RMAX> rg;
-  ring_geo_load<RMAX>(g, i, rg);
-  const double art2 = g.art2[i], nnv = g.nn_hvc[i];
-  const bool keep = nact && g.rec_keep[i] != 0;
-  // element part: element b*EB + t + 1
-  const int e = b * g.cvb_eb + t + 1;
-  const bool eact = t < g.cvb_eb && e <= g.N;
-  const int ee = eact ? e : 1;
-  const ElemFaces<FT> f = elem_faces<FT>(g, ee);
-  const int pk = g.cvb_loc[ee];
-  const int l0 = pk & 255, l1 = (pk >> 8) & 255, l2 = (pk >> 16) & 255;
-  const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu, one = (FT)1.0;
-  const double dt1 = a.dt1[ee];
-  const long cs = (long)g.NCV + 1, p0 = 3L * (ee - 1) + 1;
-  double um = a.u[ee], vm = a.v[ee];   // layer 1 (ADV_Q's interface interpolation)
   const int k0 = KIND == 0 ? 1 : 2;
-  for (int k = k0; k <= g.KBM1; ++k) {
-    if (nact) {
-      const CvRecT<FT> r = cv_record<KIND, RMAX, FT>(g, a, i, rg, art2, nnv, fld, mean, cmean, k);
-      srec[t] = r;
-      if (keep || (KIND == 0 && k == g.KBM1)) rec[(long)(k - 1) * g.m1 + i] = r;
-    }
-    __syncthreads();
-    if (eact) {
-      const long eo = (long)(k - 1) * g.n1;
-      const CvRecT<FT> r[3] = {srec[l0], srec[l1], srec[l2]};
-      FT* out = cvflux + (long)(k - 1) * cs + p0;
-      if (KIND == 0) {
-        const FT dtij = (FT)(dt1 * DZ1(e, k));
-        cv_faces<FT, false>(f, r, (FT)a.u[eo + e], (FT)a.v[eo + e], dtij, fact, one, out);
-      } else {
-        const double uk = a.u[eo + e], vk = a.v[eo + e];
-        const double dz1k = DZ1(e, k), dz1m = DZ1(e, k - 1), dzz1 = DZZ1(e, k - 1);
-        const FT uq = (FT)((uk * dz1m + um * dz1k) / (dz1k + dz1m));
-        const FT vq = (FT)((vk * dz1m + vm * dz1k) / (dz1k + dz1m));
-        um = uk;
-        vm = vk;
-        cv_faces<FT, true>(f, r, uq, vq, (FT)(dt1 * dzz1), fact, rhprnu, out);
+  const int nk = g.KBM1 - k0 + 1;   // layers (interfaces) of the pass
+  // The role is taken from the wave's first lane, so that the branch is
+  // uniform over each wave and both roles pass nk + 1 barriers.
+  if (__builtin_amdgcn_readfirstlane(t) < FV_CVB) {
+    // records: local node t of the block
+    CvRecT<FT>* rec = (CvRecT<FT>*)a.rec;
+    const bool nact = t < g.cvb_nn[b];
+    const int i = nact ? g.cvb_nodes[(long)FV_CVB * b + t] : 1;
+    RingGeo<RMAX> rg;
+    ring_geo_load<RMAX>(g, i, rg);
+    const double art2 = g.art2[i], nnv = g.nn_hvc[i];
+    const bool keep = nact && g.rec_keep[i] != 0;
+    // the field's values are loaded a layer ahead (rings of up to 10
+    // nodes; the 16-node form has no registers to spare)
+    constexpr bool ahead = RMAX <= 10;
+    CvLoads<RMAX> L;
+    if (ahead) cv_load<RMAX>(g, i, rg, fld, k0, L);
+    for (int s = 0; s <= nk; ++s) {
+      const int k = k0 + s;
+      if (s < nk) {
+        if (nact) {
+          if (!ahead) cv_load<RMAX>(g, i, rg, fld, k, L);
+          const CvRecT<FT> r = cv_record_of<KIND, RMAX, FT>(g, a, i, rg, art2, nnv, L, mean, cmean, k);
+          srec[s & 1].put(t, r);
+          if (keep || (KIND == 0 && k == g.KBM1)) rec[(long)(k - 1) * g.m1 + i] = r;
+        }
+        if (ahead && s + 1 < nk) cv_load<RMAX>(g, i, rg, fld, k + 1, L);
       }
+      __syncthreads();
     }
-    __syncthreads();
+  } else {
+    // faces: local element t - FV_CVB of the block
+    FT* cvflux = (FT*)a.cvflux;
+    const int te = t - FV_CVB;
+    const bool eact = te < g.cvb_ne[b];
+    const int e = eact ? g.cvb_elems[(long)FV_CVB * b + te] : 1;
+    const ElemFaces<FT> f = elem_faces<FT>(g, e);
+    const int pk = g.cvb_loc[e];
+    const int l0 = pk & 255, l1 = (pk >> 8) & 255, l2 = (pk >> 16) & 255;
+    const FT fact = (FT)g.fact, rhprnu = (FT)g.hprnu, one = (FT)1.0;
+    const double dt1 = a.dt1[e];
+    const long cs = (long)g.NCV + 1, p0 = 3L * (e - 1) + 1;
+    double um = a.u[e], vm = a.v[e];   // layer 1 (ADV_Q's interface interpolation)
+    for (int s = 0; s <= nk; ++s) {
+      if (s > 0) {
+        const int k = k0 + s - 1;
+        const long eo = (long)(k - 1) * g.n1;
+        const double uk = a.u[eo + e], vk = a.v[eo + e];
+        const double dz1k = DZ1(e, k);
+        if (eact) {
+          const CvRecLds<FT>& sr = srec[(s - 1) & 1];
+          const CvRecT<FT> r[3] = {sr.get(l0), sr.get(l1), sr.get(l2)};
+          FT* out = cvflux + (long)(k - 1) * cs + p0;
+          if (KIND == 0) {
+            const FT dtij = (FT)(dt1 * dz1k);
+            cv_faces<FT, false>(f, r, (FT)uk, (FT)vk, dtij, fact, one, out);
+          } else {
+            const double dz1m = DZ1(e, k - 1), dzz1 = DZZ1(e, k - 1);
+            const FT uq = (FT)((uk * dz1m + um * dz1k) / (dz1k + dz1m));
+            const FT vq = (FT)((vk * dz1m + vm * dz1k) / (dz1k + dz1m));
+            um = uk;
+            vm = vk;
+            cv_faces<FT, true>(f, r, uq, vq, (FT)(dt1 * dzz1), fact, rhprnu, out);
+          }
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 

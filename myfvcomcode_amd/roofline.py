@@ -103,7 +103,7 @@ def _cvrecflux(s, layers, nfields, uv_layers, bottom_records, real):
     M, N, NCV, RSN = s["M"], s["N"], s["NCV"], s["RSN"]
     reads = (RSN * (I4 + 2 * real) + M * (2 * I4 + 2 * real)   # ring lists and factors, NTSN, ART2, NN_HVC
              + nfields * M * layers * real                     # field, mean, VISCOFH
-             + N * I4 + M * I4                                 # block node lists, element local indices
+             + 2 * N * I4 + M * 13 // 10 * I4                  # block element lists and local indices; node lists (about 1.3 per node)
              + NCV * (3 * I4 + 6 * real) + N * real            # sub-edge geometry, DT1
              + N * uv_layers * 2 * real)                       # U, V
     writes = NCV * layers * real + (M * 8 * real if bottom_records else 0)
@@ -141,49 +141,15 @@ def adv_q(s, real=R8):
     return reads + writes
 
 
-def _uvedge(s, real, visc):
-    M, N, NE, KBM1 = s["M"], s["N"], s["NE"], s["KBM1"]
-    reads = (NE * (5 * I4 + 4 * real)                  # IEC, IENODE, ISBC; XIJC, YIJC, DLTXC, DLTYC
-             + N * (3 * I4 + 12 * real)                # NBE; A1U, A2U, XC, YC, ART, CC_HVC
-             + M * real                                # DT
-             + N * KBM1 * 2 * real)                    # U, V
-    writes = NE * KBM1 * (4 * real + (real if visc else 0))
-    return reads + writes
-
-
-def uvedge_adv(s, real=R8):
-    """ADVECTION_EDGE_GCN edge terms per edge and layer."""
-    return _uvedge(s, real, False)
-
-
-def uvedge_uv(s, real=R8):
-    """ADV_UV_EDGE_GCN edge terms (and edge viscosity) per edge and layer."""
-    return _uvedge(s, real, True)
-
-
-def adv3d(s, real=R8):
-    """Edge-term gather summed over the column into ADX2D/ADY2D."""
-    N, NE, KBM1 = s["N"], s["NE"], s["KBM1"]
-    return N * (5 * I4 + real) + NE * I4 + NE * KBM1 * 4 * real + 2 * N * real
-
-
-def advuv(s, real=R8):
-    """Edge-term gather + vertical advection + PG + Coriolis -> UF, VF, VISCOFM."""
-    M, N, NE, KB, KBM1 = s["M"], s["N"], s["NE"], s["KB"], s["KBM1"]
-    reads = (N * (5 * I4 + 7 * real) + NE * (3 * I4 + 2 * real) + M * real   # lists, constants, edge geometry, EGF
-             + NE * KBM1 * 5 * real                                          # EdgeFlux4 + viscosity
-             + N * KBM1 * 4 * real + N * KB * real)                          # U, V, DRHOX, DRHOY; W
-    writes = N * KBM1 * 3 * real + N * real
-    return reads + writes
-
-
-def _uv_elem(s, real, uv):
-    """Element-centric ADVECTION / ADV_UV: the element's three edges
-    evaluated by the element (no edge-layer buffer)."""
+def _uv_blk(s, real, uv):
+    """ADVECTION / ADV_UV by element block: the element and edge data, the
+    velocities of each layer once, and the outputs; records and edge terms
+    stay in LDS."""
     M, N, NE, KB, KBM1 = s["M"], s["N"], s["NE"], s["KB"], s["KBM1"]
     reads = (NE * (5 * I4 + 4 * real)                  # IEC, IENODE, ISBC; XIJC, YIJC, DLTXC, DLTYC
              + N * (3 * I4 + 12 * real)                # NBE; A1U, A2U, XC, YC, ART, CC_HVC
              + N * (4 * I4 + 2 * real)                 # edge list, IUCP/ISBCE, river offset; EPOR, ...
+             + N * 7 * I4                              # block tables: records, edges (about two per element each), edge slots
              + M * real                                # DT
              + N * KBM1 * 2 * real)                    # U, V
     if uv:
@@ -195,14 +161,14 @@ def _uv_elem(s, real, uv):
     return reads + writes
 
 
-def adv3d_elem(s, real=R8):
-    """ADVECTION_EDGE_GCN per element column, summed into ADX2D/ADY2D."""
-    return _uv_elem(s, real, False)
+def adv3d_blk(s, real=R8):
+    """ADVECTION_EDGE_GCN by element block, summed into ADX2D/ADY2D."""
+    return _uv_blk(s, real, False)
 
 
-def advuv_elem(s, real=R8):
-    """ADV_UV_EDGE_GCN per element column -> UF, VF, VISCOFM."""
-    return _uv_elem(s, real, True)
+def advuv_blk(s, real=R8):
+    """ADV_UV_EDGE_GCN by element block -> UF, VF, VISCOFM."""
+    return _uv_blk(s, real, True)
 
 
 def vertvl(s, real=R8):
@@ -254,7 +220,7 @@ def n2e3d(s, real=R8):
 FAMILIES = {f.__name__: f for f in (
     ext_elevation, advave_edge, ext_momentum, cvrec_ts, cvrec_q, cvflux_ts, cvflux_q, cvrecflux_ts, cvrecflux_q,
     adv_ts, adv_q,
-    uvedge_adv, uvedge_uv, adv3d, advuv, adv3d_elem, advuv_elem, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts,
+    adv3d_blk, advuv_blk, vertvl, viscofh, vdif_q, vdif_uv, vdif_ts,
     n2e3d)}
 
 
