@@ -374,7 +374,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext, iext == 1);
   TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
-#define VERTVL_CALL(SM) TLAUNCH("vertvl", k_vertvl_r<SM>, g.M + 1, g, a)
+#define VERTVL_CALL(SM) LAUNCH("vertvl", k_vertvl_tile<SM>, g.nt_nblk, 256, g, a)
   FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
   TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
@@ -529,22 +529,14 @@ int build_lists(fvgpu_handle* h) {
     if (smax > FV_MAX_CV) return fail("a node has more control-volume sub-edges than FV_MAX_CV");
     g.cv_max = smax;
     const size_t m1 = (size_t)M + 1, ns = (size_t)std::max(smax, 1) * m1;
-    std::vector<int> le(ns, 0), lel(ns, 0);
-    std::vector<double> sdx(ns, 0.0), sdy(ns, 0.0);
+    std::vector<int> le(ns, 0);
     for (int i = 0; i <= M; ++i)
       for (int j = 0; j < cnt[i]; ++j) {
         const int e = ent[off[i] + j], c = e >> 1;
-        const size_t q = (size_t)j * m1 + i;
-        le[q] = (pos[c] << 1) | (e & 1);
-        lel[q] = md->ntrg[c];
-        sdx[q] = (e & 1) ? md->dltxe[c] : -md->dltxe[c];
-        sdy[q] = (e & 1) ? md->dltye[c] : -md->dltye[c];
+        le[(size_t)j * m1 + i] = (pos[c] << 1) | (e & 1);
       }
     if (dupload(h, &g.cv_cnt, cnt.data(), m1)) return 1;
     if (dupload(h, &g.cvl_ent, le.data(), ns)) return 1;
-    if (dupload(h, &g.cvl_elem, lel.data(), ns)) return 1;
-    if (dupload(h, &g.cvl_sdx, sdx.data(), ns)) return 1;
-    if (dupload(h, &g.cvl_sdy, sdy.data(), ns)) return 1;
   }
   // element -> its three edges sorted by edge index
   std::vector<int> ed(3 * (size_t)(N + 1), 0), efill(N + 1, 0);
@@ -639,6 +631,82 @@ int build_lists(fvgpu_handle* h) {
       bne[b] = hi - lo;
     }
     g.cvb_nblk = nblk;
+    // node tiles (VERTVL): recursive coordinate bisection of the nodes down
+    // to tiles of at most FV_NTN nodes whose elements (every element that
+    // has one of them as a vertex) number at most FV_NTE; each owned node's
+    // sub-edges as positions 3*(local element) + slot in the tile's element
+    // list, sub-edge order kept, side in bit 0
+    {
+      std::vector<int> nids(M);
+      for (int i = 1; i <= M; ++i) nids[i - 1] = i;
+      std::vector<int> emark(N + 1, -1);
+      int estamp = 0;
+      auto elems_around = [&](int lo, int hi) {
+        ++estamp;
+        int n = 0;
+        for (int q = lo; q < hi; ++q) {
+          const int i = nids[q];
+          for (int r = md->nbve_off[i]; r < md->nbve_off[i] + md->ntve[i]; ++r)
+            if (emark[md->nbve[r]] != estamp) { emark[md->nbve[r]] = estamp; ++n; }
+        }
+        return n;
+      };
+      std::vector<std::pair<int, int>> st{{0, M}}, tl;
+      while (!st.empty()) {
+        const std::pair<int, int> r = st.back();
+        st.pop_back();
+        const int lo = r.first, hi = r.second, n = hi - lo;
+        if (n <= FV_NTN && elems_around(lo, hi) <= FV_NTE) { tl.push_back(r); continue; }
+        if (n == 1) return fail("a node has more elements around it than a node tile holds");
+        double x0 = md->vx[nids[lo]], x1 = x0, y0 = md->vy[nids[lo]], y1 = y0;
+        for (int q = lo + 1; q < hi; ++q) {
+          const double x = md->vx[nids[q]], y = md->vy[nids[q]];
+          x0 = std::min(x0, x); x1 = std::max(x1, x); y0 = std::min(y0, y); y1 = std::max(y1, y);
+        }
+        const double* key = (x1 - x0) >= (y1 - y0) ? md->vx : md->vy;
+        const int mid = lo + (n + 1) / 2;
+        std::nth_element(nids.begin() + lo, nids.begin() + mid, nids.begin() + hi,
+                         [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+        st.push_back({mid, hi});
+        st.push_back({lo, mid});
+      }
+      const int ntb = (int)tl.size();
+      const size_t m1 = (size_t)M + 1;
+      std::vector<int> tnown(ntb, 0), tnel(ntb, 0), tnodes((size_t)ntb * FV_NTN, 1), telems((size_t)ntb * FV_NTE, 1);
+      std::vector<int> tlpos((size_t)std::max(g.cv_max, 1) * m1, 0), eloc(N + 1, 0);
+      for (int b = 0; b < ntb; ++b) {
+        const int lo = tl[b].first, hi = tl[b].second;
+        std::sort(nids.begin() + lo, nids.begin() + hi);
+        std::vector<int> els;
+        for (int q = lo; q < hi; ++q) {
+          const int i = nids[q];
+          tnodes[(size_t)b * FV_NTN + (q - lo)] = i;
+          for (int r = md->nbve_off[i]; r < md->nbve_off[i] + md->ntve[i]; ++r) els.push_back(md->nbve[r]);
+        }
+        std::sort(els.begin(), els.end());
+        els.erase(std::unique(els.begin(), els.end()), els.end());
+        for (size_t q = 0; q < els.size(); ++q) {
+          telems[(size_t)b * FV_NTE + q] = els[q];
+          eloc[els[q]] = (int)q;
+        }
+        tnown[b] = hi - lo;
+        tnel[b] = (int)els.size();
+        for (int q = lo; q < hi; ++q) {
+          const int i = nids[q];
+          for (int j = 0; j < cnt[i]; ++j) {
+            const int en = ent[off[i] + j], c = en >> 1;
+            const int e = md->ntrg[c], slot = (pos[c] - 1) % 3;
+            tlpos[(size_t)j * m1 + i] = ((3 * eloc[e] + slot) << 1) | (en & 1);
+          }
+        }
+      }
+      g.nt_nblk = ntb;
+      if (dupload(h, &g.nt_nown, tnown.data(), tnown.size())) return 1;
+      if (dupload(h, &g.nt_nodes, tnodes.data(), tnodes.size())) return 1;
+      if (dupload(h, &g.nt_nel, tnel.data(), tnel.size())) return 1;
+      if (dupload(h, &g.nt_elems, telems.data(), telems.size())) return 1;
+      if (dupload(h, &g.nt_lpos, tlpos.data(), tlpos.size())) return 1;
+    }
     std::vector<int> keep(M + 1, 0);
     for (int i = 0; i < md->iobcn; ++i) {
       const int j = md->i_obc_n[i];

@@ -19,6 +19,8 @@
 #define FV_MAX_CV 32   // control-volume sub-edges per node (twice the node degree)
 #define FV_MAX_RING 16 // NTSN / NTVE bound of the node rings held in registers
 #define FV_CVB 128     // elements and nodes per block of the fused CV kernels
+#define FV_NTN 64      // owned nodes per node tile
+#define FV_NTE 192     // elements per node tile
 
 struct DevGrid {
   int M, N, NE, NCV, KB, KBM1, KBM2, IOBCN, NUMQBC, NTIDE;
@@ -45,14 +47,12 @@ struct DevGrid {
   // gather lists
   const int *cv_off, *cv_ent;      // node -> (cv<<1 | side), sorted by cv
   // the same lists slot-major (entry j of node i at j*m1+i, j < cv_cnt[i]):
-  // consecutive nodes read consecutive addresses for each slot.  cvl_sdx/sdy
-  // are DLTXE/DLTYE negated on side 0 (XFLUX(IA) -= f  ==  XFLUX(IA) += (-f)).
-  // cvl_ent holds (p<<1 | side), p the element-major position of the
-  // sub-edge's flux in the per-layer flux buffers (below).
+  // consecutive nodes read consecutive addresses for each slot.  cvl_ent
+  // holds (p<<1 | side), p the element-major position of the sub-edge's
+  // flux in the per-layer flux buffers (below).
   int cv_max;
   int ring_max, ve_max;            // max NTSN, max NTVE
-  const int *cv_cnt, *cvl_ent, *cvl_elem;
-  const double *cvl_sdx, *cvl_sdy;
+  const int *cv_cnt, *cvl_ent;
   // element -> its 3 sub-edges, slot-major (slot j of element e at
   // j*n1+e), slot j being the sub-edge on the element edge from NV(e,j+1)
   // to NV(e,(j+1)%3+1): the sub-edge index, and 1 if its IA is the second
@@ -74,6 +74,13 @@ struct DevGrid {
   // at every layer (the OBC nodes and the other ends of their sub-edges).
   int cvb_nblk;
   const int *cvb_nn, *cvb_nodes, *cvb_ne, *cvb_elems, *cvb_loc, *rec_keep;
+  // Node tiles (VERTVL): tile b owns the nt_nown[b] (<= FV_NTN) nodes
+  // nt_nodes[FV_NTN*b + t]; nt_elems[FV_NTE*b + t] are the nt_nel[b]
+  // (<= FV_NTE) elements that have one of them as a vertex.  Slot j of
+  // owned node i: nt_lpos[j*m1 + i] = (3*local element + element slot) << 1
+  // | side, in the order of cvl_ent.
+  int nt_nblk;
+  const int *nt_nown, *nt_nodes, *nt_nel, *nt_elems, *nt_lpos;
   // Element blocks of the blocked ADVECTION / ADV_UV kernels: block b owns
   // elements b*ub_eb+1 .. (ub_eb <= 128; record slots 0 ..), followed by
   // the other elements of their edges: ub_ns[b] (<= 256) records, element

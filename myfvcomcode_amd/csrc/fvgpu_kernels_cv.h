@@ -632,92 +632,130 @@ __global__ __launch_bounds__(256) void k_adv_q_gather(DevGrid g, IntArgs a, cons
   }
 }
 
-// VERTVL_EDGE (vertvl_edge.F:73-553): CV divergence per layer (sums over
-// the node's sub-edges in sub-edge order), the column integration of omega
-// stored to WTS as it goes, then the linear mass-error correction and DTFA
-// in a second pass over the column.  Layers are outermost, so one layer of
-// the elements around a wavefront's nodes is gathered at a time and each
-// element-layer value crosses HBM once; the sub-edge elements, their DT1
-// and the signed DLTXE/DLTYE stay in registers (SMAX >= the node's sub-edge
-// count), so the per-layer reads are the U/V gathers alone and issue
-// together.
+// VERTVL_EDGE (vertvl_edge.F:73-553) by node tile: a workgroup of 256 lanes
+// takes a tile of at most FV_NTN = 64 nodes and the at most FV_NTE = 192
+// elements around them (nt_*).  Its first three waves form each element's
+// three sub-edge terms of layer k, DT1*DZ1*(-U*DLTYE + V*DLTXE) as seen from
+// the sub-edge's IB node, into one of two LDS buffers; its last wave sums
+// the owned nodes' sub-edges of layer k-1 from the other buffer in sub-edge
+// order -- adding a term on side IB, subtracting it on side IA, which is the
+// reference's term with the signed DLTXE/DLTYE bit for bit, since negation
+// is exact -- then adds the rivers and integrates omega down the column into
+// WTS; after the last layer the linear mass-error correction of WTS and
+// DTFA (vertvl_edge.F:500-533).  One barrier per layer hands the buffers
+// over; the role is read from the wave's first lane, so that the branch is
+// uniform over each wave and both roles pass KBM1 + 1 barriers.  Each
+// element-layer velocity is loaded once per tile, and the per-layer sums
+// read LDS instead of gathering the elements' velocities again for each of
+// their nodes.
 template <int SMAX>
-__global__ __launch_bounds__(256) void k_vertvl_r(DevGrid g, IntArgs a) {
-  const int i = (int)xcd_thread();
-  if (i > g.M) return;
-  if (i == 0) {
+__global__ __launch_bounds__(256) void k_vertvl_tile(DevGrid g, IntArgs a) {
+  __shared__ double sterm[2][3 * FV_NTE];
+  const int b = xcd_block(), t = threadIdx.x;
+  const int KBM1 = g.KBM1;
+  if (b == 0 && t == 0) {
     for (int k = 1; k <= g.KB; ++k) a.wts[NI(g, 0, k)] = 0.0;
     a.dtfa[0] = a.d[0];
-    return;
   }
-  const int KBM1 = g.KBM1;
-  const int n = g.cv_cnt[i];
-  int ce[SMAX];
-  double dte[SMAX], sdx[SMAX], sdy[SMAX];
-  static_for<SMAX>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    const long q = (long)j * g.m1 + i;
-    ce[j] = j < n ? g.cvl_elem[q] : 0;
-    dte[j] = j < n ? a.dt1[ce[j]] : 0.0;
-    sdx[j] = j < n ? g.cvl_sdx[q] : 0.0;
-    sdy[j] = j < n ? g.cvl_sdy[q] : 0.0;
-  });
-  const bool obc = g.isonb[i] == 2;
-  const int rv = g.node_riv[i];
-  const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
-  double w = 0.0;
-  a.wts[NI(g, i, 1)] = 0.0;
-  for (int k = 1; k <= KBM1; ++k) {
-    const long ko = (long)(k - 1) * g.n1;
-    // Slots in groups of 8: a group's loads are unconditional (slots past
-    // the count read element 0) and issue together, then its terms are
-    // summed in slot order.
-    double x = 0.0;
-    constexpr int CH = 8;
-    static_for<(SMAX + CH - 1) / CH>([&](auto cc) {
-      constexpr int c0 = decltype(cc)::value * CH;
-      if (c0 < n) {
-        double uu[CH], vv[CH], dz[CH];
-        static_for<CH>([&](auto jc) {
-          constexpr int j = c0 + decltype(jc)::value;
-          if constexpr (j < SMAX) {
-            uu[j - c0] = a.u[ko + ce[j]];
-            vv[j - c0] = a.v[ko + ce[j]];
-            dz[j - c0] = DZ1(ce[j], k);
+  if (__builtin_amdgcn_readfirstlane(t) < FV_NTE) {
+    // element terms: local element t of the tile
+    const bool eact = t < g.nt_nel[b];
+    const int e = eact ? g.nt_elems[(long)FV_NTE * b + t] : 1;
+    const double dt1 = a.dt1[e];
+    double dx[3], dy[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      dx[j] = g.ecv_dx[(long)j * g.n1 + e];
+      dy[j] = g.ecv_dy[(long)j * g.n1 + e];
+    }
+    // the velocities and DZ1 of layer k are loaded two layers ahead (every
+    // index is valid for idle lanes), so that their latency overlaps the
+    // barriers of the layers between
+    double u0 = a.u[e], v0 = a.v[e], z0 = DZ1(e, 1);
+    const long k2 = KBM1 > 1 ? g.n1 : 0;
+    double u1 = a.u[k2 + e], v1 = a.v[k2 + e], z1 = DZ1(e, KBM1 > 1 ? 2 : 1);
+    for (int s = 0; s <= KBM1; ++s) {
+      if (s < KBM1) {
+        const double u = u0, v = v0, dz = z0;
+        u0 = u1; v0 = v1; z0 = z1;
+        if (s + 2 < KBM1) {
+          const long kn = (long)(s + 2) * g.n1;
+          u1 = a.u[kn + e]; v1 = a.v[kn + e]; z1 = DZ1(e, s + 3);
+        }
+        if (eact) {
+          const double dij = dt1 * dz;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) sterm[s & 1][3 * t + j] = dij * (-u * dy[j] + v * dx[j]);
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    // column integration: owned node t - FV_NTE of the tile
+    const int tn = t - FV_NTE;
+    const bool nact = tn < g.nt_nown[b];
+    const int i = nact ? g.nt_nodes[(long)FV_NTN * b + tn] : 1;
+    const int n = g.cv_cnt[i];
+    int lp[SMAX];
+    static_for<SMAX>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      lp[j] = j < n ? g.nt_lpos[(long)j * g.m1 + i] : 0;
+    });
+    const bool obc = g.isonb[i] == 2;
+    const int rv = g.node_riv[i];
+    const double art1 = g.art1[i], dd = a.d[i] - a.dt[i];
+    double w = 0.0;
+    if (nact) a.wts[NI(g, i, 1)] = 0.0;
+    for (int s = 0; s <= KBM1; ++s) {
+      if (s > 0 && nact) {
+        const int k = s;
+        const double* st = sterm[(s - 1) & 1];
+        double x = 0.0;
+        static_for<SMAX>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j < n) {
+            const double tm = st[lp[j] >> 1];
+            x = (lp[j] & 1) ? x + tm : x - tm;
           }
         });
-        static_for<CH>([&](auto jc) {
-          constexpr int j = c0 + decltype(jc)::value;
-          if constexpr (j < SMAX) {
-            if (j < n) {
-              const double dij = dte[j] * dz[j - c0];
-              x = x + dij * (-uu[j - c0] * sdy[j] + vv[j - c0] * sdx[j]);
+        double xflux = obc ? 0.0 : x;
+        if (rv >= 0)
+          for (int j = rv; j < g.NUMQBC; ++j)
+            if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+        w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
+        a.wts[NI(g, i, k + 1)] = w;
+      }
+      __syncthreads();
+    }
+    if (nact) {
+      const double wbottom = 0.0;
+      double dtfa = a.d[i];
+      const bool corr = fabs(w - wbottom) > 1.0E-8 && !obc;
+      if (corr) {
+        double tmp1 = a.elf[i] * (double)KBM1 - (w - wbottom) * g.dti / DZ(i, 1);
+        tmp1 = tmp1 / (double)KBM1;
+        dtfa = tmp1 + g.h[i];
+        // eight levels' loads, then their stores, so that the loads of a
+        // batch are in flight together
+        const double dw = w - wbottom;
+        for (int k0 = 2; k0 <= KBM1 + 1; k0 += 8) {
+          double wv[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (k0 + q <= KBM1 + 1) wv[q] = a.wts[NI(g, i, k0 + q)];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int k = k0 + q;
+            if (k <= KBM1 + 1) {
+              const double ratio = (double)((float)(k - 1) / (float)KBM1);
+              a.wts[NI(g, i, k)] = wv[q] - ratio * dw;
             }
           }
-        });
+        }
       }
-    });
-    double xflux = obc ? 0.0 : x;
-    if (rv >= 0)
-      for (int j = rv; j < g.NUMQBC; ++j)
-        if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
-    w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
-    a.wts[NI(g, i, k + 1)] = w;
-  }
-  const double wbottom = 0.0;
-  double dtfa = a.d[i];
-  const bool corr = fabs(w - wbottom) > 1.0E-8 && !obc;
-  if (corr) {
-    double tmp1 = a.elf[i] * (double)KBM1 - (w - wbottom) * g.dti / DZ(i, 1);
-    tmp1 = tmp1 / (double)KBM1;
-    dtfa = tmp1 + g.h[i];
-    for (int k = 2; k <= KBM1 + 1; ++k) {
-      const double ratio = (double)((float)(k - 1) / (float)KBM1);
-      const long q = NI(g, i, k);
-      a.wts[q] = a.wts[q] - ratio * (w - wbottom);
+      a.dtfa[i] = dtfa;
     }
   }
-  a.dtfa[i] = dtfa;
 }
 
 // VISCOF_H (viscofh.F:33-127): nodal strain-rate magnitude from the NBVE

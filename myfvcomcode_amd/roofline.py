@@ -172,11 +172,13 @@ def advuv_blk(s, real=R8):
 
 
 def vertvl(s, real=R8):
-    """VERTVL_EDGE: sub-edge divergence, column integral, correction."""
+    """VERTVL_EDGE by node tile: the elements' sub-edge geometry and U, V,
+    the nodes' sub-edge positions, column integral, correction."""
     M, N, NCV, KB, KBM1 = s["M"], s["N"], s["NCV"], s["KB"], s["KBM1"]
-    reads = (2 * NCV * (I4 + 2 * real) + M * (3 * I4 + 6 * real)
-             + N * real + N * KBM1 * 2 * real)
-    writes = M * KB * real + M * real
+    reads = (2 * NCV * I4 + M * (3 * I4 + 6 * real)   # sub-edge positions; counts, flags, ART1, D, DT, ELF, H
+             + N * (I4 + 7 * real)                    # tile element lists; DT1, DLTXE/DLTYE of 3 sub-edges
+             + N * KBM1 * 2 * real)                   # U, V
+    writes = M * KB * real + M * real                 # WTS, DTFA
     return reads + writes
 
 
