@@ -58,25 +58,30 @@ __device__ __forceinline__ void ring_geo_load(const DevGrid& g, int i, RingGeo<R
 // DZ-weighted interface viscosity (adv_q.F) times NN_HVC.  The sums are
 // FP64; the record is stored in the flux type FT.
 //
-// The field's values the record reads (the ring's and the node's own) are
+// The values the record reads -- the field at the node and its ring, and
+// for the tracer the mean at the ring and VISCOFH at the layer -- are
 // loaded by cv_load, so that a caller can issue them a layer ahead.
+// (ADV_Q's record keeps its mean and its VISCOFH loads, of two levels, in
+// the record: loaded ahead they measured slower.)
 template <int RMAX>
 struct CvLoads {
-  double fv[RMAX];
-  double c;
+  double fv[RMAX], mv[RMAX];
+  double c, vh;
 };
 
-template <int RMAX>
-__device__ __forceinline__ void cv_load(const DevGrid& g, int i, const RingGeo<RMAX>& rg, const double* f, int k,
-                                        CvLoads<RMAX>& L) {
+template <int KIND, int RMAX>
+__device__ __forceinline__ void cv_load(const DevGrid& g, const IntArgs& a, int i, const RingGeo<RMAX>& rg,
+                                        const double* f, const double* mean, double cmean, int k, CvLoads<RMAX>& L) {
   const long ko = (long)(k - 1) * g.m1;
   // the ring's values are loaded unconditionally (slots past the ring
   // read node 0) so that the loads issue together
   static_for<RMAX>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
     L.fv[j] = f[ko + rg.rn[j]];
+    if (KIND == 0) L.mv[j] = mean ? mean[ko + rg.rn[j]] : cmean;
   });
   L.c = f[ko + i];
+  if (KIND == 0) L.vh = a.viscofh[ko + i];
 }
 
 template <int KIND, int RMAX, typename FT>
@@ -90,7 +95,7 @@ __device__ __forceinline__ CvRecT<FT> cv_record_of(const DevGrid& g, const IntAr
   double mv[RMAX];
   static_for<RMAX>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
-    mv[j] = mean ? mean[ko + rg.rn[j]] : cmean;
+    mv[j] = KIND == 0 ? L.mv[j] : (mean ? mean[ko + rg.rn[j]] : cmean);
   });
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
   double f1 = fv[0];
@@ -115,7 +120,7 @@ __device__ __forceinline__ CvRecT<FT> cv_record_of(const DevGrid& g, const IntAr
   const double c = L.c;
   double vhn;
   if (KIND == 0) {
-    vhn = a.viscofh[ko + i] * nn;
+    vhn = L.vh * nn;
   } else {
     const double vfa = (a.viscofh[ko + i] * DZ(i, k - 1) + a.viscofh[ko - g.m1 + i] * DZ(i, k)) /
                        (DZ(i, k) + DZ(i, k - 1));
@@ -138,7 +143,7 @@ __device__ __forceinline__ CvRecT<FT> cv_record(const DevGrid& g, const IntArgs&
                                                 double art2, double nn, const double* f, const double* mean,
                                                 double cmean, int k) {
   CvLoads<RMAX> L;
-  cv_load<RMAX>(g, i, rg, f, k, L);
+  cv_load<KIND, RMAX>(g, a, i, rg, f, mean, cmean, k, L);
   return cv_record_of<KIND, RMAX, FT>(g, a, i, rg, art2, nn, L, mean, cmean, k);
 }
 
@@ -514,17 +519,17 @@ __global__ __launch_bounds__(256) void k_cvrecflux(DevGrid g, IntArgs a, const d
     // nodes; the 16-node form has no registers to spare)
     constexpr bool ahead = RMAX <= 10;
     CvLoads<RMAX> L;
-    if (ahead) cv_load<RMAX>(g, i, rg, fld, k0, L);
+    if (ahead) cv_load<KIND, RMAX>(g, a, i, rg, fld, mean, cmean, k0, L);
     for (int s = 0; s <= nk; ++s) {
       const int k = k0 + s;
       if (s < nk) {
         if (nact) {
-          if (!ahead) cv_load<RMAX>(g, i, rg, fld, k, L);
+          if (!ahead) cv_load<KIND, RMAX>(g, a, i, rg, fld, mean, cmean, k, L);
           const CvRecT<FT> r = cv_record_of<KIND, RMAX, FT>(g, a, i, rg, art2, nnv, L, mean, cmean, k);
           srec[s & 1].put(t, r);
           if (keep || (KIND == 0 && k == g.KBM1)) rec[(long)(k - 1) * g.m1 + i] = r;
         }
-        if (ahead && s + 1 < nk) cv_load<RMAX>(g, i, rg, fld, k + 1, L);
+        if (ahead && s + 1 < nk) cv_load<KIND, RMAX>(g, a, i, rg, fld, mean, cmean, k + 1, L);
       }
       __syncthreads();
     }
