@@ -364,7 +364,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   IntArgs& a = h->ia;
   const long mk = g.m1 * g.KB, nk = g.n1 * g.KB;
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
-  TLAUNCH("adjust1_brough", k_adjust1_brough, g.N + 1, g, a);
+#define ADJ1_CALL(KM) TLAUNCH("adjust1_brough", k_adjust1_brough<KM>, g.N + 1, g, a)
+  FV_KMAX_DISPATCH(g.KBM1, ADJ1_CALL);
   LAUNCH("adv3d_blk", (k_uv_blk<0, FT>), g.ub_nblk, 256, g, a);
   TLAUNCH("baropg", k_baropg, g.N + 1, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, g.M + 1, g, a);
@@ -373,7 +374,8 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   TLAUNCH("zero", k_zero, g.m1, a.egf, g.m1);
   if (g.IOBCN > 0) TLAUNCH("zero", k_zero, g.IOBCN, a.uard_obcn, (long)g.IOBCN);
   for (int iext = 1; iext <= g.isplit; ++iext) enqueue_external_step_t<FT>(h, iext, iext == 1);
-  TLAUNCH("adjust2", k_adjust2, g.N + 1, g, a);
+#define ADJ2_CALL(KM) TLAUNCH("adjust2", k_adjust2<KM>, g.N + 1, g, a)
+  FV_KMAX_DISPATCH(g.KBM1, ADJ2_CALL);
 #define VERTVL_CALL(SM) LAUNCH("vertvl", k_vertvl_tile<SM>, g.nt_nblk, 256, g, a)
   FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
   TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);

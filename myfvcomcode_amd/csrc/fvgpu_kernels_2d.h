@@ -78,30 +78,52 @@ __global__ void k_ext_wind(DevGrid g, const double* u0, const double* v0, const 
   psimple_drag(uu, vv, sx[i], sy[i]);
 }
 
-// Wall projection of a velocity pair (BCOND_GCN cases 2/3/5).
-__device__ __forceinline__ void wall_project(const DevGrid& g, int i, double& u, double& v) {
+// Wall projection of a velocity pair (BCOND_GCN cases 2/3/5): the element's
+// case, found once (wall_prep), then applied to any number of pairs of its
+// column (wall_apply).  Case 3 zeroes the pair; on a solid wall (ISBCE 1)
+// an element that is the first or last of a river node's NBVE ring keeps
+// only the inflowing part along the river, any other the component along
+// the wall.
+struct WallOp {
+  int kind;   // 0: unchanged, 1: river (p, q = COS, SIN of its angle), 2: wall (p, q = SIN, COS of ALPHA), 3: zero
+  double p, q;
+};
+
+__device__ __forceinline__ WallOp wall_prep(const DevGrid& g, int i) {
   const int b = g.isbce[i];
-  if (b == 3) { u = 0.0; v = 0.0; }
-  if (b != 1) return;
+  if (b == 3) return {3, 0.0, 0.0};
+  if (b != 1) return {0, 0.0, 0.0};
   for (int j = 0; j < g.NUMQBC; ++j) {
     const int i1 = g.inodeq[j];
     const int j1 = g.nbve[g.nbve_off[i1]];
     const int j2 = g.nbve[g.nbve_off[i1] + g.ntve[i1] - 1];
-    if (i == j1 || i == j2) {
-      const double ca = g.cos_angleq[j], sa = g.sin_angleq[j];
-      double untmp = u * ca + v * sa;
-      const double vntmp = -u * sa + v * ca;
-      untmp = fmax(untmp, 0.0);
-      u = untmp * ca - vntmp * sa;
-      v = untmp * sa + vntmp * ca;
-      return;
-    }
+    if (i == j1 || i == j2) return {1, g.cos_angleq[j], g.sin_angleq[j]};
   }
-  const double s = g.sin_alpha[i], c = g.cos_alpha[i];
-  const double ui = u * (s * s) - v * s * c;
-  const double vi = -u * s * c + v * (c * c);
-  u = ui;
-  v = vi;
+  return {2, g.sin_alpha[i], g.cos_alpha[i]};
+}
+
+__device__ __forceinline__ void wall_apply(const WallOp& w, double& u, double& v) {
+  if (w.kind == 3) {
+    u = 0.0;
+    v = 0.0;
+  } else if (w.kind == 1) {
+    const double ca = w.p, sa = w.q;
+    double untmp = u * ca + v * sa;
+    const double vntmp = -u * sa + v * ca;
+    untmp = fmax(untmp, 0.0);
+    u = untmp * ca - vntmp * sa;
+    v = untmp * sa + vntmp * ca;
+  } else if (w.kind == 2) {
+    const double s = w.p, c = w.q;
+    const double ui = u * (s * s) - v * s * c;
+    const double vi = -u * s * c + v * (c * c);
+    u = ui;
+    v = vi;
+  }
+}
+
+__device__ __forceinline__ void wall_project(const DevGrid& g, int i, double& u, double& v) {
+  wall_apply(wall_prep(g, i), u, v);
 }
 
 // Spectral tide at OBC node index `obc` (BCOND_ASL, mod_obcs.F:552-582).

@@ -69,24 +69,39 @@ __global__ void k_copy(double* dst, const double* src, long n) {
 
 // ADJUST2D3D(1) (adjust2d3d.F:52-63), BCOND_GCN(5,0) (bcond_gcn.F:537-616)
 // and BOTTOM_ROUGHNESS (brough.F:88-151) for one element column.
-__global__ void k_adjust1_brough(DevGrid g, IntArgs a) {
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_adjust1_brough(DevGrid g, IntArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 1 || i > g.N) return;
+  const int KBM1 = g.KBM1;
+  // the column is read once into registers (KMAX >= KBM1) and written once
+  double uk[KMAX], vk[KMAX];
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) { uk[k - 1] = a.u[EI(g, i, k)]; vk[k - 1] = a.v[EI(g, i, k)]; }
+  });
   double uac = 0.0, vac = 0.0;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    uac = uac + a.u[EI(g, i, k)] * DZ1(i, k);
-    vac = vac + a.v[EI(g, i, k)] * DZ1(i, k);
-  }
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) {
+      uac = uac + uk[k - 1] * DZ1(i, k);
+      vac = vac + vk[k - 1] * DZ1(i, k);
+    }
+  });
   const double du = a.ua[i] - uac, dv = a.va[i] - vac;
+  const WallOp wop = wall_prep(g, i);
   double ub = 0.0, vb = 0.0;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    double uu = a.u[EI(g, i, k)] + 1.0 * du;
-    double vv = a.v[EI(g, i, k)] + 1.0 * dv;
-    wall_project(g, i, uu, vv);
-    a.u[EI(g, i, k)] = uu;
-    a.v[EI(g, i, k)] = vv;
-    ub = uu; vb = vv;
-  }
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) {
+      double uu = uk[k - 1] + 1.0 * du;
+      double vv = vk[k - 1] + 1.0 * dv;
+      wall_apply(wop, uu, vv);
+      a.u[EI(g, i, k)] = uu;
+      a.v[EI(g, i, k)] = vv;
+      ub = uu; vb = vv;
+    }
+  });
   const double vk2 = .160;
   const double zb = ZZ1(i, g.KBM1) - Z1(i, g.KB);
   double cbc;
@@ -192,7 +207,8 @@ __global__ void k_advave_int(DevGrid g, IntArgs a) {
 }
 
 // ADJUST2D3D(2) (adjust2d3d.F:65-110).
-__global__ void k_adjust2(DevGrid g, IntArgs a) {
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_adjust2(DevGrid g, IntArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > g.N) return;
   const double uard = a.uard[i] / (double)g.isplit;
@@ -200,18 +216,31 @@ __global__ void k_adjust2(DevGrid g, IntArgs a) {
   a.uard[i] = uard;
   a.vard[i] = vard;
   if (i < 1) return;
+  const int KBM1 = g.KBM1;
+  // the column is read once into registers (KMAX >= KBM1) and written once
+  double uk[KMAX], vk[KMAX];
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) { uk[k - 1] = a.u[EI(g, i, k)]; vk[k - 1] = a.v[EI(g, i, k)]; }
+  });
   double utmp = 0.0, vtmp = 0.0;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    utmp = utmp + a.u[EI(g, i, k)] * DZ1(i, k);
-    vtmp = vtmp + a.v[EI(g, i, k)] * DZ1(i, k);
-  }
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) {
+      utmp = utmp + uk[k - 1] * DZ1(i, k);
+      vtmp = vtmp + vk[k - 1] * DZ1(i, k);
+    }
+  });
   const double dt1 = a.dt1[i];
   utmp = utmp * dt1;
   vtmp = vtmp * dt1;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    a.u[EI(g, i, k)] = a.u[EI(g, i, k)] - (utmp - uard) / dt1;
-    a.v[EI(g, i, k)] = a.v[EI(g, i, k)] - (vtmp - vard) / dt1;
-  }
+  static_for<KMAX>([&](auto kc) {
+    constexpr int k = decltype(kc)::value + 1;
+    if (k <= KBM1) {
+      a.u[EI(g, i, k)] = uk[k - 1] - (utmp - uard) / dt1;
+      a.v[EI(g, i, k)] = vk[k - 1] - (vtmp - vard) / dt1;
+    }
+  });
 }
 
 // N2E3D (mod_main.F:1633-1645) over layers 1..kmax, one thread per element
