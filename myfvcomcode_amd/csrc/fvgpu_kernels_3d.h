@@ -134,15 +134,26 @@ __global__ void k_baropg(DevGrid g, IntArgs a, int barotropic) {
     dij[j - 1] = 0.5 * (a.dt[nn[j - 1][0]] + a.dt[nn[j - 1][1]]);
   }
   const double dt1 = a.dt1[i], ge = g.grav_e[i], ramp = a.clk->ramp_int;
+  const int nv[3] = {NV_(i, 1), NV_(i, 2), NV_(i, 3)};
+  // edge j joins nodes NV(J1) and NV(J2) with (J1, J2) = (2,3), (3,1), (1,2)
+  constexpr int e0[3] = {1, 2, 0}, e1[3] = {2, 0, 1};
   double rprev[3] = {0, 0, 0}, d1c[3] = {0, 0, 0}, d2c = 0.0, rhoprev = 0.0, zzprev = 0.0;
   double sx = 0.0, sy = 0.0;
   for (int k = 1; k <= g.KBM1; ++k) {
     const double zzk = ZZ1(i, k);
-    const double rhok = a.rho[EI(g, i, k)] - a.rmean[EI(g, i, k)];
+    const long q = EI(g, i, k);
+    const double rho_k = a.rho[q], rmean_k = a.rmean[q];
+    const double rhok = rho_k - rmean_k;
+    a.rho[q] = rhok + rmean_k;   // the RMEAN round trip of RHO (baropg.F:57, 161)
+    // RHO1 - RMEAN1 at the element's three nodes, each loaded once
+    double rn3[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) rn3[m] = a.rho1[NI(g, nv[m], k)] - a.rmean1[NI(g, nv[m], k)];
     double rij[3];
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const double r0 = a.rho1[NI(g, nn[j][0], k)] - a.rmean1[NI(g, nn[j][0], k)];
-      const double r1 = a.rho1[NI(g, nn[j][1], k)] - a.rmean1[NI(g, nn[j][1], k)];
+      const double r0 = rn3[e0[j]];
+      const double r1 = rn3[e1[j]];
       rij[j] = 0.5 * (r0 + r1);
       if (k == 1) d1c[j] = rij[j] * (-zzk);
       else {
@@ -176,10 +187,8 @@ __global__ void k_baropg(DevGrid g, IntArgs a, int barotropic) {
   }
   a.drx2d[i] = sx;
   a.dry2d[i] = sy;
-  for (int k = 1; k <= g.KB; ++k) {
-    const long q = EI(g, i, k);
-    a.rho[q] = (a.rho[q] - a.rmean[q]) + a.rmean[q];
-  }
+  const long qb = EI(g, i, g.KB);
+  a.rho[qb] = (a.rho[qb] - a.rmean[qb]) + a.rmean[qb];
 }
 
 __global__ void k_rho1_roundtrip(DevGrid g, IntArgs a) {
