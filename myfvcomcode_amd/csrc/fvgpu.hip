@@ -574,6 +574,40 @@ int build_lists(fvgpu_handle* h) {
   // bisection's depth-first order (neighbouring tiles in neighbouring
   // blocks), each block's elements ascending.
   {
+    // connected components of the mesh (the instances of a union of
+    // meshes share coordinates but no node): the bisection starts from one
+    // range per component, so that no tile mixes two of them
+    std::vector<int> ecomp(N + 1, -1);
+    int ncomp = 0;
+    {
+      std::vector<int> queue;
+      for (int s0 = 1; s0 <= N; ++s0) {
+        if (ecomp[s0] >= 0) continue;
+        ecomp[s0] = ncomp;
+        queue.assign(1, s0);
+        for (size_t q = 0; q < queue.size(); ++q) {
+          const int e = queue[q];
+          for (int j = 0; j < 3; ++j) {
+            const int x = md->nbe[(size_t)j * (N + 1) + e];
+            if (x > 0 && ecomp[x] < 0) { ecomp[x] = ncomp; queue.push_back(x); }
+          }
+        }
+        ++ncomp;
+      }
+    }
+    std::vector<int> ncompv(M + 1, 0);
+    for (int e = 1; e <= N; ++e)
+      for (int j = 0; j < 3; ++j) ncompv[md->nv[(size_t)j * (N + 1) + e]] = ecomp[e];
+    // ranges [lo, hi) of ids sorted by component, one per component
+    auto component_ranges = [&](std::vector<int>& idv, const std::vector<int>& comp) {
+      std::stable_sort(idv.begin(), idv.end(), [&](int a, int b) { return comp[a] < comp[b]; });
+      std::vector<std::pair<int, int>> r;
+      int lo = 0;
+      for (int q = 1; q <= (int)idv.size(); ++q)
+        if (q == (int)idv.size() || comp[idv[q]] != comp[idv[lo]]) { r.push_back({lo, q}); lo = q; }
+      std::reverse(r.begin(), r.end());   // popped from the back: first component first
+      return r;
+    };
     std::vector<int> ids(N);
     for (int e = 1; e <= N; ++e) ids[e - 1] = e;
     std::vector<int> mark(M + 1, -1);
@@ -588,7 +622,7 @@ int build_lists(fvgpu_handle* h) {
         }
       return n;
     };
-    std::vector<std::pair<int, int>> stack{{0, N}}, leaves;
+    std::vector<std::pair<int, int>> stack = component_ranges(ids, ecomp), leaves;
     while (!stack.empty()) {
       const std::pair<int, int> r = stack.back();
       stack.pop_back();
@@ -653,7 +687,7 @@ int build_lists(fvgpu_handle* h) {
         }
         return n;
       };
-      std::vector<std::pair<int, int>> st{{0, M}}, tl;
+      std::vector<std::pair<int, int>> st = component_ranges(nids, ncompv), tl;
       while (!st.empty()) {
         const std::pair<int, int> r = st.back();
         st.pop_back();
