@@ -84,6 +84,7 @@ struct fvgpu_handle {
   const fv_model* md = nullptr;
   int precision = 0;
   int device = 0;
+  int cu_count = 0;       // compute units of the device
   hipStream_t stream = nullptr;
   // Independent chains of the step (the 2-D elevation and edge terms, the
   // T and S passes, the Q2 and Q2L advection) fork onto stream2 and join
@@ -1074,6 +1075,7 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   h->md = md;
   h->precision = precision;
   HIPCHK(hipGetDevice(&h->device));
+  HIPCHK(hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device));
   if (!scratch_of(h->device)) { delete h; return 1; }
   // FP64: CV records + face fluxes in one launch; the FP32 flux form keeps
   // the two launches (measured faster for it: its records are half the
@@ -1196,7 +1198,9 @@ int ens_update(fvgpu_handle* h, const char* name, int32_t nrens, const int64_t* 
   HIPCHK(hipMemcpyAsync(dx5, x5, (size_t)nrens * nrens * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(doff, offl.data(), (size_t)nrens * sizeof(long), hipMemcpyHostToDevice, h->stream));
   const long total = (long)len * nl;
-  const unsigned blocks = (unsigned)((total + FV_ENS_TILE - 1) / FV_ENS_TILE);
+  // persistent blocks: two per CU (the 65 KB of LDS each allows two)
+  const long ntiles = (total + FV_ENS_TILE - 1) / FV_ENS_TILE;
+  const unsigned blocks = (unsigned)std::max(1L, std::min(ntiles, 2L * std::max(h->cu_count, 1)));
   hipLaunchKernelGGL(k_ens_update, dim3(blocks), dim3(256), 0, h->stream, arr, (const long*)doff, (long)len,
                      (long)ls, nl, (int)nrens, (const double*)dx5);
   HIPCHK(hipGetLastError());
@@ -1660,7 +1664,7 @@ int fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode) {
 
 int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
   static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
-                               "cvflux_q", "adv_ts", "adv_q", "vertvl",
+                               "cvflux_q", "adv_ts", "adv_q",
                                "viscofh", "vdif_q", "vdif_ts", "n2e3d", "adjust1_brough", "adjust2", "baropg",
                                "wreal", "uv_finish", "copy", "closure_features", "closure_apply"};
   static const int cands[] = {64, 128, 256};

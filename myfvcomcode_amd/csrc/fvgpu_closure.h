@@ -474,12 +474,14 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
 // A <- A*X5, the ensemble update of the EnKF analysis (`multa`, the last
 // step of ANALYSIS, mod_enkfassim.F:2746-3010), over the member slices of
 // one array of a batched union: member i's entries of layer k are
-// arr[k*ls + off[i] + 1 + p], p < len, for nl layers.  A block stages a tile
-// of FV_ENS_TILE positions of every member and X5 in LDS, then writes each
-// member's new values in place.  Each of the 256 threads forms 4 positions
-// x 4 members (register blocking: 8 LDS reads per 16 FMAs); each output is
-// summed over the members in increasing order (sum_i A(p,i)*X5(i,j)), so a
-// plain loop reproduces it.
+// arr[k*ls + off[i] + 1 + p], p < len, for nl layers.  A block loads X5 into
+// LDS once and then walks its tiles of FV_ENS_TILE positions (tile b,
+// b + gridDim.x, ...): it stages the tile of every member in LDS and writes
+// each member's new values in place, loading the next tile's values into
+// registers while it computes the current one.  Each of the 256 threads
+// forms 4 positions x 4 members (register blocking: 8 LDS reads per 16
+// products and sums); each output is summed over the members in increasing
+// order (sum_i A(p,i)*X5(i,j)), so a plain loop reproduces it.
 #define FV_ENS_MAX 64
 #define FV_ENS_TILE 64
 __global__ __launch_bounds__(256) void k_ens_update(double* arr, const long* off, long len, long ls, int nl,
@@ -490,49 +492,67 @@ __global__ __launch_bounds__(256) void k_ens_update(double* arr, const long* off
   for (int t = threadIdx.x; t < nrens * nrens; t += blockDim.x) sx[t] = x5[t];
   if (threadIdx.x < nrens) so[threadIdx.x] = off[threadIdx.x];
   __syncthreads();
-  const long total = len * nl, p0 = (long)blockIdx.x * FV_ENS_TILE;
-  {
-    // 256 threads: position q = t % 64 is the same in every pass, members
-    // t / 64, t / 64 + 4, ...; the layer split of the position once
-    const int q = threadIdx.x % FV_ENS_TILE, i0 = threadIdx.x / FV_ENS_TILE;
-    const long p = p0 + q;
+  const long total = len * nl;
+  const long ntiles = (total + FV_ENS_TILE - 1) / FV_ENS_TILE;
+  // staging: position q = t % 64 of the tile, members t / 64 + 4m
+  const int q = threadIdx.x % FV_ENS_TILE, i0 = threadIdx.x / FV_ENS_TILE;
+  constexpr int NPRE = FV_ENS_MAX / (256 / FV_ENS_TILE);
+  double pre[NPRE];
+  auto load_tile = [&](long tile) {
+    const long p = tile * FV_ENS_TILE + q;
     long base = -1;
     if (p < total) {
       const long k = p / len;
       base = k * ls + 1 + (p - k * len);
     }
-    for (int i = i0; i < nrens; i += 256 / FV_ENS_TILE) sa[i][q] = base >= 0 ? arr[base + so[i]] : 0.0;
-  }
-  __syncthreads();
-  const int tq = threadIdx.x % 16, tj = threadIdx.x / 16;   // positions tq+16r, members tj+16c
-  double acc[4][4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-  for (int i = 0; i < nrens; ++i) {
-    double av[4], xv[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) av[r] = sa[i][tq + 16 * r];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int j = tj + 16 * c;
-      xv[c] = j < nrens ? sx[i + j * nrens] : 0.0;
+    for (int m = 0; m < NPRE; ++m) {
+      const int i = i0 + m * (256 / FV_ENS_TILE);
+      pre[m] = (base >= 0 && i < nrens) ? arr[base + so[i]] : 0.0;
     }
+  };
+  const int tq = threadIdx.x % 16, tj = threadIdx.x / 16;   // positions tq+16r, members tj+16c
+  long tile = blockIdx.x;
+  if (tile < ntiles) load_tile(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();   // the previous tile's products have read sa
+#pragma unroll
+    for (int m = 0; m < NPRE; ++m) {
+      const int i = i0 + m * (256 / FV_ENS_TILE);
+      if (i < nrens) sa[i][q] = pre[m];
+    }
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
+    double acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[r][c] = acc[r][c] + av[r] * xv[c];
-  }
+      for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+    for (int i = 0; i < nrens; ++i) {
+      double av[4], xv[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const long p = p0 + tq + 16 * r;
-    if (p >= total) continue;
-    const long k = p / len, base = k * ls + 1 + (p - k * len);
+      for (int r = 0; r < 4; ++r) av[r] = sa[i][tq + 16 * r];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int j = tj + 16 * c;
-      if (j < nrens) arr[base + so[j]] = acc[r][c];
+      for (int c = 0; c < 4; ++c) {
+        const int j = tj + 16 * c;
+        xv[c] = j < nrens ? sx[i + j * nrens] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = acc[r][c] + av[r] * xv[c];
+    }
+    const long p0 = tile * FV_ENS_TILE;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long p = p0 + tq + 16 * r;
+      if (p >= total) continue;
+      const long k = p / len, base = k * ls + 1 + (p - k * len);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int j = tj + 16 * c;
+        if (j < nrens) arr[base + so[j]] = acc[r][c];
+      }
     }
   }
 }
