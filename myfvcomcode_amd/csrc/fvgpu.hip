@@ -875,6 +875,15 @@ int upload_grid(fvgpu_handle* h) {
   UP(xc, n1); UP(yc, n1); UP(art, n1); UP(h1, n1); UP(cor, n1); UP(cc_hvc, n1); UP(cc_sponge, n1);
   UP(cc_z0b, n1); UP(grav_e, n1); UP(epor, n1); UP(isbce, n1); UP(iucp, n1);
   UP(nv, 3 * n1); UP(nbe, 3 * n1); UP(a1u, 4 * n1); UP(a2u, 4 * n1); UP(awx, 3 * n1); UP(awy, 3 * n1);
+  {
+    std::vector<double> a12(8 * n1);
+    for (size_t e = 0; e < n1; ++e)
+      for (int j = 0; j < 4; ++j) {
+        a12[8 * e + j] = md->a1u[(size_t)j * n1 + e];
+        a12[8 * e + 4 + j] = md->a2u[(size_t)j * n1 + e];
+      }
+    if (dupload(h, &g.a12, a12.data(), a12.size())) return 1;
+  }
   const size_t ne1 = md->ne + 1, cv1 = md->ncv + 1;
   UP(iec, 2 * ne1); UP(ienode, 2 * ne1); UP(isbc, ne1); UP(xijc, ne1); UP(yijc, ne1); UP(dltxc, ne1); UP(dltyc, ne1);
   UP(niec, 2 * cv1); UP(ntrg, cv1); UP(dltxe, cv1); UP(dltye, cv1); UP(dltxncve, 2 * cv1); UP(dltyncve, 2 * cv1);

@@ -39,6 +39,7 @@ struct DevGrid {
   const double *sin_alpha, *cos_alpha, *wsx, *wsy;
   const int *isbce, *iucp, *nv, *nbe;
   const double *a1u, *a2u, *awx, *awy;
+  const double* a12;               // A1U(e,1..4), A2U(e,1..4) of element e at 8e..8e+7 (64-byte aligned)
   // edges and CV sub-edges
   const int *iec, *ienode, *isbc, *niec, *ntrg;
   const double *xijc, *yijc, *dltxc, *dltyc, *dltxe, *dltye, *dltxncve, *dltyncve;
@@ -137,6 +138,16 @@ __device__ __forceinline__ double SIGE_(const DevGrid& g, const double* a, int i
 #define NIEC_(c, j) g.niec[(long)((j)-1) * (g.NCV + 1) + (c)]
 #define DLTXNCVE_(c, j) g.dltxncve[(long)((j)-1) * (g.NCV + 1) + (c)]
 #define DLTYNCVE_(c, j) g.dltyncve[(long)((j)-1) * (g.NCV + 1) + (c)]
+
+// The least-squares coefficients A1U(e,1..4) and A2U(e,1..4) of element
+// e, in the flux type, from the interleaved copy (two 32-byte loads each).
+template <typename FT>
+__device__ __forceinline__ void lsq_coef(const DevGrid& g, int e, FT (&a1)[4], FT (&a2)[4]) {
+  const double4* p = reinterpret_cast<const double4*>(g.a12) + 2L * e;
+  const double4 u = p[0], w = p[1];
+  a1[0] = (FT)u.x; a1[1] = (FT)u.y; a1[2] = (FT)u.z; a1[3] = (FT)u.w;
+  a2[0] = (FT)w.x; a2[1] = (FT)w.y; a2[2] = (FT)w.z; a2[3] = (FT)w.w;
+}
 
 __device__ __forceinline__ double fsign1(double b) { return b >= 0.0 ? 1.0 : -1.0; }
 __device__ __forceinline__ float fsign1(float b) { return b >= 0.0f ? 1.0f : -1.0f; }

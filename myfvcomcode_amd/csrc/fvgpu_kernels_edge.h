@@ -55,8 +55,7 @@ __global__ __launch_bounds__(256) void k_uv_blk(DevGrid g, IntArgs a) {
   const int x = ract ? g.ub_elems[256L * b + t] : 0;
   const int x1 = NBE_(x, 1), x2 = NBE_(x, 2), x3 = NBE_(x, 3);
   FT ra1[4], ra2[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) { ra1[j] = (FT)A1U_(x, j + 1); ra2[j] = (FT)A2U_(x, j + 1); }
+  lsq_coef<FT>(g, x, ra1, ra2);
   const FT rart = (FT)g.art[x], rcc = (FT)g.cc_hvc[x];
   // 2. edge part: edge c
   const bool eact = t < g.ub_ne[b];
