@@ -43,7 +43,7 @@ FAMILY_OF = [
     (r"k_advave_edge<", "advave_edge"), (r"k_ext_elevation\b", "ext_elevation"),
     (r"k_ext_momentum<", "ext_momentum"), (r"k_cvflux_ts<", "cvflux_ts"), (r"k_cvflux_q<", "cvflux_q"),
     (r"k_cvrec<0,", "cvrec_ts"), (r"k_cvrec<1,", "cvrec_q"), (r"k_adv_ts_gather<", "adv_ts"),
-    (r"k_adv_q_gather<", "adv_q"), (r"k_vertvl_r<", "vertvl"),
+    (r"k_adv_q_gather<", "adv_q"), (r"k_vertvl_tile<", "vertvl"),
     (r"k_viscofh_r<", "viscofh"), (r"k_vdif_q_r<", "vdif_q"), (r"k_vdif_uv_r<", "vdif_uv"),
     (r"k_vdif_ts_r<", "vdif_ts"), (r"k_n2e3d\b", "n2e3d"),
     (r"k_cvrecflux<0,", "cvrecflux_ts"), (r"k_cvrecflux<1,", "cvrecflux_q"),
