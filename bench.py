@@ -105,6 +105,10 @@ def main():
                             "bytes": b, "achieved_GBps": round(gbps, 1), "frac": round(gbps * 1e9 / PEAK_HBM, 4)}
         tot_b += b * count / survey_steps
         tot_t += avg_us * 1e-6 * count / survey_steps
+    # the families without a byte model (small, pointwise or boundary work)
+    unmodelled = {name: {"avg_us": round(avg_us, 2), "launches_per_step": count / survey_steps}
+                  for name, (avg_us, count) in sorted(stats.items())
+                  if name not in roofline.FAMILIES and count > 0}
     # dominant kernel = the modelled family with the largest device time per step
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
 
@@ -185,6 +189,7 @@ def main():
                           "kernel_ms_per_step": round(step_kernel_ms, 3)},
         "roofline_by_kernel": per_kernel,
         "roofline_by_kernel_from": "untimed survey pass of %d steps, HIP events around every launch" % survey_steps,
+        "other_kernels": unmodelled,
         "variants": variants,
         "cpu_baseline": cpu,
     }

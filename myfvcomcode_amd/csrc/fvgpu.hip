@@ -217,6 +217,12 @@ void drop_graphs(fvgpu_handle* h) {
 
 inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
 
+// N2E3D of a node field over layers 1..KB (k_n2e3d's grid: elements by
+// groups of eight layers, x extent a multiple of 8).
+#define N2E3D(NVAR, EVAR)                                                                        \
+  LAUNCH("n2e3d", k_n2e3d, dim3((nb(g.N) + 7) / 8 * 8, (unsigned)((g.KB + 7) / 8)), 256, g, NVAR, EVAR, \
+         g.KB)
+
 // Launch of a 1-D kernel family over n work items with the family's
 // workgroup size (256 unless the autotuner chose 64 or 128).
 int block_of(const fvgpu_handle* h, const char* name) {
@@ -330,13 +336,17 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
 #define ADV_TS_CALL16(SM) TLAUNCH("adv_ts", (k_adv_ts_gather<SM, 16, FT>), g.M + 1, g, a, f1, ff, fdis, which, fct)
   if (g.ring_max <= 10) FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL10);
   else FV_SMAX_DISPATCH(g.cv_max, ADV_TS_CALL16);
-  if (g.IOBCN > 0) TLAUNCH("adv_ts_obc", k_adv_ts_obc<FT>, (long)g.IOBCN * g.KBM1, g, a);
+  if (g.IOBCN > 0) {
+    const long np = (long)g.IOBCN * g.KBM1;
+    if (g.cv_max <= 16) LAUNCH("adv_ts_obc", (k_adv_ts_obc<16, FT>), (unsigned)((np + 15) / 16), 256, g, a);
+    else LAUNCH("adv_ts_obc", (k_adv_ts_obc<32, FT>), (unsigned)((np + 7) / 8), 256, g, a);
+  }
 #define VDIF_TS_CALL(KM) TLAUNCH("vdif_ts", k_vdif_ts_r<KM>, g.M + 1, g, a, ff, which)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_TS_CALL);
   TLAUNCH("bcond_ts", k_bcond_ts_sums, g.IOBCN + 1, g, a, which, a.obcsum);
   if (g.IOBCN > 0) TLAUNCH("bcond_ts", k_bcond_ts_layers, (long)g.IOBCN * g.KBM1, g, a, which, (const double*)a.obcsum);
   TLAUNCH("copy", k_copy, mk, f1, ff, mk);
-  TLAUNCH("n2e3d", k_n2e3d, g.N, g, f1, which == 1 ? a.t : a.s, g.KB);
+  N2E3D(f1, which == 1 ? a.t : a.s);
 }
 
 // ADV_Q(Q, QF) with FCT_Q2 (adv_q.F, fct_q2.F) on h->cur with the scratch
@@ -369,7 +379,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   FV_KMAX_DISPATCH(g.KBM1, ADJ1_CALL);
   LAUNCH("adv3d_blk", (k_uv_blk<0, FT>), g.ub_nblk, 256, g, a);
   TLAUNCH("baropg", k_baropg, g.N + 1, g, a, (int)h->md->p.barotropic);
-  if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, g.M + 1, g, a);
+  if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, (long)g.KBM1 * g.m1, g, a);
   TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
   TLAUNCH("advave_int", k_advave_int<FT>, g.N + 1, g, a);
   TLAUNCH("zero", k_zero, g.m1, a.egf, g.m1);
@@ -379,14 +389,14 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   FV_KMAX_DISPATCH(g.KBM1, ADJ2_CALL);
 #define VERTVL_CALL(SM) LAUNCH("vertvl", k_vertvl_tile<SM>, g.nt_nblk, 256, g, a)
   FV_SMAX_DISPATCH(g.cv_max, VERTVL_CALL);
-  TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.wts, a.w, g.KB);
+  N2E3D(a.wts, a.w);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
   LAUNCH("advuv_blk", (k_uv_blk<1, FT>), g.ub_nblk, 256, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   TLAUNCH("uv_finish", k_uv_finish, (long)g.N * g.KBM1, g, a);
-  TLAUNCH("wreal", k_wreal, g.N + 1, g, a);
+  LAUNCH("wreal", k_wreal, nb(g.N), 256, g, a);
   const int fct = h->md->p.scalar_positivity ? 1 : 0;
   if (h->md->p.vertical_closure && h->closure_mode == 1) {
     // learned closure: features, MFMA inference, realizability, then the
@@ -422,7 +432,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
   }
-  TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.km, a.km1, g.KB);
+  N2E3D(a.km, a.km1);
   // ADV_T .. T1 = TF1 on the main stream, the same for S on stream2 (own
   // scratch); the two passes share no written array
   const bool t_on = h->md->p.temperature_active, s_on = h->md->p.salinity_active;
@@ -441,7 +451,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   if (!g.river_ts_specified && g.NUMQBC > 0) LAUNCH("adjust_ts", k_adjust_ts, 1, 256, g, a);
   if (!h->md->p.barotropic) {
     TLAUNCH("dens2", k_dens2, mk, g, a);
-    TLAUNCH("n2e3d", k_n2e3d, g.N, g, a.rho1, a.rho, g.KB);
+    N2E3D(a.rho1, a.rho);
   }
   TLAUNCH("copy", k_copy, nk, a.u, a.uf, nk);
   TLAUNCH("copy", k_copy, nk, a.v, a.vf, nk);
@@ -1674,8 +1684,8 @@ int fvgpu_set_concurrency(fvgpu_handle* h, int32_t mode) {
 int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_len) {
   static const char* fams[] = {"ext_elevation", "advave_edge", "ext_momentum", "cvrec_ts", "cvrec_q", "cvflux_ts",
                                "cvflux_q", "adv_ts", "adv_q",
-                               "viscofh", "vdif_q", "vdif_ts", "n2e3d", "adjust1_brough", "adjust2", "baropg",
-                               "wreal", "uv_finish", "copy", "closure_features", "closure_apply"};
+                               "viscofh", "vdif_q", "vdif_ts", "adjust1_brough", "adjust2", "baropg",
+                               "uv_finish", "copy", "closure_features", "closure_apply"};
   static const int cands[] = {64, 128, 256};
   if (steps < 1) return fail("autotune needs at least one step per candidate");
   // save the state; every candidate runs real steps

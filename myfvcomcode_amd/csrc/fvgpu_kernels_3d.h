@@ -191,13 +191,12 @@ __global__ void k_baropg(DevGrid g, IntArgs a, int barotropic) {
   a.rho[qb] = (a.rho[qb] - a.rmean[qb]) + a.rmean[qb];
 }
 
+// The RMEAN1 round trip of RHO1 (baropg.F:57, 161) at every node and layer
+// 1..KBM1, one thread per node-layer.
 __global__ void k_rho1_roundtrip(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > g.M) return;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    const long q = NI(g, i, k);
-    a.rho1[q] = (a.rho1[q] - a.rmean1[q]) + a.rmean1[q];
-  }
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)g.KBM1 * g.m1) return;
+  a.rho1[q] = (a.rho1[q] - a.rmean1[q]) + a.rmean1[q];
 }
 
 // Ext-mode ADVAVE of INTERNAL_STEP and GX/GY (internal_step.F:387-392),
@@ -252,34 +251,56 @@ __global__ __launch_bounds__(256) void k_adjust2(DevGrid g, IntArgs a) {
   });
 }
 
-// N2E3D (mod_main.F:1633-1645) over layers 1..kmax, one thread per element
-// column with its three nodes in registers.
+// N2E3D (mod_main.F:1633-1645) over layers 1..kmax: one thread per element
+// and group of eight layers (blockIdx.y), the group's node loads issued
+// together.  The grid's x extent is a multiple of 8, so every layer group
+// of an element range runs on the same XCD and shares its L2.
 __global__ __launch_bounds__(256) void k_n2e3d(DevGrid g, const double* nvar, double* evar, int kmax) {
   const int i = (int)xcd_thread() + 1;
   if (i > g.N) return;
   const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
   const double one_third = 1.0 / 3.0;
-#pragma unroll 4
-  for (int k = 1; k <= kmax; ++k)
-    evar[EI(g, i, k)] = one_third * (nvar[NI(g, j1, k)] + nvar[NI(g, j2, k)] + nvar[NI(g, j3, k)]);
+  const int k0 = (int)blockIdx.y * 8 + 1;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = k0 + q;
+    if (k <= kmax) evar[EI(g, i, k)] = one_third * (nvar[NI(g, j1, k)] + nvar[NI(g, j2, k)] + nvar[NI(g, j3, k)]);
+  }
 }
 
-// WREAL (wreal.F:32-140).
-__global__ void k_wreal(DevGrid g, IntArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > g.N) return;
+// WREAL (wreal.F:32-140): one thread per element column, its layers taken
+// eight at a time with the eight layers' loads issued before their
+// arithmetic.
+__global__ __launch_bounds__(256) void k_wreal(DevGrid g, IntArgs a) {
+  const int i = (int)xcd_thread() + 1;
+  if (i > g.N) return;
   const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
   const double dddx = AWX_(i, 1) * a.d[j1] + AWX_(i, 2) * a.d[j2] + AWX_(i, 3) * a.d[j3];
   const double dddy = AWY_(i, 1) * a.d[j1] + AWY_(i, 2) * a.d[j2] + AWY_(i, 3) * a.d[j3];
   const double dedx = AWX_(i, 1) * a.elf[j1] + AWX_(i, 2) * a.elf[j2] + AWX_(i, 3) * a.elf[j3];
   const double dedy = AWY_(i, 1) * a.elf[j1] + AWY_(i, 2) * a.elf[j2] + AWY_(i, 3) * a.elf[j3];
   const double etf1aa = (1.0 / 3.0) * (a.el[j1] + a.el[j2] + a.el[j3]);
-  const double ww2c = (etf1aa - a.et1[i]) / g.dti;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    const long q = EI(g, i, k);
-    const double ww1 = 0.5 * (a.w[q] + a.w[q + g.n1]) + a.u[q] * (ZZ1(i, k) * dddx + dedx) + a.v[q] * (ZZ1(i, k) * dddy + dedy);
-    const double ww2 = (ZZ1(i, k) + 1.) * (etf1aa - a.et1[i]) / g.dti;
-    a.ww[q] = ww1 + ww2;
-    (void)ww2c;
+  const double et1 = a.et1[i];
+  const int KBM1 = g.KBM1;
+  for (int k0 = 1; k0 <= KBM1; k0 += 8) {
+    double w0[8], w1[8], uk[8], vk[8], zz[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = min(k0 + q, KBM1);
+      const long p = EI(g, i, k);
+      w0[q] = a.w[p];
+      w1[q] = a.w[p + g.n1];
+      uk[q] = a.u[p];
+      vk[q] = a.v[p];
+      zz[q] = ZZ1(i, k);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = k0 + q;
+      if (k > KBM1) break;
+      const double ww1 = 0.5 * (w0[q] + w1[q]) + uk[q] * (zz[q] * dddx + dedx) + vk[q] * (zz[q] * dddy + dedy);
+      const double ww2 = (zz[q] + 1.) * (etf1aa - et1) / g.dti;
+      a.ww[EI(g, i, k)] = ww1 + ww2;
+    }
   }
 }

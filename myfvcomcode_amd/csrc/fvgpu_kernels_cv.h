@@ -313,19 +313,24 @@ __global__ __launch_bounds__(256) void k_cvflux_ts(DevGrid g, IntArgs a) {
 }
 
 // Advective part of the horizontal flux at the open-boundary nodes
-// (XFLUX_OBC of adv_t.F:507-520), one thread per OBC node and layer.
-template <typename FT>
-__global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)g.IOBCN * g.KBM1) return;
+// (XFLUX_OBC of adv_t.F:507-520).  LP lanes per OBC node and layer (LP >=
+// the node's sub-edge count): lane j evaluates the node's j-th sub-edge face
+// into LDS, signed by the node's side (x - y and x + (-y) are the same IEEE
+// operation), then the first lane sums them in sub-edge order.  Every
+// face's loads issue at once instead of one face after another.
+template <int LP, typename FT>
+__global__ __launch_bounds__(256) void k_adv_ts_obc(DevGrid g, IntArgs a) {
+  __shared__ double sadv[256];
+  const int lane = threadIdx.x % LP;
+  const long t = (long)blockIdx.x * (256 / LP) + threadIdx.x / LP;
+  const bool act = t < (long)g.IOBCN * g.KBM1;
   const CvRecT<FT>* rec = (const CvRecT<FT>*)a.rec;
-  const int obc = (int)(t % g.IOBCN), k = (int)(t / g.IOBCN) + 1;
+  const int obc = act ? (int)(t % g.IOBCN) : 0, k = act ? (int)(t / g.IOBCN) + 1 : 1;
   const int i = g.i_obc_n[obc];
-  const int n = g.cv_cnt[i];
+  const int n = act ? g.cv_cnt[i] : 0;
   const long ko = (long)(k - 1) * g.m1, eo = (long)(k - 1) * g.n1;
-  double xadv = 0.0;
-  for (int j = 0; j < n; ++j) {
-    const int e = g.cv_ent[g.cv_off[i] + j], c = e >> 1;
+  if (lane < n) {
+    const int e = g.cv_ent[g.cv_off[i] + lane], c = e >> 1;
     const int ia = NIEC_(c, 1), ib = NIEC_(c, 2), i1 = g.ntrg[c];
     const FT dltx = (FT)g.dltxe[c], dlty = (FT)g.dltye[c];
     const FT dtij = (FT)(a.dt1[i1] * DZ1(i1, k));
@@ -335,9 +340,12 @@ __global__ void k_adv_ts_obc(DevGrid g, IntArgs a) {
                                      (FT)DLTXNCVE_(c, 2), (FT)DLTYNCVE_(c, 2), (FT)g.fact, visc0,
                                      rec[ko + ia], rec[ko + ib]);
     const double adv = (double)(r.exflux - r.fxx - r.fyy);
-    if (e & 1) xadv = xadv - adv;
-    else xadv = xadv + adv;
+    sadv[threadIdx.x] = (e & 1) ? -adv : adv;
   }
+  __syncthreads();
+  if (!act || lane != 0) return;
+  double xadv = 0.0;
+  for (int j = 0; j < n; ++j) xadv = xadv + sadv[threadIdx.x + j];
   a.xflux_obc[(long)(k - 1) * g.IOBCN + obc] = xadv;
 }
 

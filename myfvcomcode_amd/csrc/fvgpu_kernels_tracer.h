@@ -55,10 +55,25 @@ __global__ void k_bcond_ts_sums(DevGrid g, IntArgs a, int ncon, double* sums) {
   if (!g.obc_on || t >= g.IOBCN) return;
   const int j = g.i_obc_n[t], j1 = g.next_obc[t];
   double t2d = 0.0, t2d_next = 0.0, xflux2d = 0.0;
-  for (int k = 1; k <= g.KBM1; ++k) {
-    t2d = t2d + f1[NI(g, j, k)] * DZ(j, k);
-    t2d_next = t2d_next + ff[NI(g, j1, k)] * DZ(j1, k);
-    xflux2d = xflux2d + a.xflux_obc[(long)(k - 1) * g.IOBCN + t];
+  // eight layers' loads issued together, then summed in layer order
+  for (int k0 = 1; k0 <= g.KBM1; k0 += 8) {
+    double fa[8], za[8], fb[8], zb[8], xf[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = min(k0 + q, g.KBM1);
+      fa[q] = f1[NI(g, j, k)];
+      za[q] = DZ(j, k);
+      fb[q] = ff[NI(g, j1, k)];
+      zb[q] = DZ(j1, k);
+      xf[q] = a.xflux_obc[(long)(k - 1) * g.IOBCN + t];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (k0 + q > g.KBM1) break;
+      t2d = t2d + fa[q] * za[q];
+      t2d_next = t2d_next + fb[q] * zb[q];
+      xflux2d = xflux2d + xf[q];
+    }
   }
   sums[3 * t] = t2d;
   sums[3 * t + 1] = t2d_next;
