@@ -131,8 +131,12 @@ enum fv_extent {
   X(double,  dz,        FVX_M1K)  X(double,  dzz,       FVX_M1K)            \
   X(double,  z1,        FVX_N1K)  X(double,  zz1,       FVX_N1K)            \
   X(double,  dz1,       FVX_N1K)  X(double,  dzz1,      FVX_N1K)            \
-  /* open boundary (all nodes of type 1: elevation specified) */            \
+  /* open boundary; TYPE_OBC per node (mod_obcs.F:30-39): 1/2 elevation   \
+   * specified (ASL), 3/4 zero elevation (ASL_CLP), 5/6 gravity-wave       \
+   * radiation (GWI), 7/8 Blumberg-Kantha (BKI); odd types make the       \
+   * node's adjacent cells linear (IUCP = 0).  Built with type 1.          */ \
   X(int32_t, i_obc_n,   FVX_OBC)  X(int32_t, next_obc,  FVX_OBC)            \
+  X(int32_t, type_obc,  FVX_OBC)                                            \
   X(int32_t, nadjc_obc, FVX_OBC)  X(int32_t, adjc_obc,  FVX_OBCx2)          \
   X(double,  fluxf_obc, FVX_OBCx2) X(double, dltn_obc,  FVX_OBC)            \
   X(double,  emean,     FVX_OBC)  X(double,  apt,       FVX_OBCxT)          \
@@ -261,6 +265,11 @@ int  fv_model_union(const fv_model* const* parts, int32_t nparts, fv_model** out
  * element within r gets CC_SPONGE = max(CC_SPONGE, c*(1 - dist/r)).      */
 int  fv_model_set_sponge(fv_model* mdl, int32_t n, const int32_t* nodes, const double* radius,
                          const double* coef);
+/* TYPE_OBC of the n = IOBCN open-boundary nodes, each 1..8 (mod_obcs.F:
+ * 30-37; the Orlanski types 9/10 are rejected), and IUCP recomputed: 0 in
+ * the adjacent cells of odd-type nodes, 1 elsewhere (mod_obcs.F:959-962,
+ * 993-996).  Replaces the reference's TYPE_OBC column of the OBC file.   */
+int  fv_model_set_obc_type(fv_model* mdl, int32_t n, const int32_t* types);
 /* Cold start: EL=0, rest, DENS2 density, RMEAN/TMEAN/SMEAN, default
  * turbulence (mod_startup.F:1062-1181, 1332-1408).                       */
 int  fv_model_cold_start(fv_model* mdl);

@@ -90,6 +90,7 @@ def grid_lib():
         L.fv_model_set_tide.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_set_river.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.fv_model_set_sponge.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fv_model_set_obc_type.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.fv_rcm_order.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_union.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
         L.fv_extent_len.restype = C.c_int64
@@ -303,6 +304,15 @@ class Model:
         c = np.ascontiguousarray(np.broadcast_to(coef, nd.shape), np.float64)
         _check(L, L.fv_model_set_sponge(self._ptr, len(nd), nd.ctypes.data, r.ctypes.data, c.ctypes.data),
                "fv_model_set_sponge")
+
+    def set_obc_type(self, types):
+        """TYPE_OBC of every open-boundary node (IOBCN values in 1..8): 1/2
+        elevation specified, 3/4 zero elevation, 5/6 gravity-wave
+        radiation, 7/8 Blumberg-Kantha; odd types linearise the node's
+        adjacent cells (IUCP = 0)."""
+        L = grid_lib()
+        t = np.ascontiguousarray(types, np.int32)
+        _check(L, L.fv_model_set_obc_type(self._ptr, len(t), t.ctypes.data), "fv_model_set_obc_type")
 
     def cold_start(self):
         """Recompute the cold-start state from the current T1/S1/EL fields."""

@@ -309,6 +309,7 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext, bool fresh) {
     on_stream1(h);
 #define EXTEL_CALL(SM) TLAUNCH("ext_elevation", k_ext_elevation<SM>, g.M + 1, g, a)
     FV_SMAX_DISPATCH(g.cv_max, EXTEL_CALL);
+    if (g.obc_radiate) LAUNCH("ext_obc_radiate", k_ext_obc_radiate, nb(g.IOBCN), 256, g, a);
     join_streams(h, FV_CONC_2D);
     TLAUNCH("ext_momentum", k_ext_momentum<FT>, g.N + 1, g, a);
   }
@@ -898,7 +899,12 @@ int upload_grid(fvgpu_handle* h) {
   UP(iec, 2 * ne1); UP(ienode, 2 * ne1); UP(isbc, ne1); UP(xijc, ne1); UP(yijc, ne1); UP(dltxc, ne1); UP(dltyc, ne1);
   UP(niec, 2 * cv1); UP(ntrg, cv1); UP(dltxe, cv1); UP(dltye, cv1); UP(dltxncve, 2 * cv1); UP(dltyncve, 2 * cv1);
   const size_t ob = md->iobcn, nt = md->ntide, nr = md->numqbc;
+  for (size_t k = 0; k < ob; ++k)
+    if (md->type_obc[k] < 1 || md->type_obc[k] > 8) return fail("TYPE_OBC must be 1..8");
+  g.obc_radiate = 0;
+  for (size_t k = 0; k < ob; ++k) if (md->type_obc[k] >= 5) g.obc_radiate = 1;
   UP(i_obc_n, ob); UP(next_obc, ob); UP(nadjc_obc, ob); UP(adjc_obc, 2 * ob); UP(fluxf_obc, 2 * ob);
+  UP(type_obc, ob);
   UP(dltn_obc, ob); UP(emean, ob); UP(apt, ob * nt); UP(phai, ob * nt); UP(period, nt);
   UP(inodeq, nr); UP(qarea, nr); UP(qdis0, nr); UP(tdis, nr); UP(sdis, nr); UP(vqdist, nr * md->kbm1);
 #undef UP

@@ -92,7 +92,8 @@ struct DevGrid {
   int ub_eb, ub_nblk;
   const int *ub_ns, *ub_elems, *ub_ne, *ub_edges, *ub_eloc, *ub_epos;
   // OBC / tide / rivers
-  const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc;
+  const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc, *type_obc;
+  int obc_radiate;                 // some OBC node has TYPE_OBC 5..8
   const double *fluxf_obc, *dltn_obc, *emean, *apt, *phai, *period;
   const int *inodeq;
   const double *qarea, *cos_angleq, *sin_angleq, *qdis0, *tdis, *sdis, *vqdist;

@@ -139,6 +139,27 @@ __device__ __forceinline__ double tide_force(const DevGrid& g, int obc, double r
   return force * ramp;
 }
 
+// FLUX_OBN's flux defect of OBC node i (mod_obcs.F:1196) and, in the last
+// stage, its UARD_OBCN share (external_step.F:423-430).
+__device__ __forceinline__ void ext_obc_flux(const DevGrid& g, const ExtArgs& a, int i, int obc, double elf,
+                                             double elrk, double xobc) {
+  a.fobn[obc] = -(elf - elrk) * g.art1[i] / (a.alpha * g.dte) - xobc;
+  if (a.K == 4) {
+    const double tmp = -(elf - elrk) * g.art1[i] / g.dte - xobc;
+    a.uard_obcn[obc] = a.uard_obcn[obc] + tmp / (double)g.isplit;
+  }
+}
+
+// The new elevation of node i: EL, D = H + EL, the stage-3 EGF sum and
+// DEPTH_CHECK.
+__device__ __forceinline__ void ext_node_store(const DevGrid& g, const ExtArgs& a, int i, double elf) {
+  const double dnew = g.h[i] + elf;
+  a.nxt.el[i] = elf;
+  a.nxt.d[i] = dnew;
+  if (a.K == 3) a.egf[i] = a.egf[i] + elf / (double)g.isplit;
+  if (i >= 1 && dnew < g.min_depth) a.clk->depth_fail = 1;
+}
+
 template <int SMAX>
 __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   const int i = (int)xcd_thread();
@@ -177,22 +198,37 @@ __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   const double elrk = a.base.el[i];
   double elf = elrk - dtk * xflux / g.art1[i];
   if (obc >= 0) {
-    if (g.obc_elev) {
-      const double rk_time = a.clk->ext_time + g.dte * (a.alpha - 1.0);
-      elf = tide_force(g, obc, rk_time, a.clk->ramp_ext);
+    const int ty = g.type_obc[obc];
+    if (ty >= 5) return;   // radiation conditions: k_ext_obc_radiate, once every interior ELF is formed
+    if (ty <= 2) {         // BCOND_ASL, then the RK blend (external_step.F:262-266)
+      if (g.obc_elev) {
+        const double rk_time = a.clk->ext_time + g.dte * (a.alpha - 1.0);
+        elf = tide_force(g, obc, rk_time, a.clk->ramp_ext);
+      }
+      elf = elrk + a.alpha * (elf - elrk);
+    } else {               // BCOND_ASL_CLP (mod_obcs.F:599-617)
+      elf = 0.0;
     }
-    elf = elrk + a.alpha * (elf - elrk);
-    a.fobn[obc] = -(elf - elrk) * g.art1[i] / (a.alpha * g.dte) - xobc;
-    if (a.K == 4) {
-      const double tmp = -(elf - elrk) * g.art1[i] / g.dte - xobc;
-      a.uard_obcn[obc] = a.uard_obcn[obc] + tmp / (double)g.isplit;
-    }
+    ext_obc_flux(g, a, i, obc, elf, elrk, xobc);
   }
-  const double dnew = g.h[i] + elf;
-  a.nxt.el[i] = elf;
-  a.nxt.d[i] = dnew;
-  if (a.K == 3) a.egf[i] = a.egf[i] + elf / (double)g.isplit;
-  if (i >= 1 && dnew < g.min_depth) a.clk->depth_fail = 1;
+  ext_node_store(g, a, i, elf);
+}
+
+// BCOND_GWI (TYPE_OBC 5/6, mod_obcs.F:624-658) and BCOND_BKI (7/8,
+// mod_obcs.F:665-699): the OBC node's ELF from its interior neighbour
+// NEXT_OBC's new ELF (formed by k_ext_elevation, which leaves these nodes
+// alone), one thread per OBC node.
+__global__ void k_ext_obc_radiate(DevGrid g, ExtArgs a) {
+  const int obc = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (obc >= g.IOBCN || g.type_obc[obc] < 5) return;
+  const int i1 = g.i_obc_n[obc], i2 = g.next_obc[obc];
+  const double elrk = a.base.el[i1];
+  const double cc = sqrt(g.grav_n[i1] * g.h[i1]) * g.dte / g.dltn_obc[obc] * a.alpha;
+  const double cp = cc + 1.0;
+  const double elf = g.type_obc[obc] <= 6 ? (cc * a.nxt.el[i2] + elrk) / cp
+                                          : (cc * a.nxt.el[i2] + elrk * (1.0 - g.dte / 10800.0)) / cp;
+  ext_obc_flux(g, a, i1, obc, elf, elrk, a.xflux_obcn[obc]);
+  ext_node_store(g, a, i1, elf);
 }
 
 // ADVAVE_EDGE_GCN edge loop body (advave_edge_gcn.F:546-884) for edge e of

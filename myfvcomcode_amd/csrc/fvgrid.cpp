@@ -517,6 +517,7 @@ int build_metrics(fv_model* md, const double* x, const double* y, const int32_t*
   for (int k = 0; k < iobcn; ++k) {
     const int inode = obc[k];
     md->i_obc_n[k] = inode;
+    md->type_obc[k] = 1;
     int nadjn = 0, adjn[2] = {0, 0};
     const int o = md->nbsn_off[inode];
     for (int j = 0; j < md->ntsn[inode] - 1; ++j) {
@@ -939,6 +940,22 @@ extern "C" int fv_model_build(const fv_case* c, fv_model** out) {
   rc = fv_model_cold_start(md);
   if (rc) { fv_model_free(md); return rc; }
   *out = md;
+  return 0;
+}
+
+extern "C" int fv_model_set_obc_type(fv_model* md, int32_t n, const int32_t* types) {
+  if (!md) return fail("fv_model_set_obc_type: null model");
+  if (n != md->iobcn) return fail("fv_model_set_obc_type: n must equal IOBCN");
+  for (int k = 0; k < n; ++k)
+    if (types[k] < 1 || types[k] > 8)
+      return fail("fv_model_set_obc_type: TYPE_OBC must be 1..8 (Orlanski types 9/10 are not built)");
+  for (int k = 0; k < n; ++k) md->type_obc[k] = types[k];
+  // IUCP: SETUP_OBC marks the adjacent cells of linear (odd-type) nodes
+  for (int i = 0; i <= md->n; ++i) md->iucp[i] = 1;
+  for (int k = 0; k < n; ++k) {
+    if (md->type_obc[k] % 2 != 1) continue;
+    for (int j = 0; j < md->nadjc_obc[k]; ++j) md->iucp[md->adjc_obc[(size_t)j * n + k]] = 0;
+  }
   return 0;
 }
 

@@ -142,6 +142,13 @@ module fvgpu
       real(c_double) :: radius(*), coef(*)
       integer(c_int) :: rc
     end function
+    function fv_model_set_obc_type(mdl, n, types) bind(c, name="fv_model_set_obc_type") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: mdl
+      integer(c_int32_t), value :: n
+      integer(c_int32_t) :: types(*)
+      integer(c_int) :: rc
+    end function
     function fv_model_cold_start(mdl) bind(c, name="fv_model_cold_start") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: mdl

@@ -320,12 +320,14 @@ static void extel_edge(orc* o, int K) {
   for (int i = 0; i <= o->M; ++i) o->elf[i] = o->elrk[i] - dtk * xflux[i] / G->art1[i];
 }
 
-/* BCOND_GCN(1,K) -> BCOND_ASL (spectral), mod_obcs.F:509-592 */
+/* BCOND_GCN(1,K) -> BCOND_ASL (spectral), mod_obcs.F:509-592, on the
+ * nodes of TYPE_OBC 1/2 */
 static void bcond_asl(orc* o) {
   if (!o->p.obc_elevation_on) return;
   const double PI2 = 6.283185307179586;
   double elapsed = o->rk_time;
   for (int i = 0; i < o->IOBCN; ++i) {
+    if (G->type_obc[i] > 2) continue;
     int ii = G->i_obc_n[i];
     double force = 0.0;
     for (int j = 0; j < G->ntide; ++j) {
@@ -335,6 +337,24 @@ static void bcond_asl(orc* o) {
     }
     force = force + G->emean[i];
     o->elf[ii] = force * o->ramp;
+  }
+}
+
+/* The rest of BCOND_GCN(1,K) (bcond_gcn.F:117-122): BCOND_ASL_CLP on the
+ * nodes of TYPE_OBC 3/4 (mod_obcs.F:599-617), BCOND_GWI on 5/6
+ * (mod_obcs.F:624-658) and BCOND_BKI on 7/8 (mod_obcs.F:665-699), the
+ * radiation conditions taking the interior neighbour's new ELF. */
+static void bcond_obc_other(orc* o, int K) {
+  for (int jn = 0; jn < o->IOBCN; ++jn) {
+    const int ty = G->type_obc[jn], i1 = G->i_obc_n[jn], i2 = G->next_obc[jn];
+    if (ty == 3 || ty == 4) {
+      o->elf[i1] = 0.0;
+    } else if (ty >= 5) {
+      double cc = sqrt(G->grav_n[i1] * G->h[i1]) * o->dte / G->dltn_obc[jn] * ALPHA_RK[K];
+      double cp = cc + 1.0;
+      if (ty <= 6) o->elf[i1] = (cc * o->elf[i2] + o->elrk[i1]) / cp;
+      else o->elf[i1] = (cc * o->elf[i2] + o->elrk[i1] * (1.0 - o->dte / 10800.0)) / cp;
+    }
   }
 }
 
@@ -457,7 +477,10 @@ void orc_external_step(orc* o) {
     o->rk_time = o->ext_time + o->dte * (ALPHA_RK[K] - 1.0);
     extel_edge(o, K);
     bcond_asl(o);
+    bcond_obc_other(o, K);
+    /* the RK blend of the elevation-specified nodes (external_step.F:262-266) */
     for (int i = 0; i < o->IOBCN; ++i) {
+      if (G->type_obc[i] > 2) continue;
       int j = G->i_obc_n[i];
       o->elf[j] = o->elrk[j] + ALPHA_RK[K] * (o->elf[j] - o->elrk[j]);
     }

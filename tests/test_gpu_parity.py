@@ -66,6 +66,31 @@ def test_internal_step_parity_many(built):
 
 
 @pytest.mark.parametrize("graph", [False, True])
+def test_open_boundary_types_parity(built, graph):
+    """Every TYPE_OBC the build takes (1..8: elevation specified, clamped,
+    gravity-wave and Blumberg-Kantha radiation, each linear and nonlinear)
+    along one open boundary of the channel."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    m = F.Model.build(0)
+    types = [(k % 8) + 1 for k in range(m.hdr.iobcn)]
+    m.set_obc_type(types)
+    dev, orc = F.DeviceModel(m), Oracle(m)
+    n = 10
+    print("case: 81x21 estuary channel, TYPE_OBC 1..8 along the boundary, %d steps, graph=%s" % (n, graph),
+          flush=True)
+    dev.run(n, graph=graph)
+    orc.run(n)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    for k, v in sorted(rep.items(), key=lambda x: -x[1])[:8]:
+        print("  %-8s %.3e" % (k, v))
+    d = dev.diagnostics()
+    assert d.nonfinite == 0 and d.depth_fail == 0
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("graph", [False, True])
 def test_jittered_wind_case_parity(built, graph):
     """Config-1 generator at 101x51 nodes, KB=11: jitter, seamounts, wind,
     M2/S2/K1; eager and graph-replayed steps both match the oracle."""
