@@ -83,6 +83,7 @@ enum fv_extent {
   FVX_TIDE,     /* NTIDE                      */
   FVX_RIV,      /* NUMQBC                     */
   FVX_RIVxK,    /* (NUMQBC,KBM1)              */
+  FVX_RIVx2,    /* (NUMQBC,2)                 */
   FVX_COUNT
 };
 
@@ -141,8 +142,13 @@ enum fv_extent {
   X(double,  fluxf_obc, FVX_OBCx2) X(double, dltn_obc,  FVX_OBC)            \
   X(double,  emean,     FVX_OBC)  X(double,  apt,       FVX_OBCxT)          \
   X(double,  phai,      FVX_OBCxT) X(double, period,    FVX_TIDE)           \
-  /* rivers (RIVER_INFLOW_LOCATION = 'node') */                             \
+  /* rivers: a node river (RIVER_INFLOW_LOCATION = 'node') has its node   \
+   * in INODEQ and ICELLQ = 0; an edge river ('edge') has its cell in      \
+   * ICELLQ, INODEQ = 0, the nodes of its solid edge in N_ICELLQ and their \
+   * shares of the discharge in RDISQ (mod_force.F:3515-3568)              */ \
   X(int32_t, inodeq,    FVX_RIV)  X(double,  qarea,     FVX_RIV)            \
+  X(int32_t, icellq,    FVX_RIV)  X(int32_t, n_icellq,  FVX_RIVx2)          \
+  X(double,  rdisq,     FVX_RIVx2)                                          \
   X(double,  angleq,    FVX_RIV)  X(double,  qdis0,     FVX_RIV)            \
   X(double,  tdis,      FVX_RIV)  X(double,  sdis,      FVX_RIV)            \
   X(double,  vqdist,    FVX_RIVxK)                                          \
@@ -265,6 +271,11 @@ int  fv_model_union(const fv_model* const* parts, int32_t nparts, fv_model** out
  * element within r gets CC_SPONGE = max(CC_SPONGE, c*(1 - dist/r)).      */
 int  fv_model_set_sponge(fv_model* mdl, int32_t n, const int32_t* nodes, const double* radius,
                          const double* coef);
+/* Edge river (RIVER_INFLOW_LOCATION = 'edge', mod_force.F:3515-3568) on
+ * boundary cell `cell` (1-based, ISBCE = 1, exactly one solid edge): its
+ * discharge enters through the solid edge, split between the edge's two
+ * nodes by their control-volume areas; momentum enters the cell.         */
+int  fv_model_set_river_edge(fv_model* mdl, int32_t cell, double q, double t, double s);
 /* TYPE_OBC of the n = IOBCN open-boundary nodes, each 1..8 (mod_obcs.F:
  * 30-37; the Orlanski types 9/10 are rejected), and IUCP recomputed: 0 in
  * the adjacent cells of odd-type nodes, 1 elsewhere (mod_obcs.F:959-962,

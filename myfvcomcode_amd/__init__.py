@@ -17,7 +17,7 @@ _LIB = os.path.join(_HERE, "lib")
 
 # FVX extents in include/fvgpu.h order
 _EXT = ["M1", "M2", "N1", "N1x3", "N1x4", "NE1", "NE1x2", "CV1", "CV1x2", "M1K", "N1K",
-        "RSN", "RVE", "OBC", "OBCx2", "OBCxT", "TIDE", "RIV", "RIVxK"]
+        "RSN", "RVE", "OBC", "OBCx2", "OBCxT", "TIDE", "RIV", "RIVxK", "RIVx2"]
 
 
 class FvParams(C.Structure):
@@ -91,6 +91,7 @@ def grid_lib():
         L.fv_model_set_river.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.fv_model_set_sponge.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_set_obc_type.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.fv_model_set_river_edge.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]
         L.fv_rcm_order.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.fv_model_union.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
         L.fv_extent_len.restype = C.c_int64
@@ -295,6 +296,13 @@ class Model:
         L = grid_lib()
         _check(L, L.fv_model_set_river(self._ptr, int(node), float(q), float(t), float(s)),
                "fv_model_set_river")
+
+    def set_river_edge(self, cell, q, t=15.0, s=0.0):
+        """Edge river on a solid-boundary cell (1-based element index with
+        exactly one solid edge)."""
+        L = grid_lib()
+        _check(L, L.fv_model_set_river_edge(self._ptr, int(cell), float(q), float(t), float(s)),
+               "fv_model_set_river_edge")
 
     def set_sponge(self, nodes, radius, coef):
         """Sponge layer around nodes (1-based): CC_SPONGE = max(c*(1-dist/r))."""

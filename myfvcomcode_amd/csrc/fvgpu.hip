@@ -574,7 +574,15 @@ int build_lists(fvgpu_handle* h) {
   if (dupload(h, &g.fo_obc, fo_ent.data(), fo_ent.size())) return 1;
   std::vector<int> nobc(M + 1, -1), nriv(M + 1, -1), skip(M + 1, 0);
   for (int i = 0; i < IOBCN; ++i) { nobc[md->i_obc_n[i]] = i; skip[md->i_obc_n[i]] = 1; }
-  for (int j = md->numqbc - 1; j >= 0; --j) { nriv[md->inodeq[j]] = j; skip[md->inodeq[j]] = 1; }
+  // node -> the first river whose discharge enters it (its node river, or
+  // an edge river of which it bounds the solid edge); FCT skips them all
+  const int NQ = md->numqbc;
+  for (int j = NQ - 1; j >= 0; --j) {
+    const int nd[2] = {md->icellq[j] > 0 ? md->n_icellq[j] : md->inodeq[j],
+                       md->icellq[j] > 0 ? md->n_icellq[NQ + j] : 0};
+    for (int q = 0; q < 2; ++q)
+      if (nd[q] > 0) { nriv[nd[q]] = j; skip[nd[q]] = 1; }
+  }
   if (dupload(h, &g.node_obc, nobc.data(), nobc.size())) return 1;
   if (dupload(h, &g.node_riv, nriv.data(), nriv.size())) return 1;
   if (dupload(h, &g.fct_skip, skip.data(), skip.size())) return 1;
@@ -835,9 +843,11 @@ int build_lists(fvgpu_handle* h) {
     if (dupload(h, &g.ub_eloc, beloc.data(), beloc.size())) return 1;
     if (dupload(h, &g.ub_epos, epos.data(), epos.size())) return 1;
   }
-  // element -> rivers (NBVE(J,1) then NBVE(J,NTVE(J)) of the river node)
+  // element -> rivers in river order: NBVE(J,1) then NBVE(J,NTVE(J)) of a
+  // node river's node, an edge river's own cell
   std::vector<std::vector<int>> re(N + 1);
   for (int k = 0; k < md->numqbc; ++k) {
+    if (md->icellq[k] > 0) { re[md->icellq[k]].push_back(k); continue; }
     const int j = md->inodeq[k];
     const int i1 = md->nbve[md->nbve_off[j]], i2 = md->nbve[md->nbve_off[j] + md->ntve[j] - 1];
     re[i1].push_back(k);
@@ -906,7 +916,11 @@ int upload_grid(fvgpu_handle* h) {
   UP(i_obc_n, ob); UP(next_obc, ob); UP(nadjc_obc, ob); UP(adjc_obc, 2 * ob); UP(fluxf_obc, 2 * ob);
   UP(type_obc, ob);
   UP(dltn_obc, ob); UP(emean, ob); UP(apt, ob * nt); UP(phai, ob * nt); UP(period, nt);
+  for (size_t k = 0; k < nr; ++k)
+    if ((md->inodeq[k] > 0) == (md->icellq[k] > 0))
+      return fail("each river must be either a node river (INODEQ) or an edge river (ICELLQ)");
   UP(inodeq, nr); UP(qarea, nr); UP(qdis0, nr); UP(tdis, nr); UP(sdis, nr); UP(vqdist, nr * md->kbm1);
+  UP(icellq, nr); UP(n_icellq, 2 * nr); UP(rdisq, 2 * nr);
 #undef UP
   // host-evaluated trigonometry of setup angles (the same libm the setup uses)
   std::vector<double> cs(m1), sn(m1);

@@ -95,7 +95,9 @@ struct DevGrid {
   const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc, *type_obc;
   int obc_radiate;                 // some OBC node has TYPE_OBC 5..8
   const double *fluxf_obc, *dltn_obc, *emean, *apt, *phai, *period;
-  const int *inodeq;
+  const int *inodeq;               // node river: its node, else 0
+  const int *icellq, *n_icellq;    // edge river: its cell and the two nodes of its solid edge, else 0
+  const double *rdisq;             // edge river: the two nodes' shares of its discharge
   const double *qarea, *cos_angleq, *sin_angleq, *qdis0, *tdis, *sdis, *vqdist;
 };
 
@@ -154,6 +156,19 @@ __device__ __forceinline__ double fsign1(double b) { return b >= 0.0 ? 1.0 : -1.
 __device__ __forceinline__ float fsign1(float b) { return b >= 0.0f ? 1.0f : -1.0f; }
 __device__ __forceinline__ double sq(double x) { return x * x; }
 __device__ __forceinline__ float sq(float x) { return x * x; }
+
+// Whether river j's discharge enters node i, and its share there: all of
+// it at a node river's node (INODEQ), RDISQ(J,1) / RDISQ(J,2) at the two
+// nodes of an edge river's solid edge (N_ICELLQ).  A node river's share is
+// exactly 1, so QDIS*share is QDIS bit for bit.
+__device__ __forceinline__ bool river_share(const DevGrid& g, int j, int i, double& w) {
+  if (g.inodeq[j] == i) { w = 1.0; return true; }
+  if (g.icellq[j] > 0) {
+    if (g.n_icellq[j] == i) { w = g.rdisq[j]; return true; }
+    if (g.n_icellq[g.NUMQBC + j] == i) { w = g.rdisq[g.NUMQBC + j]; return true; }
+  }
+  return false;
+}
 
 // Device clock: the time-dependent scalars of the current step, written by
 // tiny kernels so that a captured graph replays correctly.

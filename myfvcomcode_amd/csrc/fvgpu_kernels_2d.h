@@ -93,11 +93,11 @@ __device__ __forceinline__ WallOp wall_prep(const DevGrid& g, int i) {
   const int b = g.isbce[i];
   if (b == 3) return {3, 0.0, 0.0};
   if (b != 1) return {0, 0.0, 0.0};
-  for (int j = 0; j < g.NUMQBC; ++j) {
-    const int i1 = g.inodeq[j];
-    const int j1 = g.nbve[g.nbve_off[i1]];
-    const int j2 = g.nbve[g.nbve_off[i1] + g.ntve[i1] - 1];
-    if (i == j1 || i == j2) return {1, g.cos_angleq[j], g.sin_angleq[j]};
+  // the first river (in river order) whose mouth is this cell: the first
+  // and last cells around a node river's node, an edge river's own cell
+  if (g.re_off[i] < g.re_off[i + 1]) {
+    const int j = g.re_riv[g.re_off[i]];
+    return {1, g.cos_angleq[j], g.sin_angleq[j]};
   }
   return {2, g.sin_alpha[i], g.cos_alpha[i]};
 }
@@ -191,8 +191,10 @@ __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
   if (obc >= 0) { xobc = xflux; xflux = 0.0; a.xflux_obcn[obc] = xobc; }
   const int rv = g.node_riv[i];
   if (rv >= 0) {
-    for (int j = rv; j < g.NUMQBC; ++j)
-      if (g.inodeq[j] == i) xflux = xflux - a.qdis[j];
+    for (int j = rv; j < g.NUMQBC; ++j) {
+      double w;
+      if (river_share(g, j, i, w)) xflux = xflux - a.qdis[j] * w;
+    }
   }
   const double dtk = a.alpha * g.dte;
   const double elrk = a.base.el[i];
@@ -334,10 +336,13 @@ __device__ __forceinline__ void advave_gather(const DevGrid& g, int i, const voi
     yf = (yf + fluxobn[i] * va[i]) * (double)g.iucp[i];
   }
   for (int p = g.re_off[i]; p < g.re_off[i + 1]; ++p) {
+    // a node river's momentum is split between two cells, an edge river's
+    // enters its cell whole (advave_edge_gcn.F:971-992)
     const int k = g.re_riv[p];
+    const double fq = g.icellq[k] > 0 ? 1.0 : 0.5;
     const double vlctyq = qdis[k] / g.qarea[k];
-    xf = xf - 0.5 * qdis[k] * vlctyq * g.cos_angleq[k];
-    yf = yf - 0.5 * qdis[k] * vlctyq * g.sin_angleq[k];
+    xf = xf - fq * qdis[k] * vlctyq * g.cos_angleq[k];
+    yf = yf - fq * qdis[k] * vlctyq * g.sin_angleq[k];
   }
 }
 

@@ -422,8 +422,11 @@ __global__ __launch_bounds__(256) void k_adv_ts_gather(DevGrid g, IntArgs a, con
     if (!g.river_ts_specified) {
       const int rv = g.node_riv[i];
       if (rv >= 0)
-        for (int j = rv; j < g.NUMQBC; ++j)
-          if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j] * fdis[j];
+        for (int j = rv; j < g.NUMQBC; ++j) {
+          double w;
+          if (river_share(g, j, i, w))
+            xflux = xflux - a.qdis[j] * w * g.vqdist[(long)(k - 1) * g.NUMQBC + j] * fdis[j];
+        }
     }
     if (which == 1) xflux = xflux - 0.0 * art1;
     double x = (fk - xflux / art1 * (g.dti / (a.dt[i] * dzk))) * (a.dt[i] / a.dtfa[i]);
@@ -733,8 +736,10 @@ __global__ __launch_bounds__(256) void k_vertvl_tile(DevGrid g, IntArgs a) {
         });
         double xflux = obc ? 0.0 : x;
         if (rv >= 0)
-          for (int j = rv; j < g.NUMQBC; ++j)
-            if (g.inodeq[j] == i) xflux = xflux - a.qdis[j] * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+          for (int j = rv; j < g.NUMQBC; ++j) {
+            double w;
+            if (river_share(g, j, i, w)) xflux = xflux - a.qdis[j] * w * g.vqdist[(long)(k - 1) * g.NUMQBC + j];
+          }
         w = w + xflux / art1 + DZ(i, k) * dd / g.dti;
         a.wts[NI(g, i, k + 1)] = w;
       }

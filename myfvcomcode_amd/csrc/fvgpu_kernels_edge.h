@@ -191,9 +191,10 @@ __global__ __launch_bounds__(256) void k_uv_blk(DevGrid g, IntArgs a) {
         for (int p = r0; p < r1; ++p) {
           const int r = g.re_riv[p];
           const int j = g.inodeq[r];
+          const bool edge = g.icellq[r] > 0;   // an edge river's momentum enters its cell whole
           const double vlctyq = a.qdis[r] / g.qarea[r];
           const double vq = g.vqdist[(long)(k - 1) * g.NUMQBC + r];
-          const double temp = 0.5 * a.qdis[r] * vq * vq * vlctyq / DZ(j, k);
+          const double temp = (edge ? 1.0 : 0.5) * a.qdis[r] * vq * vq * vlctyq / (edge ? DZ1(i, k) : DZ(j, k));
           f = f - temp * (cy ? g.sin_angleq[r] : g.cos_angleq[r]);
         }
         sc = sc + f;
@@ -216,9 +217,10 @@ __global__ __launch_bounds__(256) void k_uv_blk(DevGrid g, IntArgs a) {
         for (int p = r0; p < r1; ++p) {
           const int r = g.re_riv[p];
           const int j = g.inodeq[r];
+          const bool edge = g.icellq[r] > 0;   // an edge river's momentum enters its cell whole
           const double vlctyq = a.qdis[r] / g.qarea[r];
           const double vq = g.vqdist[(long)(k - 1) * g.NUMQBC + r];
-          const double temp = 0.5 * a.qdis[r] * vq * vq * vlctyq / DZ(j, k);
+          const double temp = (edge ? 1.0 : 0.5) * a.qdis[r] * vq * vq * vlctyq / (edge ? DZ1(i, k) : DZ(j, k));
           f = f - temp * (cy ? g.sin_angleq[r] : g.cos_angleq[r]);
         }
         (cy ? a.vf : a.uf)[ko + i] = ck * dt1 / d1 - g.dti * f / art / (d1 * dzk);

@@ -13,21 +13,35 @@
 // for the elements around the river nodes, the only elements whose three
 // nodes changed (the others' T/S already hold the same values).
 __global__ void k_adjust_ts(DevGrid g, IntArgs a) {
+  // a river's nodes: its node, or the two nodes of its solid edge
+  auto river_nodes = [&](int r, int (&nd)[2]) {
+    if (g.icellq[r] > 0) { nd[0] = g.n_icellq[r]; nd[1] = g.n_icellq[g.NUMQBC + r]; return 2; }
+    nd[0] = g.inodeq[r];
+    return 1;
+  };
   for (int k = threadIdx.x + 1; k <= g.KBM1; k += blockDim.x)
     for (int r = 0; r < g.NUMQBC; ++r) {
-      const long p = NI(g, g.inodeq[r], k);
-      a.t1[p] = fmax(a.t1[p], g.tdis[r]);
-      a.s1[p] = fmax(a.s1[p], g.sdis[r]);
+      int nd[2];
+      const int c = river_nodes(r, nd);
+      for (int q = 0; q < c; ++q) {
+        const long p = NI(g, nd[q], k);
+        a.t1[p] = fmax(a.t1[p], g.tdis[r]);
+        a.s1[p] = fmax(a.s1[p], g.sdis[r]);
+      }
     }
   __syncthreads();
   const double one_third = 1.0 / 3.0;
   for (int r = 0; r < g.NUMQBC; ++r) {
-    const int j = g.inodeq[r], off = g.nbve_off[j], nt = g.ntve[j];
-    for (int q = threadIdx.x; q < nt * g.KB; q += blockDim.x) {
-      const int i = g.nbve[off + q % nt], k = q / nt + 1;
-      const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
-      a.t[EI(g, i, k)] = one_third * (a.t1[NI(g, j1, k)] + a.t1[NI(g, j2, k)] + a.t1[NI(g, j3, k)]);
-      a.s[EI(g, i, k)] = one_third * (a.s1[NI(g, j1, k)] + a.s1[NI(g, j2, k)] + a.s1[NI(g, j3, k)]);
+    int nd[2];
+    const int c = river_nodes(r, nd);
+    for (int q0 = 0; q0 < c; ++q0) {
+      const int j = nd[q0], off = g.nbve_off[j], nt = g.ntve[j];
+      for (int q = threadIdx.x; q < nt * g.KB; q += blockDim.x) {
+        const int i = g.nbve[off + q % nt], k = q / nt + 1;
+        const int j1 = NV_(i, 1), j2 = NV_(i, 2), j3 = NV_(i, 3);
+        a.t[EI(g, i, k)] = one_third * (a.t1[NI(g, j1, k)] + a.t1[NI(g, j2, k)] + a.t1[NI(g, j3, k)]);
+        a.s[EI(g, i, k)] = one_third * (a.s1[NI(g, j1, k)] + a.s1[NI(g, j2, k)] + a.s1[NI(g, j3, k)]);
+      }
     }
   }
 }
@@ -48,6 +62,11 @@ __global__ void k_bcond_ts_sums(DevGrid g, IntArgs a, int ncon, double* sums) {
     double* fw = ncon == 1 ? a.tf1 : a.sf1;
     const double* fdis = ncon == 1 ? g.tdis : g.sdis;
     for (int r = 0; r < g.NUMQBC; ++r) {
+      if (g.icellq[r] > 0) {   // both nodes of an edge river's solid edge
+        const int j11 = g.n_icellq[r], j22 = g.n_icellq[g.NUMQBC + r];
+        for (int k = 1; k <= g.KBM1; ++k) { fw[NI(g, j11, k)] = fdis[r]; fw[NI(g, j22, k)] = fdis[r]; }
+        continue;
+      }
       const int j11 = g.inodeq[r];
       for (int k = 1; k <= g.KBM1; ++k) fw[NI(g, j11, k)] = fdis[r];
     }

@@ -122,6 +122,7 @@ extern "C" int64_t fv_extent_len(const fv_model* md, int32_t ext) {
     case FVX_TIDE: return md->ntide;
     case FVX_RIV: return md->numqbc;
     case FVX_RIVxK: return (int64_t)md->numqbc * md->kbm1;
+    case FVX_RIVx2: return 2 * (int64_t)md->numqbc;
   }
   return -1;
 }
@@ -741,29 +742,35 @@ extern "C" int fv_rcm_order(int32_t m, int32_t n, const int32_t* nv_rows, int32_
   return 0;
 }
 
+// Room for one more river: every per-river array grows by one entry, the
+// existing rivers kept in their order; returns the new river's index.
+static int add_river_slot(fv_model* md) {
+  const int KBM1 = md->kbm1, k = md->numqbc;
+  auto grow = [&](auto*& a, int comps) {   // (k, comps) -> (k + 1, comps), component-major
+    using T = std::remove_reference_t<decltype(*a)>;
+    T* b = (T*)xalloc(sizeof(T) * (size_t)(k + 1) * comps);
+    for (int c = 0; c < comps; ++c)
+      for (int j = 0; j < k; ++j) b[(size_t)c * (k + 1) + j] = a[(size_t)c * k + j];
+    std::free(a);
+    a = b;
+  };
+  grow(md->inodeq, 1); grow(md->qarea, 1); grow(md->angleq, 1);
+  grow(md->qdis0, 1); grow(md->tdis, 1); grow(md->sdis, 1);
+  grow(md->icellq, 1); grow(md->n_icellq, 2); grow(md->rdisq, 2);
+  grow(md->vqdist, KBM1);
+  md->numqbc = k + 1;
+  md->inodeq[k] = 0; md->icellq[k] = 0;
+  md->n_icellq[k] = 0; md->n_icellq[k + 1 + k] = 0;
+  md->rdisq[k] = 0.0; md->rdisq[k + 1 + k] = 0.0;
+  return k;
+}
+
 extern "C" int fv_model_set_river(fv_model* md, int32_t node, double q, double t, double s) {
   const int KBM1 = md->kbm1;
   const size_t m1 = md->m + 1;
   if (node < 1 || node > md->m) return fail("river node out of range");
   if (md->isonb[node] != 1) return fail("river node is not on the solid boundary");
-  const int k = md->numqbc;
-  auto grow = [&](auto*& a, size_t n) {  // n = k + 1 entries
-    using T = std::remove_reference_t<decltype(*a)>;
-    T* b = (T*)xalloc(sizeof(T) * n);
-    std::memcpy(b, a, sizeof(T) * (size_t)k);  // keep the k existing rivers
-    std::free(a);
-    a = b;
-  };
-  grow(md->inodeq, k + 1); grow(md->qarea, k + 1); grow(md->angleq, k + 1);
-  grow(md->qdis0, k + 1); grow(md->tdis, k + 1); grow(md->sdis, k + 1);
-  {
-    double* v = (double*)xalloc(sizeof(double) * (size_t)(k + 1) * KBM1);
-    for (int l = 0; l < KBM1; ++l)
-      for (int j = 0; j < k; ++j) v[(size_t)l * (k + 1) + j] = md->vqdist[(size_t)l * k + j];
-    std::free(md->vqdist);
-    md->vqdist = v;
-  }
-  md->numqbc = k + 1;
+  const int k = add_river_slot(md), nq = md->numqbc;
   md->inodeq[k] = node; md->qdis0[k] = q; md->tdis[k] = t; md->sdis[k] = s;
   // QAREA / ANGLEQ of a node river (mod_force.F:3470-3511)
   const int o = md->nbsn_off[node];
@@ -776,7 +783,36 @@ extern "C" int fv_model_set_river(fv_model* md, int32_t node, double q, double t
   const double dxy = std::sqrt(dx12 * dx12 + dy12 * dy12) + std::sqrt(dx32 * dx32 + dy32 * dy32);
   md->qarea[k] = 0.5 * dxy * md->h[node];
   md->angleq[k] = (a1 - a2) / 2. + a2;
-  for (int l = 0; l < KBM1; ++l) md->vqdist[(size_t)l * (k + 1) + k] = md->dz[(size_t)l * m1 + node];
+  for (int l = 0; l < KBM1; ++l) md->vqdist[(size_t)l * nq + k] = md->dz[(size_t)l * m1 + node];
+  return 0;
+}
+
+extern "C" int fv_model_set_river_edge(fv_model* md, int32_t cell, double q, double t, double s) {
+  const int KBM1 = md->kbm1;
+  const size_t n1 = md->n + 1;
+  if (cell < 1 || cell > md->n) return fail("river cell out of range");
+  if (md->isbce[cell] != 1) return fail("river cell is not on the solid boundary (ISBCE /= 1)");
+  int itmp = 0, jtmp = 0;
+  for (int j = 1; j <= 3; ++j)
+    if (md->nbe[(size_t)(j - 1) * n1 + cell] == 0) { jtmp = j; ++itmp; }
+  if (itmp != 1) return fail("river cell must have exactly one solid boundary edge");
+  // the solid edge joins the two nodes other than node JTMP (mod_force.F:3540-3547)
+  const int j1 = jtmp + 1 - ((jtmp + 1) / 4) * 3, j2 = jtmp + 2 - ((jtmp + 2) / 4) * 3;
+  const int i1 = md->nv[(size_t)(j1 - 1) * n1 + cell], i2 = md->nv[(size_t)(j2 - 1) * n1 + cell];
+  const int k = add_river_slot(md), nq = md->numqbc;
+  md->icellq[k] = cell; md->qdis0[k] = q; md->tdis[k] = t; md->sdis[k] = s;
+  md->n_icellq[k] = i1; md->n_icellq[nq + k] = i2;
+  // QAREA, ANGLEQ (its pi/2 a default-real constant) and RDISQ (mod_force.F:3548-3567)
+  const double htmp = 0.5 * (md->h[i1] + md->h[i2]);
+  const double dy12 = md->vy[i1] - md->vy[i2], dx12 = md->vx[i1] - md->vx[i2];
+  const double atmp1 = std::atan2(dy12, dx12);
+  md->qarea[k] = std::sqrt(dx12 * dx12 + dy12 * dy12) * htmp;
+  md->angleq[k] = atmp1 + (double)(3.1415927f / 2.0f);
+  const double areatmp = md->art1[i1] + md->art1[i2];
+  md->rdisq[k] = md->art1[i1] / areatmp;
+  md->rdisq[nq + k] = md->art1[i2] / areatmp;
+  // vertical distribution: the cell's layer thicknesses
+  for (int l = 0; l < KBM1; ++l) md->vqdist[(size_t)l * nq + k] = md->dz1[(size_t)l * n1 + cell];
   return 0;
 }
 
@@ -1071,8 +1107,8 @@ namespace {
 enum RefKind { REF_NONE = 0, REF_NODE, REF_ELEM };
 
 RefKind field_ref(const char* name) {
-  static const char* node_refs[] = {"nbsn", "inodeq", "i_obc_n", "next_obc", "ienode", "niec", "nv"};
-  static const char* elem_refs[] = {"nbe", "nbve", "iec", "ntrg", "adjc_obc"};
+  static const char* node_refs[] = {"nbsn", "inodeq", "i_obc_n", "next_obc", "ienode", "niec", "nv", "n_icellq"};
+  static const char* elem_refs[] = {"nbe", "nbve", "iec", "ntrg", "adjc_obc", "icellq"};
   for (const char* n : node_refs) if (std::strcmp(n, name) == 0) return REF_NODE;
   for (const char* n : elem_refs) if (std::strcmp(n, name) == 0) return REF_ELEM;
   return REF_NONE;
@@ -1092,6 +1128,7 @@ ExtShape ext_shape(int ext, const fv_model* md) {
     case FVX_OBCx2: return {1, 2};
     case FVX_OBCxT: return {1, md->ntide};
     case FVX_RIVxK: return {1, md->kbm1};
+    case FVX_RIVx2: return {1, 2};
     case FVX_TIDE: return {3, 1};
   }
   return {3, 1};
@@ -1106,7 +1143,7 @@ int64_t ext_count(int ext, const fv_model* md) {
     case FVX_RSN: return md->nring_sn;
     case FVX_RVE: return md->nring_ve;
     case FVX_OBC: case FVX_OBCx2: case FVX_OBCxT: return md->iobcn;
-    case FVX_RIV: case FVX_RIVxK: return md->numqbc;
+    case FVX_RIV: case FVX_RIVxK: case FVX_RIVx2: return md->numqbc;
     case FVX_TIDE: return md->ntide;
   }
   return 0;

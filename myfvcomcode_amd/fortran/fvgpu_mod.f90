@@ -142,6 +142,13 @@ module fvgpu
       real(c_double) :: radius(*), coef(*)
       integer(c_int) :: rc
     end function
+    function fv_model_set_river_edge(mdl, cell, q, t, s) bind(c, name="fv_model_set_river_edge") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: mdl
+      integer(c_int32_t), value :: cell
+      real(c_double), value :: q, t, s
+      integer(c_int) :: rc
+    end function
     function fv_model_set_obc_type(mdl, n, types) bind(c, name="fv_model_set_obc_type") result(rc)
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: mdl

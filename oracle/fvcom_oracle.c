@@ -95,6 +95,9 @@ static const double GRAV = 9.81;
 #define ADJC_OBC(i, j) G->adjc_obc[(size_t)((j)-1) * o->IOBCN + (i)]
 #define FLUXF_OBC(i, j) G->fluxf_obc[(size_t)((j)-1) * o->IOBCN + (i)]
 #define VQDIST(j, k) G->vqdist[(size_t)((k)-1) * o->NUMQBC + (j)]
+/* edge rivers (RIVER_INFLOW_LOCATION = 'edge'): N_ICELLQ(J,1:2), RDISQ(J,1:2) */
+#define N_ICELLQ(j, c) G->n_icellq[(size_t)((c)-1) * o->NUMQBC + (j)]
+#define RDISQ(j, c) G->rdisq[(size_t)((c)-1) * o->NUMQBC + (j)]
 
 static double fsign(double a, double b) { return (b >= 0.0) ? fabs(a) : -fabs(a); }
 static double sq(double x) { return x * x; }
@@ -247,9 +250,15 @@ static void wall_project(orc* o, int i, double* pu, double* pv) {
   if (G->isbce[i] != 1) return;
   double alpha1 = G->alpha[i];
   for (int j = 0; j < o->NUMQBC; ++j) {
-    int i1 = G->inodeq[j];
-    int j1 = NBVE(i1, 1), j2 = NBVE(i1, NTVE(i1));
-    if (i == j1 || i == j2) {
+    int mouth;
+    if (G->icellq[j] > 0) {          /* edge river: its cell */
+      mouth = i == G->icellq[j];
+    } else {                         /* node river: the first and last cells of its node */
+      int i1 = G->inodeq[j];
+      int j1 = NBVE(i1, 1), j2 = NBVE(i1, NTVE(i1));
+      mouth = i == j1 || i == j2;
+    }
+    if (mouth) {
       double aq = G->angleq[j];
       double untmp = *pu * cos(aq) + *pv * sin(aq);
       double vntmp = -*pu * sin(aq) + *pv * cos(aq);
@@ -313,8 +322,14 @@ static void extel_edge(orc* o, int K) {
     xflux[G->i_obc_n[i]] = 0.0;
   }
   for (int j = 0; j < o->NUMQBC; ++j) {
-    int jj = G->inodeq[j];
-    xflux[jj] = xflux[jj] - o->qdis[j];
+    if (G->icellq[j] > 0) {
+      int j1 = N_ICELLQ(j, 1), j2 = N_ICELLQ(j, 2);
+      xflux[j1] = xflux[j1] - o->qdis[j] * RDISQ(j, 1);
+      xflux[j2] = xflux[j2] - o->qdis[j] * RDISQ(j, 2);
+    } else {
+      int jj = G->inodeq[j];
+      xflux[jj] = xflux[jj] - o->qdis[j];
+    }
   }
   double dtk = ALPHA_RK[K] * o->dte;
   for (int i = 0; i <= o->M; ++i) o->elf[i] = o->elrk[i] - dtk * xflux[i] / G->art1[i];
@@ -435,6 +450,14 @@ static void advave_edge_gcn(orc* o, double* xflux, double* yflux) {
     }
   }
   for (int k = 0; k < o->NUMQBC; ++k) {
+    if (G->icellq[k] > 0) {
+      int i1 = G->icellq[k];
+      double vlctyq = o->qdis[k] / G->qarea[k];
+      double temp = o->qdis[k] * vlctyq;
+      xflux[i1] = xflux[i1] - temp * cos(G->angleq[k]);
+      yflux[i1] = yflux[i1] - temp * sin(G->angleq[k]);
+      continue;
+    }
     int j = G->inodeq[k];
     int i1 = NBVE(j, 1), i2 = NBVE(j, NTVE(j));
     double vlctyq = o->qdis[k] / G->qarea[k];
@@ -623,6 +646,16 @@ static void advection_edge_gcn(orc* o, double* xflux, double* yflux) {
     if (G->isbce[i] == 2)
       for (int k = 1; k <= KBM1; ++k) { EL3(xflux, i, k) = 0.0; EL3(yflux, i, k) = 0.0; }
   for (int ii = 0; ii < o->NUMQBC; ++ii) {
+    if (G->icellq[ii] > 0) {
+      int i1 = G->icellq[ii];
+      for (int k = 1; k <= KBM1; ++k) {
+        double vlctyq = o->qdis[ii] / G->qarea[ii];
+        double temp = o->qdis[ii] * VQDIST(ii, k) * VQDIST(ii, k) * vlctyq / EL3(G->dz1, i1, k);
+        EL3(xflux, i1, k) = EL3(xflux, i1, k) - temp * cos(G->angleq[ii]);
+        EL3(yflux, i1, k) = EL3(yflux, i1, k) - temp * sin(G->angleq[ii]);
+      }
+      continue;
+    }
     int j = G->inodeq[ii], i1 = NBVE(j, 1), i2 = NBVE(j, NTVE(j));
     for (int k = 1; k <= KBM1; ++k) {
       double vlctyq = o->qdis[ii] / G->qarea[ii];
@@ -707,6 +740,14 @@ static void vertvl_edge(orc* o) {
     for (int k = 1; k <= KBM1; ++k)
       if (G->isonb[i] == 2) ND(xflux, i, k) = 0.0;
   for (int j = 0; j < o->NUMQBC; ++j) {
+    if (G->icellq[j] > 0) {
+      int j1 = N_ICELLQ(j, 1), j2 = N_ICELLQ(j, 2);
+      for (int k = 1; k <= KBM1; ++k) {
+        ND(xflux, j1, k) = ND(xflux, j1, k) - o->qdis[j] * RDISQ(j, 1) * VQDIST(j, k);
+        ND(xflux, j2, k) = ND(xflux, j2, k) - o->qdis[j] * RDISQ(j, 2) * VQDIST(j, k);
+      }
+      continue;
+    }
     int jj = G->inodeq[j];
     for (int k = 1; k <= KBM1; ++k) ND(xflux, jj, k) = ND(xflux, jj, k) - o->qdis[j] * VQDIST(j, k);
   }
@@ -850,6 +891,16 @@ static void adv_uv_edge_gcn(orc* o) {
     if (G->isbce[i] == 2)
       for (int k = 1; k <= KBM1; ++k) { EL3(xflux, i, k) = 0.0; EL3(yflux, i, k) = 0.0; }
   for (int ii = 0; ii < o->NUMQBC; ++ii) {
+    if (G->icellq[ii] > 0) {
+      int i1 = G->icellq[ii];
+      for (int k = 1; k <= KBM1; ++k) {
+        double vlctyq = o->qdis[ii] / G->qarea[ii];
+        double temp = o->qdis[ii] * VQDIST(ii, k) * VQDIST(ii, k) * vlctyq / EL3(G->dz1, i1, k);
+        EL3(xflux, i1, k) = EL3(xflux, i1, k) - temp * cos(G->angleq[ii]);
+        EL3(yflux, i1, k) = EL3(yflux, i1, k) - temp * sin(G->angleq[ii]);
+      }
+      continue;
+    }
     int j = G->inodeq[ii], i1 = NBVE(j, 1), i2 = NBVE(j, NTVE(j));
     for (int k = 1; k <= KBM1; ++k) {
       double vlctyq = o->qdis[ii] / G->qarea[ii];
@@ -1026,7 +1077,10 @@ static void adv_q(orc* o, const double* q, double* qf) {
 
 static int fct_skip_node(orc* o, int i) {
   for (int j = 0; j < o->IOBCN; ++j) if (i == G->i_obc_n[j]) return 1;
-  for (int j = 0; j < o->NUMQBC; ++j) if (i == G->inodeq[j]) return 1;
+  for (int j = 0; j < o->NUMQBC; ++j) {
+    if (G->icellq[j] > 0) { if (i == N_ICELLQ(j, 1) || i == N_ICELLQ(j, 2)) return 1; }
+    else if (i == G->inodeq[j]) return 1;
+  }
   return 0;
 }
 
@@ -1239,8 +1293,16 @@ static void adv_ts(orc* o, int which) {
     }
   if (!o->p.river_ts_specified) {
     for (int j = 0; j < o->NUMQBC; ++j) {
-      int jj = G->inodeq[j];
       double stpoint = fdis[j];
+      if (G->icellq[j] > 0) {
+        int j1 = N_ICELLQ(j, 1), j2 = N_ICELLQ(j, 2);
+        for (int k = 1; k <= KBM1; ++k) {
+          ND(xflux, j1, k) = ND(xflux, j1, k) - o->qdis[j] * RDISQ(j, 1) * VQDIST(j, k) * stpoint;
+          ND(xflux, j2, k) = ND(xflux, j2, k) - o->qdis[j] * RDISQ(j, 2) * VQDIST(j, k) * stpoint;
+        }
+        continue;
+      }
+      int jj = G->inodeq[j];
       for (int k = 1; k <= KBM1; ++k) ND(xflux, jj, k) = ND(xflux, jj, k) - o->qdis[j] * VQDIST(j, k) * stpoint;
     }
   }
@@ -1327,6 +1389,14 @@ static void bcond_ts(orc* o, int ncon) {
   const double* f1 = ncon == 1 ? o->t1 : o->s1;
   if (o->p.river_ts_specified) {
     for (int i = 0; i < o->NUMQBC; ++i) {
+      if (G->icellq[i] > 0) {
+        int j11 = N_ICELLQ(i, 1), j22 = N_ICELLQ(i, 2);
+        for (int k = 1; k <= KBM1; ++k) {
+          ND(o->tf1, j11, k) = G->tdis[i]; ND(o->sf1, j11, k) = G->sdis[i];
+          ND(o->tf1, j22, k) = G->tdis[i]; ND(o->sf1, j22, k) = G->sdis[i];
+        }
+        continue;
+      }
       int j11 = G->inodeq[i];
       for (int k = 1; k <= KBM1; ++k) { ND(o->tf1, j11, k) = G->tdis[i]; ND(o->sf1, j11, k) = G->sdis[i]; }
     }
@@ -1382,6 +1452,16 @@ static void bcond_ts(orc* o, int ncon) {
 static void adjust_ts(orc* o) {
   if (o->NUMQBC <= 0) return;
   for (int i = 0; i < o->NUMQBC; ++i) {
+    if (G->icellq[i] > 0) {
+      int j1 = N_ICELLQ(i, 1), j2 = N_ICELLQ(i, 2);
+      for (int k = 1; k <= o->KBM1; ++k) {
+        ND(o->t1, j1, k) = fmax(ND(o->t1, j1, k), G->tdis[i]);
+        ND(o->t1, j2, k) = fmax(ND(o->t1, j2, k), G->tdis[i]);
+        ND(o->s1, j1, k) = fmax(ND(o->s1, j1, k), G->sdis[i]);
+        ND(o->s1, j2, k) = fmax(ND(o->s1, j2, k), G->sdis[i]);
+      }
+      continue;
+    }
     int j = G->inodeq[i];
     for (int k = 1; k <= o->KBM1; ++k) {
       ND(o->t1, j, k) = fmax(ND(o->t1, j, k), G->tdis[i]);
