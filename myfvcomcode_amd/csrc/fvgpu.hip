@@ -159,11 +159,13 @@ int dupload_rw(fvgpu_handle* h, double** p, const double* src, size_t n) {
   return 0;
 }
 
-// profiling helpers: events around launches when enabled (eager mode only)
+// profiling helpers: events around launches when enabled (eager mode only).
+// The enqueue helpers below leave their HIP status to the runtime's last
+// error, which the entry points check with hipGetLastError after enqueuing.
 hipEvent_t next_event(fvgpu_handle* h) {
   if (h->event_next >= h->event_pool.size()) {
     hipEvent_t e;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     h->event_pool.push_back(e);
   }
   return h->event_pool[h->event_next++];
@@ -176,12 +178,12 @@ struct Launch {
   bool on;
   Launch(fvgpu_handle* h_, const char* n) : h(h_), name(n) {
     on = h->profiling && (h->profile_only.empty() || h->profile_only == n);
-    if (on) { e0 = next_event(h); hipEventRecord(e0, h->cur); }
+    if (on) { e0 = next_event(h); (void)hipEventRecord(e0, h->cur); }
   }
   ~Launch() {
     if (on) {
       e1 = next_event(h);
-      hipEventRecord(e1, h->cur);
+      (void)hipEventRecord(e1, h->cur);
       h->pending.push_back({name, {e0, e1}});
     }
   }
@@ -192,7 +194,7 @@ int flush_profile(fvgpu_handle* h) {
   HIPCHK(hipStreamSynchronize(h->stream));
   for (auto& p : h->pending) {
     float ms = 0;
-    hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    HIPCHK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
     KernelStat& s = h->stats[p.first];
     s.total_ms += ms;
     s.count += 1;
@@ -205,8 +207,8 @@ int flush_profile(fvgpu_handle* h) {
 // The captured steps are stale (the step's launches or their options
 // changed): the next graph run captures again.
 void drop_graphs(fvgpu_handle* h) {
-  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
 }
 
 #define LAUNCH(NAME, KERNEL, GRID, BLOCK, ...)                                   \
@@ -245,13 +247,13 @@ int block_of(const fvgpu_handle* h, const char* name) {
 enum { FV_CONC_3D = 1, FV_CONC_2D = 2 };
 void fork_streams(fvgpu_handle* h, int what) {
   if (!(h->concurrent & what)) return;
-  hipEventRecord(h->ev_fork, h->stream);
-  hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+  (void)hipEventRecord(h->ev_fork, h->stream);
+  (void)hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
 }
 void join_streams(fvgpu_handle* h, int what) {
   if (!(h->concurrent & what)) return;
-  hipEventRecord(h->ev_join, h->stream2);
-  hipStreamWaitEvent(h->stream, h->ev_join, 0);
+  (void)hipEventRecord(h->ev_join, h->stream2);
+  (void)hipStreamWaitEvent(h->stream, h->ev_join, 0);
 }
 // enqueue on the second chain's stream / back on the main stream
 void on_stream2(fvgpu_handle* h, int what) { h->cur = (h->concurrent & what) ? h->stream2 : h->stream; }
@@ -1676,9 +1678,9 @@ int fvmlp_infer(fvmlp* t, int64_t lo, int64_t n, int32_t use_mfma, float* y) {
 
 void fvmlp_destroy(fvmlp* t) {
   if (!t) return;
-  if (t->stream) hipStreamSynchronize(t->stream);
-  for (void* p : t->allocs) hipFree(p);
-  if (t->stream) hipStreamDestroy(t->stream);
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  for (void* p : t->allocs) (void)hipFree(p);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
   delete t;
 }
 
