@@ -134,10 +134,12 @@ enum fv_extent {
   X(double,  dz1,       FVX_N1K)  X(double,  dzz1,      FVX_N1K)            \
   /* open boundary; TYPE_OBC per node (mod_obcs.F:30-39): 1/2 elevation   \
    * specified (ASL), 3/4 zero elevation (ASL_CLP), 5/6 gravity-wave       \
-   * radiation (GWI), 7/8 Blumberg-Kantha (BKI); odd types make the       \
-   * node's adjacent cells linear (IUCP = 0).  Built with type 1.          */ \
+   * radiation (GWI), 7/8 Blumberg-Kantha (BKI), 9/10 Orlanski (ORE); odd  \
+   * types make the node's adjacent cells linear (IUCP = 0).  Built with   \
+   * type 1.  NEXT_OBC2: the interior neighbour of NEXT_OBC most normal to \
+   * the boundary (mod_obcs.F:1072-1101)                                   */ \
   X(int32_t, i_obc_n,   FVX_OBC)  X(int32_t, next_obc,  FVX_OBC)            \
-  X(int32_t, type_obc,  FVX_OBC)                                            \
+  X(int32_t, type_obc,  FVX_OBC)  X(int32_t, next_obc2, FVX_OBC)            \
   X(int32_t, nadjc_obc, FVX_OBC)  X(int32_t, adjc_obc,  FVX_OBCx2)          \
   X(double,  fluxf_obc, FVX_OBCx2) X(double, dltn_obc,  FVX_OBC)            \
   X(double,  emean,     FVX_OBC)  X(double,  apt,       FVX_OBCxT)          \
@@ -178,7 +180,10 @@ enum fv_extent {
   X(double, q2,   FVX_M1K) X(double, q2l,  FVX_M1K)                         \
   X(double, l,    FVX_M1K) X(double, km,   FVX_M1K)                         \
   X(double, kh,   FVX_M1K) X(double, kq,   FVX_M1K)                         \
-  X(double, wts,  FVX_M1K)
+  X(double, wts,  FVX_M1K)                                                  \
+  /* the elevation one and two RK stages back (ASSIGN_ELM1_TO_ELM2,        \
+   * mod_obcs.F:178-199), kept when an Orlanski OBC node needs them        */ \
+  X(double, elm1, FVX_M1)  X(double, elm2, FVX_M1)
 
 #define FV_ALL_FIELDS(X) FV_GRID_FIELDS(X) FV_STATE_FIELDS(X)
 
@@ -276,8 +281,8 @@ int  fv_model_set_sponge(fv_model* mdl, int32_t n, const int32_t* nodes, const d
  * discharge enters through the solid edge, split between the edge's two
  * nodes by their control-volume areas; momentum enters the cell.         */
 int  fv_model_set_river_edge(fv_model* mdl, int32_t cell, double q, double t, double s);
-/* TYPE_OBC of the n = IOBCN open-boundary nodes, each 1..8 (mod_obcs.F:
- * 30-37; the Orlanski types 9/10 are rejected), and IUCP recomputed: 0 in
+/* TYPE_OBC of the n = IOBCN open-boundary nodes, each 1..10 (mod_obcs.F:
+ * 30-39), and IUCP recomputed: 0 in
  * the adjacent cells of odd-type nodes, 1 elsewhere (mod_obcs.F:959-962,
  * 993-996).  Replaces the reference's TYPE_OBC column of the OBC file.   */
 int  fv_model_set_obc_type(fv_model* mdl, int32_t n, const int32_t* types);

@@ -285,6 +285,8 @@ ExtArgs make_ext_args(fvgpu_handle* h, int K, const Ext2DState& cur, const Ext2D
   a.ws2x = h->wind_varies ? h->ws2x : h->g.wsx;
   a.ws2y = h->wind_varies ? h->ws2y : h->g.wsy;
   a.clk = h->clk;
+  a.elm1 = h->ia.elm1;
+  a.elm2 = h->ia.elm2;
   return a;
 }
 
@@ -312,6 +314,7 @@ void enqueue_external_step_t(fvgpu_handle* h, int iext, bool fresh) {
 #define EXTEL_CALL(SM) TLAUNCH("ext_elevation", k_ext_elevation<SM>, g.M + 1, g, a)
     FV_SMAX_DISPATCH(g.cv_max, EXTEL_CALL);
     if (g.obc_radiate) LAUNCH("ext_obc_radiate", k_ext_obc_radiate, nb(g.IOBCN), 256, g, a);
+    if (g.obc_ore) LAUNCH("elm_shift", k_elm_shift, nb(g.M + 1), 256, g, a);
     join_streams(h, FV_CONC_2D);
     TLAUNCH("ext_momentum", k_ext_momentum<FT>, g.N + 1, g, a);
   }
@@ -912,11 +915,15 @@ int upload_grid(fvgpu_handle* h) {
   UP(niec, 2 * cv1); UP(ntrg, cv1); UP(dltxe, cv1); UP(dltye, cv1); UP(dltxncve, 2 * cv1); UP(dltyncve, 2 * cv1);
   const size_t ob = md->iobcn, nt = md->ntide, nr = md->numqbc;
   for (size_t k = 0; k < ob; ++k)
-    if (md->type_obc[k] < 1 || md->type_obc[k] > 8) return fail("TYPE_OBC must be 1..8");
+    if (md->type_obc[k] < 1 || md->type_obc[k] > 10) return fail("TYPE_OBC must be 1..10");
   g.obc_radiate = 0;
-  for (size_t k = 0; k < ob; ++k) if (md->type_obc[k] >= 5) g.obc_radiate = 1;
+  g.obc_ore = 0;
+  for (size_t k = 0; k < ob; ++k) {
+    if (md->type_obc[k] >= 5) g.obc_radiate = 1;
+    if (md->type_obc[k] >= 9) g.obc_ore = 1;
+  }
   UP(i_obc_n, ob); UP(next_obc, ob); UP(nadjc_obc, ob); UP(adjc_obc, 2 * ob); UP(fluxf_obc, 2 * ob);
-  UP(type_obc, ob);
+  UP(type_obc, ob); UP(next_obc2, ob);
   UP(dltn_obc, ob); UP(emean, ob); UP(apt, ob * nt); UP(phai, ob * nt); UP(period, nt);
   for (size_t k = 0; k < nr; ++k)
     if ((md->inodeq[k] > 0) == (md->icellq[k] > 0))
@@ -1006,7 +1013,7 @@ int upload_state(fvgpu_handle* h) {
   a.elf = a.el;   // after the external loop ELF = EL
 #define RW(field, src, n) if (dupload_rw(h, &a.field, src, (n))) return 1
   RW(et, md->et, m1); RW(dt, md->dt, m1); RW(et1, md->et1, n1); RW(dt1, md->dt1, n1);
-  RW(egf, nullptr, m1); RW(dtfa, md->d, m1);
+  RW(egf, nullptr, m1); RW(dtfa, md->d, m1); RW(elm1, md->elm1, m1); RW(elm2, md->elm2, m1);
   RW(u, md->u, nk); RW(v, md->v, nk); RW(w, md->w, nk); RW(ww, md->ww, nk); RW(uf, nullptr, nk); RW(vf, nullptr, nk);
   RW(drhox, nullptr, nk); RW(drhoy, nullptr, nk); RW(rho, md->rho, nk); RW(rmean, md->rmean, nk);
   RW(t, md->t, nk); RW(s, md->s, nk); RW(km1, md->km1, nk); RW(viscofm, nullptr, nk);
@@ -1073,7 +1080,7 @@ double* lookup(fvgpu_handle* h, const std::string& n, size_t* len) {
   struct E { const char* n; double* p; size_t l; };
   const E t[] = {
       {"el", a.el, m1}, {"d", a.d, m1}, {"et", a.et, m1}, {"dt", a.dt, m1}, {"elf", a.el, m1},
-      {"egf", a.egf, m1}, {"dtfa", a.dtfa, m1},
+      {"egf", a.egf, m1}, {"dtfa", a.dtfa, m1}, {"elm1", a.elm1, m1}, {"elm2", a.elm2, m1},
       {"ua", a.ua, n1}, {"va", a.va, n1}, {"el1", a.el1, n1}, {"d1", a.d1, n1}, {"et1", a.et1, n1},
       {"dt1", a.dt1, n1}, {"adx2d", a.adx2d, n1}, {"ady2d", a.ady2d, n1}, {"drx2d", a.drx2d, n1},
       {"dry2d", a.dry2d, n1}, {"cbc", a.cbc, n1}, {"wubot", a.wubot, n1}, {"wvbot", a.wvbot, n1},

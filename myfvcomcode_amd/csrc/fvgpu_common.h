@@ -93,7 +93,9 @@ struct DevGrid {
   const int *ub_ns, *ub_elems, *ub_ne, *ub_edges, *ub_eloc, *ub_epos;
   // OBC / tide / rivers
   const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc, *type_obc;
-  int obc_radiate;                 // some OBC node has TYPE_OBC 5..8
+  const int *next_obc2;
+  int obc_radiate;                 // some OBC node has TYPE_OBC 5..10
+  int obc_ore;                     // some OBC node has TYPE_OBC 9/10 (Orlanski)
   const double *fluxf_obc, *dltn_obc, *emean, *apt, *phai, *period;
   const int *inodeq;               // node river: its node, else 0
   const int *icellq, *n_icellq;    // edge river: its cell and the two nodes of its solid edge, else 0

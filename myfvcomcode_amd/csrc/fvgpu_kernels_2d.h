@@ -51,6 +51,7 @@ struct ExtArgs {
   double* phi;           // per sub-edge EXTEL terms (read for the stage input,
                          // written by k_ext_momentum for the next stage)
   const double *ws2x, *ws2y;   // wind stress of the external step (before RAMP*0.001)
+  double *elm1, *elm2;         // the elevation one and two RK stages back (Orlanski OBC)
   DevClock* clk;
 };
 
@@ -219,18 +220,42 @@ __global__ void k_ext_elevation(DevGrid g, ExtArgs a) {
 // BCOND_GWI (TYPE_OBC 5/6, mod_obcs.F:624-658) and BCOND_BKI (7/8,
 // mod_obcs.F:665-699): the OBC node's ELF from its interior neighbour
 // NEXT_OBC's new ELF (formed by k_ext_elevation, which leaves these nodes
-// alone), one thread per OBC node.
+// alone); BCOND_ORE (9/10, mod_obcs.F:706-737): from the stage input EL
+// and the elevations ELM1/ELM2 one and two stages back.  One thread per
+// OBC node.
 __global__ void k_ext_obc_radiate(DevGrid g, ExtArgs a) {
   const int obc = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (obc >= g.IOBCN || g.type_obc[obc] < 5) return;
+  if (obc >= g.IOBCN) return;
+  const int ty = g.type_obc[obc];
+  if (ty < 5) return;
   const int i1 = g.i_obc_n[obc], i2 = g.next_obc[obc];
   const double elrk = a.base.el[i1];
-  const double cc = sqrt(g.grav_n[i1] * g.h[i1]) * g.dte / g.dltn_obc[obc] * a.alpha;
-  const double cp = cc + 1.0;
-  const double elf = g.type_obc[obc] <= 6 ? (cc * a.nxt.el[i2] + elrk) / cp
-                                          : (cc * a.nxt.el[i2] + elrk * (1.0 - g.dte / 10800.0)) / cp;
+  double elf;
+  if (ty >= 9) {
+    const int i3 = g.next_obc2[obc];
+    const double cl = (a.elm2[i2] - a.cur.el[i2]) / (a.cur.el[i2] + a.elm2[i2] - 2.0 * a.elm1[i3]);
+    double mu;                            // a NaN CL (0/0 at rest) gives MU = 0, as in the reference
+    if (cl >= 1.0) mu = 1.0;
+    else if (cl > 0.0 && cl < 1.0) mu = cl;
+    else mu = 0.0;
+    elf = (a.elm1[i1] * (1.0 - mu) + 2.0 * mu * a.cur.el[i2]) / (1.0 + mu);
+  } else {
+    const double cc = sqrt(g.grav_n[i1] * g.h[i1]) * g.dte / g.dltn_obc[obc] * a.alpha;
+    const double cp = cc + 1.0;
+    elf = ty <= 6 ? (cc * a.nxt.el[i2] + elrk) / cp : (cc * a.nxt.el[i2] + elrk * (1.0 - g.dte / 10800.0)) / cp;
+  }
   ext_obc_flux(g, a, i1, obc, elf, elrk, a.xflux_obcn[obc]);
   ext_node_store(g, a, i1, elf);
+}
+
+// ASSIGN_ELM1_TO_ELM2 (mod_obcs.F:178-199) at the end of a stage
+// (external_step.F:333): ELM2 = ELM1, ELM1 = the stage input EL.  Run
+// after k_ext_obc_radiate, the only reader.
+__global__ void k_elm_shift(DevGrid g, ExtArgs a) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > g.M) return;
+  a.elm2[i] = a.elm1[i];
+  a.elm1[i] = a.cur.el[i];
 }
 
 // ADVAVE_EDGE_GCN edge loop body (advave_edge_gcn.F:546-884) for edge e of

@@ -25,6 +25,7 @@ struct IntArgs {
   double *l, *km, *kh, *kq, *wts, *viscofh;
   // node-layer scratch: ring gradients and bounds
   double* gx;            // node-layer scratch (PROD of VDIF_Q)
+  double *elm1, *elm2;   // the elevation one and two RK stages back (Orlanski OBC)
   // flux buffers, of the flux type FT (double, or float in the mixed form)
   void* ef2;             // per-edge ADVAVE terms (2-D), EdgeFlux4T<FT>
   double* elij2;         // per-edge mean elevation (2-D)

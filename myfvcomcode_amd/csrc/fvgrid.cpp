@@ -572,6 +572,23 @@ int build_metrics(fv_model* md, const double* x, const double* y, const int32_t*
     }
     if (nextn == 0) return fail("no interior neighbour for OBC node");
     md->next_obc[k] = nextn;
+    // NEXT_OBC2: NEXT_OBC's ring neighbour off the boundary most normal to it
+    // (mod_obcs.F:1072-1101, the direction normalised with a 1e-9 guard)
+    {
+      double dotmax2 = -1.0;
+      int next2 = 0;
+      const int o2 = md->nbsn_off[nextn];
+      for (int j = 0; j < md->ntsn[nextn] - 1; ++j) {
+        const int jn = md->nbsn[o2 + j];
+        if (md->isonb[jn] == 2) continue;
+        const double dxt = md->vx[jn] - md->vx[nextn], dyt = md->vy[jn] - md->vy[nextn];
+        const double dxn = dxt / (std::sqrt(dxt * dxt + dyt * dyt) + 1.0e-9);
+        const double dyn = dyt / (std::sqrt(dxt * dxt + dyt * dyt) + 1.0e-9);
+        const double dot = dxn * nxo + dyn * nyo;
+        if (dot > dotmax2) { dotmax2 = dot; next2 = jn; }
+      }
+      md->next_obc2[k] = next2;
+    }
     md->dltn_obc[k] = std::fabs((md->vx[nextn] - md->vx[inode]) * nxo + (md->vy[nextn] - md->vy[inode]) * nyo);
   }
 
@@ -983,8 +1000,7 @@ extern "C" int fv_model_set_obc_type(fv_model* md, int32_t n, const int32_t* typ
   if (!md) return fail("fv_model_set_obc_type: null model");
   if (n != md->iobcn) return fail("fv_model_set_obc_type: n must equal IOBCN");
   for (int k = 0; k < n; ++k)
-    if (types[k] < 1 || types[k] > 8)
-      return fail("fv_model_set_obc_type: TYPE_OBC must be 1..8 (Orlanski types 9/10 are not built)");
+    if (types[k] < 1 || types[k] > 10) return fail("fv_model_set_obc_type: TYPE_OBC must be 1..10");
   for (int k = 0; k < n; ++k) md->type_obc[k] = types[k];
   // IUCP: SETUP_OBC marks the adjacent cells of linear (odd-type) nodes
   for (int i = 0; i <= md->n; ++i) md->iucp[i] = 1;
@@ -1107,7 +1123,8 @@ namespace {
 enum RefKind { REF_NONE = 0, REF_NODE, REF_ELEM };
 
 RefKind field_ref(const char* name) {
-  static const char* node_refs[] = {"nbsn", "inodeq", "i_obc_n", "next_obc", "ienode", "niec", "nv", "n_icellq"};
+  static const char* node_refs[] = {"nbsn", "inodeq", "i_obc_n", "next_obc", "next_obc2", "ienode", "niec", "nv",
+                                    "n_icellq"};
   static const char* elem_refs[] = {"nbe", "nbve", "iec", "ntrg", "adjc_obc", "icellq"};
   for (const char* n : node_refs) if (std::strcmp(n, name) == 0) return REF_NODE;
   for (const char* n : elem_refs) if (std::strcmp(n, name) == 0) return REF_ELEM;

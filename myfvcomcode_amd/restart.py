@@ -35,7 +35,7 @@ STATE = ["el", "d", "et", "dt", "egf", "dtfa",
          "wvbot", "uard", "vard",
          "u", "v", "w", "ww", "uf", "vf", "drhox", "drhoy", "rho", "rmean", "t", "s", "km1", "viscofm",
          "t1", "s1", "tf1", "sf1", "rho1", "rmean1", "tmean1", "smean1", "q2", "q2l", "q2f", "q2lf", "l",
-         "km", "kh", "kq", "wts", "viscofh"]
+         "km", "kh", "kq", "wts", "viscofh", "elm1", "elm2"]
 
 # FVCOM restart name, horizontal location, vertical coordinate (None: 2-D)
 FVCOM = {"el": ("zeta", "node", None), "et": ("et", "node", None),
@@ -120,7 +120,8 @@ def load(path, model):
             raise ValueError("%s holds a %dx%dx%d state, the model is %dx%dx%d" % (
                 path, dims["node"], dims["nele"], dims["siglev"], model.m, model.n, model.kb))
         iint = int(f.variables["iint"][0])
-        arrays = {name: np.array(f.variables["state_" + name][0, :], np.float64) for name in STATE}
+        arrays = {name: np.array(f.variables["state_" + name][0, :], np.float64) for name in STATE
+                  if "state_" + name in f.variables}   # files written before ELM1/ELM2 lack them
     finally:
         f.close()
     model.hdr.p.iint0 = iint

@@ -72,7 +72,7 @@ class Oracle:
 
     def _len(self, name):
         m1, n1, kb = self.model.m + 1, self.model.n + 1, self.model.kb
-        node2 = {"el", "et", "d", "dt", "elf", "dtfa", "egf", "elrk"}
+        node2 = {"el", "et", "d", "dt", "elf", "dtfa", "egf", "elrk", "elm1", "elm2"}
         elem2 = {"ua", "va", "el1", "et1", "d1", "dt1", "uard", "vard", "adx2d", "ady2d", "drx2d",
                  "dry2d", "advua", "advva", "pstx", "psty", "cbc", "wubot", "wvbot", "wusurf",
                  "wvsurf", "wusurf2", "wvsurf2", "uaf", "vaf", "uark", "vark", "elrk1", "elf1"}

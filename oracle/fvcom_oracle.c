@@ -39,7 +39,8 @@ typedef struct orc {
   int iext;
   double ramp, int_time, ext_time, rk_time;
   /* state, 2-D nodes */
-  double *el, *et, *d, *dt, *elf, *elrk, *dtfa, *egf;
+  double *el, *et, *d, *dt, *elf, *elrk, *dtfa, *egf, *elm1, *elm2;
+  int has_ore;   /* some OBC node is of TYPE_OBC 9/10 */
   /* state, 2-D elements */
   double *ua, *va, *el1, *et1, *d1, *dt1, *elf1, *elrk1, *uaf, *vaf, *uark, *vark;
   double *uard, *vard, *adx2d, *ady2d, *drx2d, *dry2d, *advua, *advva, *pstx, *psty;
@@ -122,7 +123,8 @@ orc* orc_create(const fv_model* g) {
 #define AN(x) o->x = zal(o->n1)
 #define AMK(x) o->x = zal(mk)
 #define ANK(x) o->x = zal(nk)
-  AM(el); AM(et); AM(d); AM(dt); AM(elf); AM(elrk); AM(dtfa); AM(egf);
+  AM(el); AM(et); AM(d); AM(dt); AM(elf); AM(elrk); AM(dtfa); AM(egf); AM(elm1); AM(elm2);
+  for (int i = 0; i < o->IOBCN; ++i) if (g->type_obc[i] >= 9) o->has_ore = 1;
   AN(ua); AN(va); AN(el1); AN(et1); AN(d1); AN(dt1); AN(elf1); AN(elrk1); AN(uaf); AN(vaf);
   AN(uark); AN(vark); AN(uard); AN(vard); AN(adx2d); AN(ady2d); AN(drx2d); AN(dry2d);
   AN(advua); AN(advva); AN(pstx); AN(psty); AN(fluxobn); AN(cbc); AN(wubot); AN(wvbot);
@@ -140,7 +142,7 @@ orc* orc_create(const fv_model* g) {
   o->qdis = zal(o->NUMQBC);
   /* state from the model */
 #define CPY(x, n) memcpy(o->x, g->x, sizeof(double) * (n))
-  CPY(el, o->m1); CPY(et, o->m1); CPY(d, o->m1); CPY(dt, o->m1);
+  CPY(el, o->m1); CPY(et, o->m1); CPY(d, o->m1); CPY(dt, o->m1); CPY(elm1, o->m1); CPY(elm2, o->m1);
   CPY(ua, o->n1); CPY(va, o->n1); CPY(el1, o->n1); CPY(et1, o->n1); CPY(d1, o->n1); CPY(dt1, o->n1);
   CPY(u, nk); CPY(v, nk); CPY(w, nk); CPY(ww, nk); CPY(t, nk); CPY(s, nk); CPY(rho, nk);
   CPY(rmean, nk); CPY(km1, nk);
@@ -152,7 +154,7 @@ orc* orc_create(const fv_model* g) {
 
 void orc_destroy(orc* o) {
   if (!o) return;
-  double** f[] = {&o->el, &o->et, &o->d, &o->dt, &o->elf, &o->elrk, &o->dtfa, &o->egf, &o->ua, &o->va,
+  double** f[] = {&o->el, &o->elm1, &o->elm2, &o->et, &o->d, &o->dt, &o->elf, &o->elrk, &o->dtfa, &o->egf, &o->ua, &o->va,
                   &o->el1, &o->et1, &o->d1, &o->dt1, &o->elf1, &o->elrk1, &o->uaf, &o->vaf, &o->uark,
                   &o->vark, &o->uard, &o->vard, &o->adx2d, &o->ady2d, &o->drx2d, &o->dry2d, &o->advua,
                   &o->advva, &o->pstx, &o->psty, &o->fluxobn, &o->cbc, &o->wubot, &o->wvbot, &o->wusurf,
@@ -174,6 +176,7 @@ static double* orc_lookup(orc* o, const char* name, size_t* len) {
   struct { const char* n; double* p; size_t l; } t[] = {
       {"el", o->el, o->m1}, {"et", o->et, o->m1}, {"d", o->d, o->m1}, {"dt", o->dt, o->m1},
       {"elf", o->elf, o->m1}, {"dtfa", o->dtfa, o->m1}, {"egf", o->egf, o->m1},
+      {"elm1", o->elm1, o->m1}, {"elm2", o->elm2, o->m1},
       {"ua", o->ua, o->n1}, {"va", o->va, o->n1}, {"el1", o->el1, o->n1}, {"et1", o->et1, o->n1},
       {"d1", o->d1, o->n1}, {"dt1", o->dt1, o->n1}, {"uard", o->uard, o->n1}, {"vard", o->vard, o->n1},
       {"adx2d", o->adx2d, o->n1}, {"ady2d", o->ady2d, o->n1}, {"drx2d", o->drx2d, o->n1},
@@ -358,12 +361,21 @@ static void bcond_asl(orc* o) {
 /* The rest of BCOND_GCN(1,K) (bcond_gcn.F:117-122): BCOND_ASL_CLP on the
  * nodes of TYPE_OBC 3/4 (mod_obcs.F:599-617), BCOND_GWI on 5/6
  * (mod_obcs.F:624-658) and BCOND_BKI on 7/8 (mod_obcs.F:665-699), the
- * radiation conditions taking the interior neighbour's new ELF. */
+ * radiation conditions taking the interior neighbour's new ELF, and
+ * BCOND_ORE on 9/10 (mod_obcs.F:706-737) from EL, ELM1 and ELM2. */
 static void bcond_obc_other(orc* o, int K) {
   for (int jn = 0; jn < o->IOBCN; ++jn) {
     const int ty = G->type_obc[jn], i1 = G->i_obc_n[jn], i2 = G->next_obc[jn];
     if (ty == 3 || ty == 4) {
       o->elf[i1] = 0.0;
+    } else if (ty >= 9) {
+      int i3 = G->next_obc2[jn];
+      double cl = (o->elm2[i2] - o->el[i2]) / (o->el[i2] + o->elm2[i2] - 2.0 * o->elm1[i3]);
+      double mu;
+      if (cl >= 1.0) mu = 1.0;
+      else if (cl > 0.0 && cl < 1.0) mu = cl;
+      else mu = 0.0;
+      o->elf[i1] = (o->elm1[i1] * (1.0 - mu) + 2.0 * mu * o->el[i2]) / (1.0 + mu);
     } else if (ty >= 5) {
       double cc = sqrt(G->grav_n[i1] * G->h[i1]) * o->dte / G->dltn_obc[jn] * ALPHA_RK[K];
       double cp = cc + 1.0;
@@ -512,6 +524,10 @@ void orc_external_step(orc* o) {
     advave_edge_gcn(o, o->advua, o->advva);
     extuv_edge(o, K);
     for (int i = 1; i <= o->N; ++i) wall_project(o, i, &o->uaf[i], &o->vaf[i]);  /* BCOND_GCN(2,K) */
+    if (o->has_ore) {   /* ASSIGN_ELM1_TO_ELM2 (external_step.F:333, mod_obcs.F:187-188) */
+      memcpy(o->elm2, o->elm1, sizeof(double) * M1);
+      memcpy(o->elm1, o->el, sizeof(double) * M1);
+    }
     memcpy(o->el, o->elf, sizeof(double) * M1);
     memcpy(o->el1, o->elf1, sizeof(double) * N1);
     n2e2d(o, o->el, o->el1);

@@ -67,17 +67,17 @@ def test_internal_step_parity_many(built):
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_open_boundary_types_parity(built, graph):
-    """Every TYPE_OBC the build takes (1..8: elevation specified, clamped,
-    gravity-wave and Blumberg-Kantha radiation, each linear and nonlinear)
+    """Every TYPE_OBC (1..10: elevation specified, clamped, gravity-wave,
+    Blumberg-Kantha and Orlanski radiation, each linear and nonlinear)
     along one open boundary of the channel."""
     import myfvcomcode_amd as F
     from oracle import Oracle
     m = F.Model.build(0)
-    types = [(k % 8) + 1 for k in range(m.hdr.iobcn)]
+    types = [(k % 10) + 1 for k in range(m.hdr.iobcn)]
     m.set_obc_type(types)
     dev, orc = F.DeviceModel(m), Oracle(m)
     n = 10
-    print("case: 81x21 estuary channel, TYPE_OBC 1..8 along the boundary, %d steps, graph=%s" % (n, graph),
+    print("case: 81x21 estuary channel, TYPE_OBC 1..10 along the boundary, %d steps, graph=%s" % (n, graph),
           flush=True)
     dev.run(n, graph=graph)
     orc.run(n)

@@ -1,4 +1,4 @@
-"""Open-boundary types of the external mode (TYPE_OBC, mod_obcs.F:30-37) on
+"""Open-boundary types of the external mode (TYPE_OBC, mod_obcs.F:30-39) on
 the oracle, pinned without the oracle's own formulas where the physics
 fixes the answer:
 
@@ -12,7 +12,11 @@ fixes the answer:
                 624-699); after a whole external step the last stage's
                 relation holds between the new EL and ELRK, the elevation
                 the step started from
-  radiation     a gravity wave that reaches a radiating boundary (5, 7)
+  ORE           types 9/10 (mod_obcs.F:706-737): the boundary value is a
+                convex combination of its own elevation a stage back and
+                the neighbour's stage input; after a step these are ELM2 of
+                the node and ELM1 of NEXT_OBC
+  radiation     a gravity wave that reaches a radiating boundary (5, 7, 9)
                 leaves the basin, while a clamped boundary (3) reflects it:
                 the energy left in the basin after the wave's round trip
                 tells them apart (Blumberg and Kantha 1985)
@@ -55,7 +59,7 @@ def test_set_obc_type_sets_iucp_by_parity(built):
     m.set_obc_type(np.full(nob, 2))
     assert np.all(m.field("iucp")[1:] == 1)
     # mixed: exactly the cells of odd-type nodes
-    types = np.array([(k % 8) + 1 for k in range(nob)], np.int32)
+    types = np.array([(k % 10) + 1 for k in range(nob)], np.int32)
     m.set_obc_type(types)
     want = np.ones(m.n + 1, np.int32)
     for k in range(nob):
@@ -65,7 +69,7 @@ def test_set_obc_type_sets_iucp_by_parity(built):
     assert np.array_equal(m.field("type_obc"), types)
 
 
-@pytest.mark.parametrize("bad", [0, 9, 10])
+@pytest.mark.parametrize("bad", [0, 11])
 def test_set_obc_type_refuses_unbuilt_types(built, bad):
     m = _channel()
     t = np.ones(m.hdr.iobcn, np.int32)
@@ -150,7 +154,30 @@ def test_radiating_boundary_lets_a_wave_leave(built):
     e0, e_clp = _wave_energy(3, steps)
     _, e_gwi = _wave_energy(5, steps)
     _, e_bki = _wave_energy(7, steps)
-    print("energy left: start %.4e  clamped %.4e  GWI %.4e  BKI %.4e" % (e0, e_clp, e_gwi, e_bki))
+    _, e_ore = _wave_energy(9, steps)
+    print("energy left: start %.4e  clamped %.4e  GWI %.4e  BKI %.4e  ORE %.4e" % (e0, e_clp, e_gwi, e_bki,
+                                                                                e_ore))
     assert e_clp > 0.8 * e0          # reflected (little friction on this time scale)
     assert e_gwi < 0.75 * e_clp      # the east-going half (about half the energy) has left
     assert e_bki < 0.75 * e_clp
+    assert e_ore < 0.75 * e_clp
+
+
+@pytest.mark.parametrize("ty", [9, 10])
+def test_orlanski_value_is_a_convex_combination(built, ty):
+    m = _bump_channel(ty, 3000.0)
+    assert np.all(m.field("next_obc2") > 0)
+    o = Oracle(m)
+    obc, nxt = m.field("i_obc_n"), m.field("next_obc")
+    isonb = m.field("isonb")
+    assert np.all(isonb[m.field("next_obc2")] != 2)
+    moved = 0
+    for _ in range(60):
+        o.external_step(1)
+        el, elm1, elm2 = o.get("el"), o.get("elm1"), o.get("elm2")
+        a, b = elm2[obc], elm1[nxt]        # ELM1(I1) and EL(I2) of the last stage
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        tol = 1e-15 * np.maximum(1.0, np.abs(hi))
+        assert np.all(el[obc] >= lo - tol) and np.all(el[obc] <= hi + tol)
+        moved += int(np.any(el[obc] != 0.0))
+    assert moved > 10                      # the boundary does move as the bump arrives
