@@ -375,6 +375,9 @@ int  fvgpu_kernel_stat(fvgpu_handle* h, int32_t idx, const char** name, double* 
 int  fvgpu_set_profiling(fvgpu_handle* h, int32_t on);
 /* Restrict the timing events to one kernel family (NULL or "": all).     */
 int  fvgpu_profile_only(fvgpu_handle* h, const char* name);
+/* Time one in `every` launches of the timed families (1: every launch);
+ * the averages are then over that sample.                                */
+int  fvgpu_profile_every(fvgpu_handle* h, int32_t every);
 int  fvgpu_synchronize(fvgpu_handle* h);
 /* Measurement aid: average time of a plain device copy of n doubles
  * (k_copy, the same launch shape as the step's copies) over reps launches;

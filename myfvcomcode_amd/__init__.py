@@ -143,6 +143,7 @@ def gpu_lib():
         L.fvmlp_infer.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32, fp]
         L.fvmlp_destroy.argtypes = [vp]
         L.fvgpu_profile_only.argtypes = [vp, C.c_char_p]
+        L.fvgpu_profile_every.argtypes = [vp, C.c_int32]
         L.fvgpu_synchronize.argtypes = [vp]
         L.fvgpu_stream_copy.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
         L.fvgpu_destroy.argtypes = [vp]
@@ -484,10 +485,12 @@ class DeviceModel:
             i += 1
         return out
 
-    def set_profiling(self, on, kernel=None):
-        """HIP events around launches (all families, or only ``kernel``)."""
+    def set_profiling(self, on, kernel=None, every=1):
+        """HIP events around launches (all families, or only ``kernel``),
+        around one in ``every`` of them."""
         L = gpu_lib()
         _check(L, L.fvgpu_profile_only(self._h, (kernel or "").encode()), "fvgpu_profile_only")
+        _check(L, L.fvgpu_profile_every(self._h, int(every)), "fvgpu_profile_every")
         _check(L, L.fvgpu_set_profiling(self._h, 1 if on else 0), "fvgpu_set_profiling")
 
     def stream_copy(self, n_doubles, reps=20):

@@ -115,6 +115,8 @@ struct fvgpu_handle {
   // profiling
   bool profiling = false;
   std::string profile_only;    // empty: every kernel family
+  int profile_every = 1;       // time one in this many launches of the timed families
+  long profile_seen = 0;       // launches of the timed families since profiling began
   std::map<std::string, KernelStat> stats;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
@@ -178,6 +180,7 @@ struct Launch {
   bool on;
   Launch(fvgpu_handle* h_, const char* n) : h(h_), name(n) {
     on = h->profiling && (h->profile_only.empty() || h->profile_only == n);
+    if (on && h->profile_every > 1) on = h->profile_seen++ % h->profile_every == 0;
     if (on) { e0 = next_event(h); (void)hipEventRecord(e0, h->cur); }
   }
   ~Launch() {
@@ -1338,7 +1341,13 @@ int fvgpu_kernel_stat(fvgpu_handle* h, int32_t idx, const char** name, double* a
 int fvgpu_set_profiling(fvgpu_handle* h, int32_t on) {
   if (h->profiling && flush_profile(h)) return 1;
   h->profiling = on != 0;
-  if (h->profiling) h->stats.clear();   // statistics stay readable after switching off
+  if (h->profiling) { h->stats.clear(); h->profile_seen = 0; }   // statistics stay readable after switching off
+  return 0;
+}
+
+int fvgpu_profile_every(fvgpu_handle* h, int32_t every) {
+  if (every < 1) return fail("fvgpu_profile_every: every must be >= 1");
+  h->profile_every = every;
   return 0;
 }
 

@@ -39,3 +39,26 @@ def test_concurrency_off_is_bit_identical(built):
     b.run(2, graph=True)
     for name in ("el", "ua", "va", "u", "v", "w", "t1", "s1", "q2", "q2l", "l", "km", "kh", "kq"):
         assert np.array_equal(a.get(name), b.get(name)), name
+
+
+def test_profiling_times_one_in_n_launches(built):
+    """Timing events around one in `every` launches of a family: the count
+    is the sample, the average a device time, the results unchanged."""
+    m = F.Model.build(0)
+    print("case: 81x21 channel, 2 eager steps timing 1 in 4 ext_momentum launches", flush=True)
+    a = F.DeviceModel(m)
+    a.set_profiling(True, kernel="ext_momentum", every=4)
+    a.run(2, graph=False)
+    a.set_profiling(False)
+    stats = a.kernel_stats()
+    us, cnt = stats["ext_momentum"]
+    per_step = 4 * m.hdr.p.isplit
+    assert cnt == 2 * per_step // 4
+    assert 0.0 < us < 1e5
+    assert set(stats) == {"ext_momentum"}
+    b = F.DeviceModel(m)
+    b.run(2, graph=False)
+    for name in ("el", "ua", "va", "u", "t1"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    with pytest.raises(RuntimeError):
+        a.set_profiling(True, kernel="ext_momentum", every=0)
