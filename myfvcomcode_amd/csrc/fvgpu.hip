@@ -386,7 +386,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   LAUNCH("int_clock", k_int_clock, 1, 64, g, h->clk, a.qdis);
 #define ADJ1_CALL(KM) TLAUNCH("adjust1_brough", k_adjust1_brough<KM>, g.N + 1, g, a)
   FV_KMAX_DISPATCH(g.KBM1, ADJ1_CALL);
-  LAUNCH("adv3d_blk", (k_uv_blk<0, FT>), g.ub_nblk, 256, g, a);
+  LAUNCH("adv3d_blk", (k_uv_blk<0, FT>), g.ub_tile.nblk, 256, g, a);
   TLAUNCH("baropg", k_baropg, g.N + 1, g, a, (int)h->md->p.barotropic);
   if (!h->md->p.barotropic) TLAUNCH("rho1_roundtrip", k_rho1_roundtrip, (long)g.KBM1 * g.m1, g, a);
   TLAUNCH("advave_edge", k_advave_edge<FT>, g.NE + 1, g, a.ua, a.va, a.d, a.el, a.ef2, a.elij2);
@@ -401,7 +401,7 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
   N2E3D(a.wts, a.w);
 #define VISCOFH_CALL(RM) TLAUNCH("viscofh", k_viscofh_r<RM>, g.M + 1, g, a)
   FV_RMAX_DISPATCH(g.ve_max, VISCOFH_CALL);
-  LAUNCH("advuv_blk", (k_uv_blk<1, FT>), g.ub_nblk, 256, g, a);
+  LAUNCH("advuv_blk", (k_uv_blk<1, FT>), g.ub_run.nblk, 256, g, a);
 #define VDIF_UV_CALL(KM) LAUNCH("vdif_uv", k_vdif_uv_r<KM>, dim3(nb(g.N + 1), 2), 256, g, a)
   FV_KMAX_DISPATCH(g.KBM1, VDIF_UV_CALL);
   TLAUNCH("uv_finish", k_uv_finish, (long)g.N * g.KBM1, g, a);
@@ -594,6 +594,7 @@ int build_lists(fvgpu_handle* h) {
   if (dupload(h, &g.node_obc, nobc.data(), nobc.size())) return 1;
   if (dupload(h, &g.node_riv, nriv.data(), nriv.size())) return 1;
   if (dupload(h, &g.fct_skip, skip.data(), skip.size())) return 1;
+  std::vector<std::vector<int>> tile_own;   // the element tiles below, for the momentum blocks too
   // element blocks of the fused record + face-flux kernels: recursive
   // coordinate bisection of the element centroids (each range halved at the
   // median of its longer extent) down to blocks of at most FV_CVB elements
@@ -695,6 +696,9 @@ int build_lists(fvgpu_handle* h) {
       bne[b] = hi - lo;
     }
     g.cvb_nblk = nblk;
+    tile_own.assign(nblk, {});
+    for (int b = 0; b < nblk; ++b)
+      tile_own[b].assign(belems.begin() + (size_t)b * FV_CVB, belems.begin() + (size_t)b * FV_CVB + bne[b]);
     // node tiles (VERTVL): recursive coordinate bisection of the nodes down
     // to tiles of at most FV_NTN nodes whose elements (every element that
     // has one of them as a vertex) number at most FV_NTE; each owned node's
@@ -788,25 +792,26 @@ int build_lists(fvgpu_handle* h) {
     if (dupload(h, &g.cvb_loc, loc.data(), loc.size())) return 1;
     if (dupload(h, &g.rec_keep, keep.data(), keep.size())) return 1;
   }
-  // element blocks of the blocked ADVECTION / ADV_UV kernels: the largest
-  // block size (at most 128 elements, two lanes each, a multiple of 8)
-  // whose blocks all need at most 256 records (their own elements and the
-  // other elements of their edges) and at most 256 edges
+  // element blocks of the blocked ADVECTION / ADV_UV kernels.  ADVECTION
+  // takes the element tiles above (compact, so few neighbours form their
+  // records twice) if every tile needs at most 256 records (its own
+  // elements and the other elements of their edges) and at most 256 edges;
+  // ADV_UV, whose element part reads and writes many element-layer
+  // arrays, takes runs of consecutive elements (contiguous addresses), the
+  // largest run length (a multiple of 8, at most 128) that keeps every
+  // block within both, and so does ADVECTION where a tile would not fit
   {
-    std::vector<int> bns, belems, bne, bedges, beloc, epos;
+    std::vector<int> bns, bnown, belems, bne, bedges, beloc, epos;
     std::vector<int> rslot(N + 1, -1), rblk(N + 1, -1), eslot(NE + 1, -1), eblk(NE + 1, -1);
-    int eb = 128;
-    for (;;) {
-      if (eb < 8) return fail("no element block size keeps the blocked momentum kernels within 256 records");
-      const int nblk = (N + eb - 1) / eb;
-      bns.assign(nblk, 0); bne.assign(nblk, 0);
+    auto build_blocks = [&](const std::vector<std::vector<int>>& own) {
+      const int nblk = (int)own.size();
+      bns.assign(nblk, 0); bnown.assign(nblk, 0); bne.assign(nblk, 0);
       belems.assign((size_t)nblk * 256, 0); bedges.assign((size_t)nblk * 256, 1); beloc.assign((size_t)nblk * 256, 0);
       epos.assign((size_t)N + 1, 0);
       std::fill(rblk.begin(), rblk.end(), -1);
       std::fill(eblk.begin(), eblk.end(), -1);
       int worst = 0;
       for (int b = 0; b < nblk; ++b) {
-        const int e0 = b * eb + 1, e1 = std::min(N, (b + 1) * eb);
         int ns = 0, ne = 0;
         auto slot_of = [&](int x) {
           if (rblk[x] != b) {
@@ -816,8 +821,8 @@ int build_lists(fvgpu_handle* h) {
           }
           return rslot[x];
         };
-        for (int e = e0; e <= e1; ++e) slot_of(e);   // own elements first, in order
-        for (int e = e0; e <= e1; ++e) {
+        for (int e : own[b]) slot_of(e);   // own elements first, in order
+        for (int e : own[b]) {
           int packed = 0;
           for (int q = 0; q < 3; ++q) {
             const int c = ed[3 * (size_t)e + q] >> 1;
@@ -836,20 +841,37 @@ int build_lists(fvgpu_handle* h) {
           epos[e] = packed;
         }
         bns[b] = ns;
+        bnown[b] = (int)own[b].size();
         bne[b] = ne;
         worst = std::max(worst, std::max(ns, ne));
       }
-      if (worst <= 256) { g.ub_nblk = nblk; break; }
+      return worst;
+    };
+    auto upload_blocks = [&](UvBlocks& u) {
+      u.nblk = (int)bns.size();
+      return dupload(h, &u.nown, bnown.data(), bnown.size()) || dupload(h, &u.ns, bns.data(), bns.size()) ||
+             dupload(h, &u.elems, belems.data(), belems.size()) || dupload(h, &u.ne, bne.data(), bne.size()) ||
+             dupload(h, &u.edges, bedges.data(), bedges.size()) ||
+             dupload(h, &u.eloc, beloc.data(), beloc.size()) || dupload(h, &u.epos, epos.data(), epos.size());
+    };
+    int eb = 128;
+    for (;;) {
+      if (eb < 8) return fail("no element block size keeps the blocked momentum kernels within 256 records");
+      std::vector<std::vector<int>> runs((N + eb - 1) / eb);
+      for (int e = 1; e <= N; ++e) runs[(e - 1) / eb].push_back(e);
+      const int worst = build_blocks(runs);
+      if (worst <= 256) break;
       // shrink in proportion, to a multiple of 8, by at least 8
       eb = std::min(eb - 8, (int)((long)eb * 256 / worst) / 8 * 8);
     }
-    g.ub_eb = eb;
-    if (dupload(h, &g.ub_ns, bns.data(), bns.size())) return 1;
-    if (dupload(h, &g.ub_elems, belems.data(), belems.size())) return 1;
-    if (dupload(h, &g.ub_ne, bne.data(), bne.size())) return 1;
-    if (dupload(h, &g.ub_edges, bedges.data(), bedges.size())) return 1;
-    if (dupload(h, &g.ub_eloc, beloc.data(), beloc.size())) return 1;
-    if (dupload(h, &g.ub_epos, epos.data(), epos.size())) return 1;
+    if (upload_blocks(g.ub_run)) return 1;
+    bool tiles_fit = true;
+    for (const auto& t : tile_own) tiles_fit = tiles_fit && t.size() <= 128;
+    if (tiles_fit && build_blocks(tile_own) <= 256) {
+      if (upload_blocks(g.ub_tile)) return 1;
+    } else {
+      g.ub_tile = g.ub_run;
+    }
   }
   // element -> rivers in river order: NBVE(J,1) then NBVE(J,NTVE(J)) of a
   // node river's node, an edge river's own cell

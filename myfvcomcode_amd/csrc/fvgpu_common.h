@@ -22,6 +22,18 @@
 #define FV_NTN 64      // owned nodes per node tile
 #define FV_NTE 192     // elements per node tile
 
+// Element blocks of the blocked ADVECTION / ADV_UV kernels: block b owns
+// nown[b] (<= 128) elements, its records 0 .. nown[b]-1, followed by the
+// other elements of their edges: ns[b] (<= 256) records, element
+// elems[256*b + t]; its ne[b] (<= 256) edges edges[256*b + t], with the
+// record slots of IA and IB in bits 0-7 and 8-15 of eloc[256*b + t];
+// epos[e] packs the edge slots of element e's three edges (in ed_ent
+// order) in bits 0-7, 8-15, 16-23.
+struct UvBlocks {
+  int nblk;
+  const int *nown, *ns, *elems, *ne, *edges, *eloc, *epos;
+};
+
 struct DevGrid {
   int M, N, NE, NCV, KB, KBM1, KBM2, IOBCN, NUMQBC, NTIDE;
   long m1, n1;
@@ -82,15 +94,9 @@ struct DevGrid {
   // | side, in the order of cvl_ent.
   int nt_nblk;
   const int *nt_nown, *nt_nodes, *nt_nel, *nt_elems, *nt_lpos;
-  // Element blocks of the blocked ADVECTION / ADV_UV kernels: block b owns
-  // elements b*ub_eb+1 .. (ub_eb <= 128; record slots 0 ..), followed by
-  // the other elements of their edges: ub_ns[b] (<= 256) records, element
-  // ub_elems[256*b + t]; its ub_ne[b] (<= 256) edges ub_edges[256*b + t],
-  // with the record slots of IA and IB in bits 0-7 and 8-15 of
-  // ub_eloc[256*b + t]; ub_epos[e] packs the edge slots of element e's
-  // three edges (in ed_ent order) in bits 0-7, 8-15, 16-23.
-  int ub_eb, ub_nblk;
-  const int *ub_ns, *ub_elems, *ub_ne, *ub_edges, *ub_eloc, *ub_epos;
+  // Element blocks of the blocked ADVECTION / ADV_UV kernels (UvBlocks):
+  // compact tiles for ADVECTION, runs of consecutive elements for ADV_UV.
+  UvBlocks ub_tile, ub_run;
   // OBC / tide / rivers
   const int *i_obc_n, *next_obc, *nadjc_obc, *adjc_obc, *type_obc;
   const int *next_obc2;

@@ -11,16 +11,18 @@
 #pragma once
 #include "fvgpu_kernels_cv.h"
 
-// One workgroup per block of EB <= 128 consecutive elements (the block
-// tables ub_* of DevGrid).  Per layer, lane t
+// One workgroup per block of at most 128 own elements (the block tables
+// UvBlocks of DevGrid: for ADVECTION the compact element tiles of the
+// recursive coordinate bisection, for ADV_UV runs of consecutive
+// elements).  Per layer, lane t
 //   1. forms the least-squares gradient record of the block's t-th element
 //      (its own elements first, then the other element of each of their
 //      edges) into LDS: U, V, the four gradient terms COFA1/2/5/6 and the
 //      element's viscosity factor ART*sqrt(..)*CC_HVC;
 //   2. evaluates the block's t-th edge once from the records of its IA and
 //      IB, each in its reference role, into LDS;
-//   3. finishes one component of one own element: lanes 0..EB-1 the x
-//      sums of elements 1..EB of the block, lanes EB..2EB-1 the y sums, each
+//   3. finishes one component of one own element: lanes 0..127 the x
+//      sums of the block's own elements, lanes 128..255 the y sums, each
 //      summing its three edges in increasing edge index -- the reference's
 //      scatter order -- with the reference's operations (a component's
 //      operands are selected, never combined), so the sums are
@@ -51,16 +53,17 @@ __global__ __launch_bounds__(256) void k_uv_blk(DevGrid g, IntArgs a) {
   if (MODE == 1 && b == 0 && t == 0)
     for (int k = 1; k <= g.KB; ++k) a.viscofm[EI(g, 0, k)] = 0.0;
   // 1. record part: element x
-  const bool ract = t < g.ub_ns[b];
-  const int x = ract ? g.ub_elems[256L * b + t] : 0;
+  const UvBlocks& B = MODE == 0 ? g.ub_tile : g.ub_run;
+  const bool ract = t < B.ns[b];
+  const int x = ract ? B.elems[256L * b + t] : 0;
   const int x1 = NBE_(x, 1), x2 = NBE_(x, 2), x3 = NBE_(x, 3);
   FT ra1[4], ra2[4];
   lsq_coef<FT>(g, x, ra1, ra2);
   const FT rart = (FT)g.art[x], rcc = (FT)g.cc_hvc[x];
   // 2. edge part: edge c
-  const bool eact = t < g.ub_ne[b];
-  const int c = eact ? g.ub_edges[256L * b + t] : 1;
-  const int pl = eact ? g.ub_eloc[256L * b + t] : 0;
+  const bool eact = t < B.ne[b];
+  const int c = eact ? B.edges[256L * b + t] : 1;
+  const int pl = eact ? B.eloc[256L * b + t] : 0;
   const int sa = pl & 255, sb = (pl >> 8) & 255;
   const int ia = IEC_(c, 1), ib = IEC_(c, 2), j1 = IENODE_(c, 1), j2 = IENODE_(c, 2);
   const FT xija = (FT)(g.xijc[c] - g.xc[ia]), yija = (FT)(g.yijc[c] - g.yc[ia]);
@@ -71,13 +74,12 @@ __global__ __launch_bounds__(256) void k_uv_blk(DevGrid g, IntArgs a) {
   const double dtj1 = a.dt[j1], dtj2 = a.dt[j2];
   const FT isbc_tmp = (FT)g.isbc[c];
   // 3. element part: component cy (0: x, 1: y) of own element i
-  const int eb = g.ub_eb;
-  const bool cy = t >= eb;
-  const int tl = cy ? t - eb : t;
-  const int i = b * eb + tl + 1;
-  const bool oact = t < 2 * eb && i <= g.N;
-  const int io = oact ? i : 1;
-  const int ep3 = g.ub_epos[io];
+  const bool cy = t >= 128;
+  const int tl = cy ? t - 128 : t;
+  const bool oact = tl < B.nown[b];
+  const int i = oact ? B.elems[256L * b + tl] : 1;   // the block's own elements are its first records
+  const int io = i;
+  const int ep3 = B.epos[io];
   int slot[3];
   bool flip[3];       // the element is IB of the edge
   double tp[3], elij[3], dlp[3];
