@@ -55,7 +55,7 @@ typedef struct fv_params {
   int32_t salinity_active;     /* SALINITY_ACTIVE                         */
   int32_t wind_on;             /* WIND_ON, WIND_TYPE='speed'              */
   int32_t river_ts_specified;  /* RIVER_TS_SETTING == 'specified'         */
-  int32_t type_tsobc;          /* OBC_TS_TYPE 1..3                        */
+  int32_t type_tsobc;          /* OBC_TS_TYPE 1..4                        */
   int32_t obc_on;              /* OBC_ON                                  */
   int32_t obc_elevation_on;    /* OBC_ELEVATION_FORCING_ON (spectral)     */
   int64_t iint0;               /* IINT before the first step              */

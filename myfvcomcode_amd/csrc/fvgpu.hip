@@ -916,6 +916,7 @@ int upload_grid(fvgpu_handle* h) {
   g.cvhp1 = std::pow(15.8 * 100.0, 0.6666667);
   g.isplit = p.isplit; g.iramp = p.iramp; g.noflux_bot = p.noflux_bot;
   g.river_ts_specified = p.river_ts_specified; g.type_tsobc = p.type_tsobc; g.obc_on = p.obc_on;
+  if (p.type_tsobc < 1 || p.type_tsobc > 4) return fail("TYPE_TSOBC must be 1..4");
   g.obc_elev = p.obc_elevation_on; g.wind_on = p.wind_on;
 #define UP(field, n) if (dupload(h, &g.field, md->field, (n))) return 1
   UP(vx, m1); UP(vy, m1); UP(h, m1); UP(art1, m1); UP(art2, m1); UP(grav_n, m1); UP(nn_hvc, m1); UP(phpn, m1);
@@ -941,6 +942,9 @@ int upload_grid(fvgpu_handle* h) {
   const size_t ob = md->iobcn, nt = md->ntide, nr = md->numqbc;
   for (size_t k = 0; k < ob; ++k)
     if (md->type_obc[k] < 1 || md->type_obc[k] > 10) return fail("TYPE_OBC must be 1..10");
+  if (p.type_tsobc == 4)
+    for (size_t k = 0; k < ob; ++k)
+      if (md->next_obc2[k] <= 0) return fail("TYPE_TSOBC 4 needs NEXT_OBC2 for every OBC node");
   g.obc_radiate = 0;
   g.obc_ore = 0;
   for (size_t k = 0; k < ob; ++k) {
@@ -1059,7 +1063,7 @@ int upload_state(fvgpu_handle* h) {
   }
   RW(elij2, nullptr, (size_t)g.NE + 1);
   if (dalloc(h, &a.vhpair, mk)) return 1;
-  if (dalloc(h, &a.obcsum, 3 * (size_t)g.IOBCN)) return 1;
+  if (dalloc(h, &a.obcsum, 5 * (size_t)g.IOBCN)) return 1;
   RW(adx2d, nullptr, n1); RW(ady2d, nullptr, n1); RW(drx2d, nullptr, n1); RW(dry2d, nullptr, n1);
   RW(cbc, nullptr, n1); RW(wubot, nullptr, n1); RW(wvbot, nullptr, n1); RW(tps, nullptr, n1);
   RW(uard, nullptr, n1); RW(vard, nullptr, n1); RW(fluxobn, nullptr, n1);
@@ -1095,7 +1099,7 @@ int upload_state(fvgpu_handle* h) {
   }
   if (dalloc(h, &b.pfpxb, m1) || dalloc(h, &b.pfpyb, m1) || dalloc(h, &b.ah_bottom, m1)) return 1;
   if (dalloc(h, &b.xflux_obc, (size_t)g.IOBCN * g.KB)) return 1;
-  if (dalloc(h, &b.obcsum, 3 * (size_t)g.IOBCN)) return 1;
+  if (dalloc(h, &b.obcsum, 5 * (size_t)g.IOBCN)) return 1;
   return 0;
 }
 

@@ -31,7 +31,7 @@ struct IntArgs {
   double* elij2;         // per-edge mean elevation (2-D)
   void* rec;             // node-layer records of the field being advected, CvRecT<FT>
   double2* vhpair;       // node-layer pair scratch: VDIF_Q's elimination coefficients
-  double* obcsum;        // BCOND_TS column sums, 3 per OBC node
+  double* obcsum;        // BCOND_TS column sums, 5 per OBC node
   void* cvflux;          // per sub-edge, per layer face flux (NCV+1) x KB FT
   // 2-D
   double *adx2d, *ady2d, *drx2d, *dry2d, *cbc, *wubot, *wvbot, *tps, *uard, *vard, *fluxobn;

@@ -94,9 +94,21 @@ __global__ void k_bcond_ts_sums(DevGrid g, IntArgs a, int ncon, double* sums) {
       xflux2d = xflux2d + xf[q];
     }
   }
-  sums[3 * t] = t2d;
-  sums[3 * t + 1] = t2d_next;
-  sums[3 * t + 2] = xflux2d;
+  sums[5 * t] = t2d;
+  sums[5 * t + 1] = t2d_next;
+  sums[5 * t + 2] = xflux2d;
+  if (g.type_tsobc == 4) {
+    // the Orlanski form's column sums of F1 at NEXT_OBC and NEXT_OBC2 (its
+    // F1M1 is F1 here: see k_bcond_ts_layers)
+    const int j2 = g.next_obc2[t];
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = 1; k <= g.KBM1; ++k) {
+      s1 = s1 + f1[NI(g, j1, k)] * DZ(j1, k);
+      s2 = s2 + f1[NI(g, j2, k)] * DZ(j2, k);
+    }
+    sums[5 * t + 3] = s1;
+    sums[5 * t + 4] = s2;
+  }
 }
 
 // Part 2, one thread per OBC node and layer: the radiated value (TYPE_TSOBC
@@ -115,7 +127,7 @@ __global__ void k_bcond_ts_layers(DevGrid g, IntArgs a, int ncon, const double* 
     ff[NI(g, j, k)] = f1[NI(g, j, k)];
     return;
   }
-  const double t2d = sums[3 * t], t2d_next = sums[3 * t + 1], xflux2d = sums[3 * t + 2];
+  const double t2d = sums[5 * t], t2d_next = sums[5 * t + 1], xflux2d = sums[5 * t + 2];
   const double tmp = xflux2d + t2d * a.uard_obcn[t];
   const double t2d_obc = (t2d * a.dt[j] - tmp * g.dti / g.art1[j]) / a.d[j];
   const double cc = sqrt(g.grav_n[j] * g.h[j]) * g.dti / g.dltn_obc[t], cp = cc + 1.0;
@@ -123,7 +135,26 @@ __global__ void k_bcond_ts_layers(DevGrid g, IntArgs a, int ncon, const double* 
   double ttmp;
   if (g.type_tsobc == 1) ttmp = ff[NI(g, j1, k)] - t2d_next;
   else if (g.type_tsobc == 2) ttmp = (cc * pert_next + pert) / cp;
-  else ttmp = (cc * pert_next + pert * (1.0 - g.dti / 10800.0)) / cp;
+  else if (g.type_tsobc == 3) ttmp = (cc * pert_next + pert * (1.0 - g.dti / 10800.0)) / cp;
+  else {
+    // Orlanski (mod_obcs.F:1258-1285, 1336-1363).  Its F1M1/F1M2 are F1
+    // one and two external RK stages back (ASSIGN_ELM1_TO_ELM2, called at
+    // every stage, mod_obcs.F:190-196); F1 does not change during the
+    // external steps, which run before BCOND_TS, so both equal F1 here, and
+    // their column sums are the same sums of F1 (T2D at the OBC node,
+    // T2D_NEXT1 at NEXT_OBC) formed in the same order.
+    const int j2 = g.next_obc2[t];
+    const double t2d_next1 = sums[5 * t + 3], tm12d_next2 = sums[5 * t + 4];
+    const double tm12d_next1 = t2d, tm22d_next1 = t2d_next1;
+    const double f1j1 = f1[NI(g, j1, k)], m2j1 = f1j1, m1j2 = f1[NI(g, j2, k)], m1j = f1[NI(g, j, k)];
+    const double cl = ((m2j1 - tm22d_next1) - (f1j1 - t2d_next1)) /
+                      ((f1j1 - t2d_next1) + (m2j1 - tm22d_next1) - 2.0 * (m1j2 - tm12d_next2));
+    double mu;
+    if (cl >= 1.0) mu = 1.0;
+    else if (cl > 0.0 && cl < 1.0) mu = cl;
+    else mu = 0.0;
+    ttmp = ((m1j - tm12d_next1) * (1.0 - mu) + 2.0 * mu * (f1j1 - t2d_next1)) / (1.0 + mu);
+  }
   double x = t2d_obc + ttmp;
   const long ko = (long)(k - 1) * g.m1;
   const int o = g.nbsn_off[j], nt = g.ntsn[j];

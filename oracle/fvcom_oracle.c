@@ -51,6 +51,7 @@ typedef struct orc {
   double *u, *v, *w, *ww, *uf, *vf, *advx, *advy, *drhox, *drhoy, *rho, *rmean, *t, *s, *km1, *viscofm;
   double *t1, *s1, *tf1, *sf1, *rho1, *rmean1, *tmean1, *smean1, *q2, *q2l, *q2f, *q2lf;
   double *l, *km, *kh, *kq, *wts, *viscofh;
+  double *t1m1, *t1m2, *s1m1, *s1m2;   /* T1/S1 one and two RK stages back (TYPE_TSOBC 4) */
   /* scratch */
   double *sc_m1[8];
   double *sc_mk[4];
@@ -134,6 +135,7 @@ orc* orc_create(const fv_model* g) {
   ANK(rho); ANK(rmean); ANK(t); ANK(s); ANK(km1); ANK(viscofm);
   AMK(t1); AMK(s1); AMK(tf1); AMK(sf1); AMK(rho1); AMK(rmean1); AMK(tmean1); AMK(smean1);
   AMK(q2); AMK(q2l); AMK(q2f); AMK(q2lf); AMK(l); AMK(km); AMK(kh); AMK(kq); AMK(wts); AMK(viscofh);
+  AMK(t1m1); AMK(t1m2); AMK(s1m1); AMK(s1m2);
   for (int i = 0; i < 8; ++i) o->sc_m1[i] = zal(o->m1 > o->n1 ? o->m1 : o->n1);
   for (int i = 0; i < 4; ++i) o->sc_mk[i] = zal(mk);
   for (int i = 0; i < 4; ++i) o->sc_nk[i] = zal(nk);
@@ -163,7 +165,7 @@ void orc_destroy(orc* o) {
                   &o->ww, &o->uf, &o->vf, &o->advx, &o->advy, &o->drhox, &o->drhoy, &o->rho, &o->rmean,
                   &o->t, &o->s, &o->km1, &o->viscofm, &o->t1, &o->s1, &o->tf1, &o->sf1, &o->rho1,
                   &o->rmean1, &o->tmean1, &o->smean1, &o->q2, &o->q2l, &o->q2f, &o->q2lf, &o->l, &o->km,
-                  &o->kh, &o->kq, &o->wts, &o->viscofh};
+                  &o->kh, &o->kq, &o->wts, &o->viscofh, &o->t1m1, &o->t1m2, &o->s1m1, &o->s1m2};
   for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); ++i) free(*f[i]);
   for (int i = 0; i < 8; ++i) free(o->sc_m1[i]);
   for (int i = 0; i < 4; ++i) { free(o->sc_mk[i]); free(o->sc_nk[i]); }
@@ -195,6 +197,7 @@ static double* orc_lookup(orc* o, const char* name, size_t* len) {
       {"smean1", o->smean1, mk}, {"q2", o->q2, mk}, {"q2l", o->q2l, mk}, {"q2f", o->q2f, mk},
       {"q2lf", o->q2lf, mk}, {"l", o->l, mk}, {"km", o->km, mk}, {"kh", o->kh, mk},
       {"kq", o->kq, mk}, {"wts", o->wts, mk}, {"viscofh", o->viscofh, mk},
+      {"t1m1", o->t1m1, mk}, {"t1m2", o->t1m2, mk}, {"s1m1", o->s1m1, mk}, {"s1m2", o->s1m2, mk},
       {"ramp", &o->ramp, 1},
       {"uard_obcn", o->uard_obcn, (size_t)o->IOBCN},
       {"xflux_obc", o->xflux_obc, (size_t)o->IOBCN * o->KB},
@@ -527,6 +530,11 @@ void orc_external_step(orc* o) {
     if (o->has_ore) {   /* ASSIGN_ELM1_TO_ELM2 (external_step.F:333, mod_obcs.F:187-188) */
       memcpy(o->elm2, o->elm1, sizeof(double) * M1);
       memcpy(o->elm1, o->el, sizeof(double) * M1);
+    }
+    if (o->p.type_tsobc == 4) {   /* its T/S part (mod_obcs.F:190-196) */
+      const size_t mk = M1 * o->KB;
+      memcpy(o->t1m2, o->t1m1, sizeof(double) * mk); memcpy(o->t1m1, o->t1, sizeof(double) * mk);
+      memcpy(o->s1m2, o->s1m1, sizeof(double) * mk); memcpy(o->s1m1, o->s1, sizeof(double) * mk);
     }
     memcpy(o->el, o->elf, sizeof(double) * M1);
     memcpy(o->el1, o->elf1, sizeof(double) * N1);
@@ -1431,11 +1439,36 @@ static void bcond_ts(orc* o, int ncon) {
       double tmp = xflux2d + t2d * o->uard_obcn[i];
       double t2d_obc = (t2d * o->dt[j] - tmp * o->dti / G->art1[j]) / o->d[j];
       double cc = sqrt(G->grav_n[j] * G->h[j]) * o->dti / G->dltn_obc[i], cp = cc + 1.0;
-      for (int k = 1; k <= KBM1; ++k) {
-        double pert_next = ND(ff, j1, k) - t2d_next, pert = ND(f1, j, k) - t2d;
-        if (o->p.type_tsobc == 1) ttmp[k] = ND(ff, j1, k) - t2d_next;
-        else if (o->p.type_tsobc == 2) ttmp[k] = (cc * pert_next + pert) / cp;
-        else ttmp[k] = (cc * pert_next + pert * (1.0 - o->dti / 10800.0)) / cp;
+      if (o->p.type_tsobc == 4) {
+        /* Orlanski radiation of the perturbation, BCOND_T/S_PERTURBATION
+         * case 4 (mod_obcs.F:1258-1285, 1336-1363) */
+        const double* m1a = ncon == 1 ? o->t1m1 : o->s1m1;
+        const double* m2a = ncon == 1 ? o->t1m2 : o->s1m2;
+        int j2 = G->next_obc2[i];
+        double t2d_next1 = 0.0, tm12d_next2 = 0.0, tm12d_next1 = 0.0, tm22d_next1 = 0.0;
+        for (int k = 1; k <= KBM1; ++k) {
+          t2d_next1 = t2d_next1 + ND(f1, j1, k) * ND(DZ, j1, k);
+          tm12d_next2 = tm12d_next2 + ND(m1a, j2, k) * ND(DZ, j2, k);
+          tm12d_next1 = tm12d_next1 + ND(m1a, j, k) * ND(DZ, j, k);
+          tm22d_next1 = tm22d_next1 + ND(m2a, j1, k) * ND(DZ, j1, k);
+        }
+        for (int k = 1; k <= KBM1; ++k) {
+          double cl = ((ND(m2a, j1, k) - tm22d_next1) - (ND(f1, j1, k) - t2d_next1)) /
+                      ((ND(f1, j1, k) - t2d_next1) + (ND(m2a, j1, k) - tm22d_next1) -
+                       2.0 * (ND(m1a, j2, k) - tm12d_next2));
+          double mu;
+          if (cl >= 1.0) mu = 1.0;
+          else if (cl > 0.0 && cl < 1.0) mu = cl;
+          else mu = 0.0;
+          ttmp[k] = ((ND(m1a, j, k) - tm12d_next1) * (1.0 - mu) + 2.0 * mu * (ND(f1, j1, k) - t2d_next1)) / (1.0 + mu);
+        }
+      } else {
+        for (int k = 1; k <= KBM1; ++k) {
+          double pert_next = ND(ff, j1, k) - t2d_next, pert = ND(f1, j, k) - t2d;
+          if (o->p.type_tsobc == 1) ttmp[k] = ND(ff, j1, k) - t2d_next;
+          else if (o->p.type_tsobc == 2) ttmp[k] = (cc * pert_next + pert) / cp;
+          else ttmp[k] = (cc * pert_next + pert * (1.0 - o->dti / 10800.0)) / cp;
+        }
       }
       for (int k = 1; k <= KBM1; ++k) ND(ff, j, k) = t2d_obc + ttmp[k];
       for (int k = 1; k <= KBM1; ++k) {

@@ -90,6 +90,26 @@ def test_open_boundary_types_parity(built, graph):
     assert not bad, bad
 
 
+def test_tsobc_orlanski_parity(built):
+    """TYPE_TSOBC 4 (Orlanski radiation of the T/S perturbation) beside the
+    ten elevation types: the device takes its T1M1/T1M2 as T1 (their value
+    by BCOND_TS), the oracle keeps and shifts the arrays at every stage."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    m = F.Model.build(0, p_type_tsobc=4)
+    m.set_obc_type([(k % 10) + 1 for k in range(m.hdr.iobcn)])
+    dev, orc = F.DeviceModel(m), Oracle(m)
+    n = 10
+    print("case: 81x21 estuary channel, TYPE_TSOBC 4, TYPE_OBC 1..10, %d steps" % n, flush=True)
+    dev.run(n, graph=True)
+    orc.run(n)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    for k, v in sorted(rep.items(), key=lambda x: -x[1])[:8]:
+        print("  %-8s %.3e" % (k, v))
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_jittered_wind_case_parity(built, graph):
     """Config-1 generator at 101x51 nodes, KB=11: jitter, seamounts, wind,

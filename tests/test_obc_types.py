@@ -181,3 +181,56 @@ def test_orlanski_value_is_a_convex_combination(built, ty):
         assert np.all(el[obc] >= lo - tol) and np.all(el[obc] <= hi + tol)
         moved += int(np.any(el[obc] != 0.0))
     assert moved > 10                      # the boundary does move as the bump arrives
+
+
+def test_tsobc_orlanski_carries_the_perturbation(built):
+    """TYPE_TSOBC 4 (Orlanski, mod_obcs.F:1258-1285): T1M1/T1M2 are T1 one
+    and two external RK stages back, set at every stage while T1 stays
+    fixed, so by BCOND_TS both equal T1; the phase speed's numerator is then
+    zero, MU = 0, and an outflow node keeps its own perturbation about the
+    2-D upstream update of its column mean (inflow nodes keep T1)."""
+    m = F.Model.build(0, nx=21, ny=7, kb=8, river=0, p_type_tsobc=4)
+    kb, m1 = m.kb, m.m + 1
+    o = Oracle(m)
+    o.run(3)
+    t1_before = o.get("t1").copy()
+    s1_before = o.get("s1").copy()
+    o.internal_step()
+    for a, b in (("t1m1", t1_before), ("t1m2", t1_before), ("s1m1", s1_before), ("s1m2", s1_before)):
+        assert np.array_equal(o.get(a), b), a
+    # one BCOND_TS call on a state with distinct perturbations at the OBC nodes
+    obc = m.field("i_obc_n")
+    DZ = m.field("dz").reshape(kb, m1)
+    art1 = m.field("art1")
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    rng = np.random.default_rng(11)
+    t1 = o.get("t1").reshape(kb, m1).copy()
+    for j in obc:
+        t1[:-1, j] = t1[:-1, j] + 0.05 * rng.standard_normal(kb - 1)
+    nob = m.hdr.iobcn
+    uo = np.abs(rng.standard_normal(nob)) * 50.0
+    uo[0] = -1.0
+    xf = np.zeros((kb, nob))
+    xf[:-1] = rng.standard_normal((kb - 1, nob)) * 3.0
+    for nm, a in (("t1", t1.ravel()), ("t1m1", t1.ravel()), ("t1m2", t1.ravel()), ("uard_obcn", uo),
+                  ("xflux_obc", xf.ravel())):
+        o.put(nm, a)
+    d, dt = o.get("d"), o.get("dt")
+    o.routine("bcond_ts", 1)
+    out = o.get("tf1").reshape(kb, m1)
+    nbsn, off, ntsn = m.field("nbsn"), m.field("nbsn_off"), m.field("ntsn")
+    checked = 0
+    for q, j in enumerate(obc):
+        if uo[q] <= 0:
+            assert np.array_equal(out[:-1, j], t1[:-1, j])
+            continue
+        t2d = np.sum(t1[:-1, j] * DZ[:-1, j])
+        mean = (t2d * dt[j] - (np.sum(xf[:-1, q]) + t2d * uo[q]) * dti / art1[j]) / d[j]
+        exp = mean + (t1[:-1, j] - t2d)
+        # the clip to the ring and vertical neighbours' range (fct-like) may act
+        ring = nbsn[off[j]:off[j] + ntsn[j]]
+        lo, hi = t1[:-1, ring].min(axis=1), t1[:-1, ring].max(axis=1)
+        inside = (exp > lo) & (exp < hi)
+        assert np.max(np.abs(out[:-1, j][inside] - exp[inside]), initial=0.0) < 1e-12
+        checked += int(inside.sum())
+    assert checked > 0
