@@ -93,7 +93,9 @@ struct fvgpu_handle {
   hipStream_t stream2 = nullptr, cur = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int concurrent = 3;     // bit 0: 3-D chain pairs, bit 1: the 2-D stage pair on stream2
-  int two_phase_cv = 0;   // CV records then face fluxes in two launches (1) or one fused launch (0)
+  // CV records then face fluxes in two launches (1) or one fused launch
+  // (0), for ADV_T/ADV_S and for ADV_Q
+  int two_phase_ts = 0, two_phase_q = 0;
   // wind speeds at the start (wu0, wv0) and end (wu1, wv1) of the coming
   // internal step; once they differ, every external step forms its own
   // stress (ws2x, ws2y), otherwise it uses the end stress g.wsx/g.wsy
@@ -333,7 +335,7 @@ void enqueue_ts_pass(fvgpu_handle* h, IntArgs& a, int which, int fct) {
   double* ff = which == 1 ? a.tf1 : a.sf1;
   const double* fm = which == 1 ? a.tmean1 : a.smean1;
   const double* fdis = which == 1 ? g.tdis : g.sdis;
-  if (h->two_phase_cv) {
+  if (h->two_phase_ts) {
 #define CVREC_TS_CALL(RM) TLAUNCH("cvrec_ts", (k_cvrec<0, RM, FT>), g.M + 1, g, a, f1, fm, 0.0, 1, g.KBM1)
     FV_RMAX_DISPATCH(g.ring_max, CVREC_TS_CALL);
     TLAUNCH("cvflux_ts", k_cvflux_ts<FT>, g.N + 1, g, a);
@@ -364,7 +366,7 @@ template <typename FT>
 void enqueue_adv_q(fvgpu_handle* h, IntArgs& a, double* q, double* qf, int fct) {
   const DevGrid& g = h->g;
   const double qmean1 = (double)1.E-8f;
-  if (h->two_phase_cv) {
+  if (h->two_phase_q) {
 #define CVREC_Q_CALL(RM) TLAUNCH("cvrec_q", (k_cvrec<1, RM, FT>), g.M + 1, g, a, q, nullptr, qmean1, 2, g.KBM1)
     FV_RMAX_DISPATCH(g.ring_max, CVREC_Q_CALL);
     TLAUNCH("cvflux_q", k_cvflux_q<FT>, g.N + 1, g, a);
@@ -1157,7 +1159,11 @@ int fvgpu_create(const fv_model* md, int32_t precision, fvgpu_handle** out) {
   // FP64: CV records + face fluxes in one launch; the FP32 flux form keeps
   // the two launches (measured faster for it: its records are half the
   // size, and the two kernels' higher occupancy outweighs their traffic)
-  h->two_phase_cv = precision == FV_MIXED ? 1 : 0;
+  // the faster form per precision: in FP32 the fused tracer pass (980
+  // against 505 + 620 us) but the two-phase Q2 pass (386 + 491 against
+  // 1013 us), in FP64 both fused
+  h->two_phase_ts = 0;
+  h->two_phase_q = precision == FV_MIXED ? 1 : 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail("stream"); }
   h->cur = h->stream;
   if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
