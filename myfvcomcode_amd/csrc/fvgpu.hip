@@ -420,12 +420,10 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
            h->cl_yp);
     LAUNCH("zero", k_zero_int, 1, 64, h->cl_nfall, 1);
     TLAUNCH("closure_apply", k_closure_apply, g.M, g, a, h->cl_yp, a.gx, h->cl_flag, h->cl_nfall);
-    TLAUNCH("copy", k_copy, mk, a.q2f, a.q2, mk);
-    TLAUNCH("copy", k_copy, mk, a.q2lf, a.q2l, mk);
+    TLAUNCH("fallback_q2f", k_fallback_q2f, g.M + 1, g, a, (const int*)h->cl_flag);
 #define VDIF_QM_CALL(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 10, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.vhpair, h->cl_flag)
 #define VDIF_QM_CALL16(KM) TLAUNCH("vdif_q", (k_vdif_q_r<KM, 16, FV_VDIFQ_VHREG>), g.M + 1, g, a, a.vhpair, h->cl_flag)
-    // (vdif_q also stores Q2 = Q2F, Q2L = Q2LF for its columns; the others
-    // hold Q2F = Q2 from the copies above)
+    // (vdif_q also stores Q2 = Q2F, Q2L = Q2LF for its columns)
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_QM_CALL16);
   } else if (h->md->p.vertical_closure) {

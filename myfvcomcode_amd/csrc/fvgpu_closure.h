@@ -470,6 +470,19 @@ __global__ void k_closure_apply(DevGrid g, IntArgs a, const float* Yp, const dou
   }
 }
 
+// The MY2.5 input of the rejected columns: Q2F = Q2, Q2LF = Q2L (no
+// horizontal advection of Q2/Q2L with the learned closure), one thread per
+// node column; the accepted columns never read Q2F/Q2LF in the step.
+__global__ void k_fallback_q2f(DevGrid g, IntArgs a, const int* flag) {
+  const int i = (int)xcd_thread();
+  if (i > g.M || !flag[i]) return;
+  for (int k = 1; k <= g.KB; ++k) {
+    const long q = NI(g, i, k);
+    a.q2f[q] = a.q2[q];
+    a.q2lf[q] = a.q2l[q];
+  }
+}
+
 // ------------------------------------------------------------ ensemble update
 // A <- A*X5, the ensemble update of the EnKF analysis (`multa`, the last
 // step of ANALYSIS, mod_enkfassim.F:2746-3010), over the member slices of
