@@ -118,3 +118,63 @@ def test_mixed_union_member_shape(built):
         err = float(np.sqrt(np.mean((a - b) ** 2)))
         print("  mixed vs fp64 %-3s rms %.3e" % (name, err), flush=True)
         assert err <= bound, (name, err)
+
+
+def test_union_carries_obc_types_and_edge_rivers(built):
+    """Per-instance boundary set-ups survive the union: TYPE_OBC of every
+    kind, NEXT_OBC2 (node numbers, offset), edge rivers (cell and node
+    numbers, offset) beside a node river; each instance of the union still
+    evolves exactly as it does alone (oracle)."""
+    from oracle import Oracle
+    from test_river_edge import _west_cell
+    a = F.Model.build(0, nx=21, ny=7, kb=6)
+    a.set_obc_type([(k % 10) + 1 for k in range(a.hdr.iobcn)])
+    b = F.Model.build(0, nx=17, ny=9, kb=6, river=0)
+    b.set_river_edge(_west_cell(b), 30.0, 12.0, 1.0)
+    b.set_obc_type([5 + (k % 2) for k in range(b.hdr.iobcn)])
+    for p in (a, b):
+        p.cold_start()
+    ps = [a, b]
+    u, offs = F.Model.union(ps)
+    assert list(u.field("type_obc")) == list(a.field("type_obc")) + list(b.field("type_obc"))
+    om_b = offs[1][0]
+    nob_a = a.hdr.iobcn
+    assert np.array_equal(u.field("next_obc2")[nob_a:], b.field("next_obc2") + om_b)
+    nq = u.hdr.numqbc
+    assert nq == 2
+    assert int(u.field("icellq")[1]) == int(b.field("icellq")[0]) + offs[1][1]
+    nic = u.field("n_icellq").reshape(2, nq)
+    assert list(nic[:, 1]) == [int(x) + om_b for x in b.field("n_icellq")]
+    ou = Oracle(u)
+    ou.run(3)
+    for p, off in zip(ps, offs):
+        op = Oracle(p)
+        op.run(3)
+        for name in FIELDS:
+            x = instance_slice(ou.get(name), u, off, u.kb)
+            y = own(op.get(name), p, u.kb)
+            assert np.isfinite(y).all(), name
+            assert np.array_equal(x, y), (name, off)
+
+
+@pytest.mark.gpu
+def test_union_with_boundary_kinds_gpu_parity(built):
+    """The same union on the HIP path against the oracle, bit for bit."""
+    from oracle import Oracle
+    from helpers import ELEM2, ELEM3, NODE2, NODE3, compare
+    from test_river_edge import _west_cell
+    a = F.Model.build(0, nx=21, ny=7, kb=6)
+    a.set_obc_type([(k % 10) + 1 for k in range(a.hdr.iobcn)])
+    b = F.Model.build(0, nx=17, ny=9, kb=6, river=0)
+    b.set_river_edge(_west_cell(b), 30.0, 12.0, 1.0)
+    b.set_obc_type([5 + (k % 2) for k in range(b.hdr.iobcn)])
+    for p in (a, b):
+        p.cold_start()
+    u, offs = F.Model.union([a, b])
+    print("case: union of two channels with OBC types 1..10 / 5-6 and an edge river, 4 steps (graph)", flush=True)
+    dev, orc = F.DeviceModel(u), Oracle(u)
+    dev.run(4, graph=True)
+    orc.run(4)
+    rep = {}
+    worst, bad = compare(dev, orc, u, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    assert not bad, bad
