@@ -240,8 +240,8 @@ int  fv_model_from_mesh(int32_t m, int32_t n, const double* x, const double* y,
                         const fv_params* p, fv_model** out);
 void fv_model_free(fv_model* mdl);
 int64_t fv_extent_len(const fv_model* mdl, int32_t extent);
-/* Reverse Cuthill-McKee numbering of a mesh (north-star locality
- * requirement; the reference keeps the grid file's numbering): nodes by
+/* Reverse Cuthill-McKee numbering of a mesh (neighbours at neighbouring
+ * addresses for the gathers; the reference keeps the grid file's numbering): nodes by
  * RCM of the node graph from a pseudo-peripheral node per component,
  * elements by their lowest/middle/highest new node index.  node_new[j-1]
  * and elem_new[e-1] receive the 1-based new index of old node j and old

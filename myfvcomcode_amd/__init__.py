@@ -72,7 +72,7 @@ def grid_lib():
     if _grid is None:
         path = lib_path("libfvgrid.so")
         if not os.path.exists(path):
-            raise RuntimeError("libfvgrid.so is not built; run __graft_entry__.build()")
+            raise RuntimeError("libfvgrid.so is not built; run myfvcomcode_amd.build.build_all()")
         L = C.CDLL(path)
         L.fv_last_error.restype = C.c_char_p
         L.fv_case_defaults.argtypes = [C.c_int32, C.POINTER(FvCase)]
@@ -106,7 +106,7 @@ def gpu_lib():
     if _gpu is None:
         path = lib_path("libfvgpu.so")
         if not os.path.exists(path):
-            raise RuntimeError("libfvgpu.so (HIP) is not built; run __graft_entry__.build()")
+            raise RuntimeError("libfvgpu.so (HIP) is not built; run myfvcomcode_amd.build.build_all()")
         L = C.CDLL(path)
         vp = C.c_void_p
         L.fvgpu_last_error.restype = C.c_char_p

@@ -5,9 +5,10 @@
  * FVCOM_source/internal_step.F:29-1607, and EXTERNAL_STEP,
  * FVCOM_source/external_step.F:30-443) in the reference's own loop and
  * scatter order, for the build variant GCN + LIMITED_NO + DOUBLE_PRECISION,
- * Cartesian, no wet/dry, no heating, spectral tide, node rivers.
+ * Cartesian, no wet/dry, no heating; open-boundary types 1-10 with the
+ * spectral tide, TYPE_TSOBC 1-4, node and edge rivers.
  *
- * It is the checker of the HIP path and the CPU baseline of the benchmark.
+ * The HIP path is tested against it, and it is the benchmark's CPU baseline.
  * Only tests/, the smoke check and the benchmark's CPU leg may load it; the
  * product library never links or calls it.
  *
@@ -16,10 +17,12 @@
  * GRID_METRICS output); state arrays are copied into the oracle.
  *
  * Pinning (what checks each routine beyond restatement) is listed in
- * tests/test_oracle.py (whole steps: budgets, seiche, tide, densities) and
- * tests/test_oracle_pins.py (single routines through orc_routine against
- * closed forms and published results); routines without such a check are
- * marked "parity unpinned" in their comment.
+ * tests/test_oracle.py (whole steps: budgets, seiche, tide, densities),
+ * tests/test_oracle_pins.py and tests/test_oracle_pins_limiters.py (single
+ * routines through orc_routine against closed forms and published
+ * results), tests/test_obc_types.py (open-boundary types) and
+ * tests/test_river_edge.py (edge rivers); routines without such a check
+ * are marked "parity unpinned" in their comment.
  */
 #include "../include/fvgpu.h"
 #include "../include/fv_detmath.h"
