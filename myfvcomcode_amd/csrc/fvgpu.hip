@@ -1770,8 +1770,9 @@ int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_
   HIPCHK(hipMemcpyAsync(arena + off, h->clk, sizeof(DevClock), hipMemcpyDeviceToDevice, h->stream));
   const bool prof = h->profiling;
   const std::string only = h->profile_only;
-  const int conc = h->concurrent;
-  h->concurrent = 0;   // each family timed alone
+  const int conc = h->concurrent, every = h->profile_every;
+  h->concurrent = 0;   // each family timed alone, at every launch
+  h->profile_every = 1;
   std::string rep;
   int rc = 0;
   for (const char* f : fams) {
@@ -1803,6 +1804,7 @@ int fvgpu_autotune(fvgpu_handle* h, int32_t steps, char* report, int64_t report_
   h->profiling = prof;
   h->profile_only = only;
   h->concurrent = conc;
+  h->profile_every = every;
   h->stats.clear();
   off = 0;
   for (auto& b : h->rw) {
