@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sim-days/wall-hour @1M elem x 40 layers; elem-layer-steps/s vs plain FP64"
 PEAK_HBM = 8.0e12
+VARIANTS = ("fp64_graph", "mixed", "mixed_learned_closure")
 PROFILE_EVERY = 4   # the timed region's events time one in this many dominant-family launches
 
 
@@ -48,6 +49,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip timing the mixed-precision and learned-closure forms")
+    ap.add_argument("--variant", choices=VARIANTS,
+                    help="only time this form (graph replays) and print its line; for profiling one form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a plain device copy of 2^25 doubles (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -73,6 +76,12 @@ def main():
     dti = model.hdr.p.dte * model.hdr.p.isplit
     print("bench: built %d nodes / %d elements x %d layers in %.1f s" % (model.m, n, kbm1, time.time() - t0),
           file=sys.stderr, flush=True)
+    if args.variant:
+        v = time_variants(F, model, args.steps, args.warmup, None, dti, n * kbm1, only=args.variant)
+        print(json.dumps({"variant": args.variant, "steps": args.steps, "warmup": args.warmup,
+                          "workload": "jittered mesh %dx%d nodes, %d elements x %d layers" % (args.nx, args.ny, n, kbm1),
+                          **v[args.variant]}), flush=True)
+        return
     dev = F.DeviceModel(model)
     dev.run(args.warmup, graph=False)
     dev.synchronize()
@@ -201,14 +210,16 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def time_variants(F, model, steps, warmup, fp64_step_s, dti, units):
+def time_variants(F, model, steps, warmup, fp64_step_s, dti, units, only=None):
     """FP64 as one hipGraph replay per step, the mixed-precision form, and
     mixed precision with the learned closure (graph replays, chains side by
-    side), on the same workload and step count."""
+    side), on the same workload and step count (``only``: that form alone)."""
     from myfvcomcode_amd.closure import load_params
     out = {}
     for name, prec, closure in (("fp64_graph", "fp64", False), ("mixed", "mixed", False),
                                 ("mixed_learned_closure", "mixed", True)):
+        if only is not None and name != only:
+            continue
         dev = F.DeviceModel(model, precision=prec)
         graph = True
         dev.set_concurrency(3)
@@ -222,8 +233,9 @@ def time_variants(F, model, steps, warmup, fp64_step_s, dti, units):
         w = (time.perf_counter() - t0) / steps
         if dev.diagnostics().nonfinite:
             raise RuntimeError("bench: the %s form became non-finite" % name)
-        out[name] = {"ms_per_step": 1000.0 * w, "value": units / w * dti / (24.0 * units),
-                     "speedup_vs_fp64": fp64_step_s / w}
+        out[name] = {"ms_per_step": 1000.0 * w, "value": units / w * dti / (24.0 * units)}
+        if fp64_step_s is not None:
+            out[name]["speedup_vs_fp64"] = fp64_step_s / w
         if closure:
             out[name]["fallback_columns_last_step"] = dev.closure_fallbacks()[0]
         print("bench: %s %.2f ms/step" % (name, 1000.0 * w), file=sys.stderr, flush=True)
