@@ -21,8 +21,16 @@
  * tests/test_oracle_pins.py and tests/test_oracle_pins_limiters.py (single
  * routines through orc_routine against closed forms and published
  * results), tests/test_obc_types.py (open-boundary types) and
- * tests/test_river_edge.py (edge rivers); routines without such a check
- * are marked "parity unpinned" in their comment.
+ * tests/test_river_edge.py (edge rivers).  Every routine has such a check;
+ * these parts of routines have none ("parity unpinned": restatement only,
+ * an error there would be shared by the HIP path):
+ *   - which bound BCOND_TS's clip lands on (the vertical interpolants
+ *     inside the range);
+ *   - VISCOF_H's closure at boundary nodes (ISONB != 0);
+ *   - VDIF_Q's stable-side STF factor;
+ *   - which side's ART enters the edge routines' Smagorinsky viscosity;
+ *   - the DZZ weights of FCT_Q2/FCT_Q2L's vertical bounds on unequal
+ *     spacing.
  */
 #include "../include/fvgpu.h"
 #include "../include/fv_detmath.h"

@@ -679,6 +679,87 @@ def test_adv_uv_momentum_terms(built):
         assert np.max(np.abs(vf[k, e] - ev)) < 1e-12, k
 
 
+@pytest.mark.parametrize("routine", ["advave", "advection", "adv_uv"])
+@pytest.mark.parametrize("closure", [0, 1])
+def test_edge_viscosity_closed_forms(built, routine, closure):
+    """The horizontal viscosity of the three edge routines (ADVAVE_EDGE_GCN
+    advave_edge_gcn.F:166-320, ADVECTION_EDGE_GCN advection_edge_gcn.F,
+    ADV_UV_EDGE_GCN adv_uv_edge_gcn.F), isolated as the difference between
+    CC_HVC = 0.3 and CC_HVC = 0 (the advective terms do not depend on it),
+    for the shear flow u = a_k y^2, v = 0 (a_k per layer) on a uniform flat
+    mesh, HPRNU = 2, at elements with interior neighbours:
+      constant (HORIZONTAL_CLOSURE off): nu = CC/HPRNU and the viscous
+        acceleration is nu lap(u) = 2 a nu, exactly (the least-squares
+        gradients of a quadratic are exact in the mean of the two sides);
+      Smagorinsky (on): nu = CC ART |S| / HPRNU with |S| = |u_y|/sqrt2
+        = sqrt2 a y, so d/dy(nu u_y) = 4 sqrt2 CC ART a^2 y / HPRNU, within
+        the scheme's discretisation error (here 6 %, 0.3 % on average).
+    The y-force of the scheme alternates between neighbouring elements but
+    sums to zero over them.  Terms: ADVUA = -D ART acc, ADVX = -D DZ_k ART
+    acc, UF gains DTI acc D/D1."""
+    c = F.Model.build(0, nx=21, ny=7, kb=6, river=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    D, D1, CC, HPRNU = 12.0, 12.4, 0.3, 2.0
+    p = F.Model.case_defaults(0).p
+    p.horizontal_closure = closure
+    p.hprnu = HPRNU
+    m = F.Model.from_mesh(x, y + 20000.0, nv, np.full(c.m, D), 6, [], p)
+    m.cold_start()
+    m.field("cor")[:] = 0.0
+    kb, n1, m1 = m.kb, m.n + 1, m.m + 1
+    dti = m.hdr.p.dte * m.hdr.p.isplit
+    yc, art, dz1 = m.field("yc"), m.field("art"), m.field("dz1").reshape(kb, n1)
+    assert np.ptp(art[1:]) == 0.0
+    a = (1e-9 if closure else 1e-7) * (1 + np.arange(kb - 1))
+    out = {}
+    for cc in (0.0, CC):
+        m.field("cc_hvc")[:] = cc
+        o = Oracle(m)
+        if routine == "advave":
+            ua = a[0] * yc ** 2
+            ua[0] = 0.0
+            for nm, arr in (("ua", ua), ("va", np.zeros(n1)), ("d", np.full(m1, D)), ("d1", np.full(n1, D)),
+                            ("el", np.zeros(m1))):
+                o.put(nm, arr)
+            o.routine("advave")
+            out[cc] = (o.get("advua")[None, :], o.get("advva")[None, :])
+            continue
+        u = np.zeros((kb, n1))
+        u[:-1, 1:] = a[:, None] * yc[None, 1:] ** 2
+        arrs = [("u", u.ravel()), ("v", np.zeros(kb * n1)), ("dt", np.full(m1, D))]
+        if routine == "adv_uv":
+            arrs += [("w", np.zeros(kb * n1)), ("dt1", np.full(n1, D)), ("d1", np.full(n1, D1)),
+                     ("egf", np.zeros(m1)), ("drhox", np.zeros(kb * n1)), ("drhoy", np.zeros(kb * n1))]
+        for nm, arr in arrs:
+            o.put(nm, arr)
+        o.routine(routine)
+        names = ("advx", "advy") if routine == "advection" else ("uf", "vf")
+        out[cc] = tuple(o.get(nm).reshape(kb, n1)[:-1] for nm in names)
+    du, dv = out[CC][0] - out[0.0][0], out[CC][1] - out[0.0][1]
+    isbce, nbe = m.field("isbce"), m.field("nbe").reshape(3, n1)
+    e = np.array([i for i in range(1, n1) if isbce[i] == 0 and all(
+        nbe[j, i] != 0 and isbce[nbe[j, i]] == 0 for j in range(3))])
+    assert len(e) > 100
+    for k in range(du.shape[0]):
+        if closure:
+            acc = 4 * np.sqrt(2) * CC * art[e] * a[k] ** 2 * yc[e] / HPRNU
+        else:
+            acc = 2 * a[k] * CC / HPRNU
+        if routine == "advave":
+            want = -acc * D * art[e]
+        elif routine == "advection":
+            want = -acc * D * dz1[k, e] * art[e]
+        else:
+            want = dti * acc * D / D1
+        r = du[k, e] / want
+        if closure:
+            assert abs(r.mean() - 1) < 0.01 and np.all(np.abs(r - 1) < 0.08), (k, r.min(), r.max(), r.mean())
+        else:
+            assert np.max(np.abs(r - 1)) < 1e-9, (k, r.min(), r.max())
+        assert abs(dv[k, e].sum()) < 1e-9 * np.abs(du[k, e]).sum(), k
+
+
 def test_bcond_ts_river_values_and_local_bounds(built):
     """BCOND_TS (bcond_ts.F:60-95, 96-160): with the river's T and S given,
     every layer of the river node takes them (both tracers in one call, as
