@@ -440,6 +440,10 @@ void enqueue_internal_step_t(fvgpu_handle* h) {
     // the same kernel, the elimination coefficients in `vhpair`.
     if (g.ve_max <= 10) FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL);
     else FV_KMAX_DISPATCH(g.KBM1, VDIF_Q_CALL16);
+  } else {
+    // VERTICAL_MIXING_TYPE 'constant'
+    const double umol = h->md->p.umol;
+    TLAUNCH("const_mixing", k_const_mixing, mk, a.km, a.kh, umol, umol * h->md->p.vprnu, mk);
   }
   N2E3D(a.km, a.km1);
   // ADV_T .. T1 = TF1 on the main stream, the same for S on stream2 (own

@@ -58,6 +58,16 @@ __global__ void k_zero(double* p, long n) {
   if (i < n) p[i] = 0.0;
 }
 
+// VERTICAL_MIXING_TYPE 'constant' (internal_step.F:1042-1045): KM = UMOL
+// and KH = UMOL*VPRNU (formed on the host) at every node and level.
+__global__ void k_const_mixing(double* km, double* kh, double umol, double kh_value, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    km[i] = umol;
+    kh[i] = kh_value;
+  }
+}
+
 __global__ void k_zero_int(int* p, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

@@ -308,6 +308,43 @@ def test_calculated_river_parity(built, graph):
     assert np.all(t1[:-1, j] >= 24.0)
 
 
+_SWITCHES = [
+    (0, {"p_horizontal_closure": 0}),          # constant horizontal viscosity (FM1 branch of every edge routine)
+    (0, {"p_vertical_closure": 0}),            # no MY2.5: KM = KH = UMOL, no ADV_Q/VDIF_Q
+    (0, {"p_scalar_positivity": 0}),           # FCT off in ADV_T/ADV_S and ADV_Q
+    (0, {"p_noflux_bot": 0}),
+    (0, {"p_obc_elevation_on": 0}),            # no tide at the open boundary
+    (0, {"p_horizontal_closure": 0, "p_vertical_closure": 0, "p_scalar_positivity": 0}),
+    (1, {"p_temperature_active": 0}),          # T frozen, S advected (one tracer pass)
+    (1, {"p_salinity_active": 0}),
+    (1, {"p_barotropic": 1}),                  # no BAROPG density term, no DENS2
+]
+
+
+@pytest.mark.parametrize("config,switches", _SWITCHES,
+                         ids=["-".join("%s=%d" % (k[2:], v) for k, v in s.items()) for _, s in _SWITCHES])
+def test_physics_switches_parity(built, config, switches):
+    """The step's run-time switches (the namelist's HORIZONTAL_MIXING_TYPE,
+    VERTICAL_MIXING_TYPE, SCALAR_POSITIVITY_CONTROL, NOFLUX_BOT_CONDITION,
+    OBC_ELEVATION_FORCING_ON, TEMPERATURE_ACTIVE, SALINITY_ACTIVE,
+    BAROTROPIC): each changes the state against the default on these cases
+    within six steps, and the device follows the oracle bit for bit."""
+    import myfvcomcode_amd as F
+    from oracle import Oracle
+    kw = {} if config == 0 else {"nx": 41, "ny": 21, "kb": 9, "n_seamounts": 2}
+    m = F.Model.build(config, **kw, **switches)
+    dev, orc = F.DeviceModel(m), Oracle(m)
+    n = 6
+    print("case: config %d %s, %s, %d steps (graph)" % (config, kw or "81x21 channel", switches, n), flush=True)
+    dev.run(n, graph=True)
+    orc.run(n)
+    rep = {}
+    worst, bad = compare(dev, orc, m, NODE2 + ELEM2 + ELEM3 + NODE3, 0.0, rep)
+    d = dev.diagnostics()
+    assert d.nonfinite == 0 and d.depth_fail == 0
+    assert not bad, bad
+
+
 def test_nonfinite_state_is_reported(built):
     """A NaN written into UA at one element: the next internal step spreads
     it and fvgpu_diagnostics reports a non-finite state, as the oracle's
