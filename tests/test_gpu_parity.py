@@ -318,17 +318,24 @@ _SWITCHES = [
     (1, {"p_temperature_active": 0}),          # T frozen, S advected (one tracer pass)
     (1, {"p_salinity_active": 0}),
     (1, {"p_barotropic": 1}),                  # no BAROPG density term, no DENS2
+    (1, {"p_type_tsobc": 1}),                  # T/S at outflow OBC nodes: upstream, no radiation
+    (1, {"p_type_tsobc": 2}),                  # radiation without relaxation
+    (0, {"p_hprnu": 2.5, "p_vprnu": 0.7, "p_umol": 3e-4}),   # Prandtl numbers off 1: KH != KM
+    (0, {"p_hprnu": 2.5, "p_vprnu": 0.7, "p_umol": 3e-4, "p_vertical_closure": 0}),
+    (0, {"p_cbcmin": 0.01}),                   # the drag coefficient's floor
 ]
 
 
 @pytest.mark.parametrize("config,switches", _SWITCHES,
-                         ids=["-".join("%s=%d" % (k[2:], v) for k, v in s.items()) for _, s in _SWITCHES])
+                         ids=["-".join("%s=%g" % (k[2:], v) for k, v in s.items()) for _, s in _SWITCHES])
 def test_physics_switches_parity(built, config, switches):
     """The step's run-time switches (the namelist's HORIZONTAL_MIXING_TYPE,
     VERTICAL_MIXING_TYPE, SCALAR_POSITIVITY_CONTROL, NOFLUX_BOT_CONDITION,
     OBC_ELEVATION_FORCING_ON, TEMPERATURE_ACTIVE, SALINITY_ACTIVE,
-    BAROTROPIC): each changes the state against the default on these cases
-    within six steps, and the device follows the oracle bit for bit."""
+    BAROTROPIC, TYPE_TSOBC) and parameters away from their defaults (HPRNU,
+    VPRNU, UMOL, CBCMIN): each changes the state against the default on
+    these cases within six steps, and the device follows the oracle bit for
+    bit."""
     import myfvcomcode_amd as F
     from oracle import Oracle
     kw = {} if config == 0 else {"nx": 41, "ny": 21, "kb": 9, "n_seamounts": 2}
