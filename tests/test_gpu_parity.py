@@ -320,6 +320,7 @@ _SWITCHES = [
     (1, {"p_barotropic": 1}),                  # no BAROPG density term, no DENS2
     (1, {"p_type_tsobc": 1}),                  # T/S at outflow OBC nodes: upstream, no radiation
     (1, {"p_type_tsobc": 2}),                  # radiation without relaxation
+    (1, {"p_obc_on": 0}),                      # BCOND_TS leaves the OBC nodes' T/S to advection
     (0, {"p_hprnu": 2.5, "p_vprnu": 0.7, "p_umol": 3e-4}),   # Prandtl numbers off 1: KH != KM
     (0, {"p_hprnu": 2.5, "p_vprnu": 0.7, "p_umol": 3e-4, "p_vertical_closure": 0}),
     (0, {"p_cbcmin": 0.01}),                   # the drag coefficient's floor
@@ -332,7 +333,7 @@ def test_physics_switches_parity(built, config, switches):
     """The step's run-time switches (the namelist's HORIZONTAL_MIXING_TYPE,
     VERTICAL_MIXING_TYPE, SCALAR_POSITIVITY_CONTROL, NOFLUX_BOT_CONDITION,
     OBC_ELEVATION_FORCING_ON, TEMPERATURE_ACTIVE, SALINITY_ACTIVE,
-    BAROTROPIC, TYPE_TSOBC) and parameters away from their defaults (HPRNU,
+    BAROTROPIC, TYPE_TSOBC, OBC_ON) and parameters away from their defaults (HPRNU,
     VPRNU, UMOL, CBCMIN): each changes the state against the default on
     these cases within six steps, and the device follows the oracle bit for
     bit."""
