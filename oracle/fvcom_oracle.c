@@ -28,7 +28,6 @@
  *     inside the range);
  *   - VISCOF_H's closure at boundary nodes (ISONB != 0);
  *   - VDIF_Q's stable-side STF factor;
- *   - which side's ART enters the edge routines' Smagorinsky viscosity;
  *   - the DZZ weights of FCT_Q2/FCT_Q2L's vertical bounds on unequal
  *     spacing.
  */

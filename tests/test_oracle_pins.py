@@ -760,6 +760,64 @@ def test_edge_viscosity_closed_forms(built, routine, closure):
         assert abs(dv[k, e].sum()) < 1e-9 * np.abs(du[k, e]).sum(), k
 
 
+@pytest.mark.parametrize("routine", ["advave", "advection", "adv_uv"])
+def test_edge_viscosity_independent_of_numbering(built, routine):
+    """The Smagorinsky viscosity of an edge averages ART |S| CC of its two
+    elements, so the viscous terms cannot depend on which element is the
+    edge's IA: numbering the elements of a jittered mesh (areas varying by
+    over 100 %) in reverse order swaps IA and IB on every interior edge and
+    leaves them unchanged up to round-off.  (A term taking one side's area
+    twice changes them by percents.)"""
+    c = F.Model.build(1, nx=31, ny=17, kb=6, n_seamounts=0)
+    x, y = c.field("vx")[1:].copy(), c.field("vy")[1:].copy()
+    nv = c.field("nv").reshape(3, c.n + 1)[:, 1:].T.copy()
+    p = F.Model.case_defaults(0).p
+    D = 12.0
+
+    def viscous(order):
+        m = F.Model.from_mesh(x, y, nv[order], np.full(c.m, D), 6, [], p)
+        m.cold_start()
+        m.field("cor")[:] = 0.0
+        kb, n1, m1 = m.kb, m.n + 1, m.m + 1
+        xc, yc = m.field("xc"), m.field("yc")
+        ua = 0.3 * np.sin(2 * np.pi * yc / y.max()) + 0.1 * np.cos(3 * xc / x.max())
+        va = 0.2 * np.cos(2 * np.pi * xc / x.max()) * np.sin(yc / y.max())
+        ua[0] = va[0] = 0.0
+        out = {}
+        for cc in (0.0, 0.3):
+            m.field("cc_hvc")[:] = cc
+            o = Oracle(m)
+            if routine == "advave":
+                for nm, arr in (("ua", ua), ("va", va), ("d", np.full(m1, D)), ("d1", np.full(n1, D)),
+                                ("el", np.zeros(m1))):
+                    o.put(nm, arr)
+                o.routine("advave")
+                out[cc] = np.stack([o.get("advua")[1:], o.get("advva")[1:]])
+                continue
+            u, v = np.zeros((kb, n1)), np.zeros((kb, n1))
+            u[:-1] = ua[None, :] * (1 + 0.1 * np.arange(kb - 1))[:, None]
+            v[:-1] = va[None, :]
+            arrs = [("u", u.ravel()), ("v", v.ravel()), ("dt", np.full(m1, D))]
+            if routine == "adv_uv":
+                arrs += [("w", np.zeros(kb * n1)), ("dt1", np.full(n1, D)), ("d1", np.full(n1, D)),
+                         ("egf", np.zeros(m1)), ("drhox", np.zeros(kb * n1)), ("drhoy", np.zeros(kb * n1))]
+            for nm, arr in arrs:
+                o.put(nm, arr)
+            o.routine(routine)
+            names = ("advx", "advy") if routine == "advection" else ("uf", "vf")
+            out[cc] = np.stack([o.get(nm).reshape(kb, n1)[:-1, 1:] for nm in names])
+        art = m.field("art")[1:]
+        return out[0.3] - out[0.0], np.ptp(art) / art.mean()
+
+    n = nv.shape[0]
+    fwd, spread = viscous(np.arange(n))
+    rev, _ = viscous(np.arange(n)[::-1])
+    rev = rev[..., ::-1]
+    assert spread > 1.0
+    assert np.abs(fwd).max() > 0.0
+    assert np.abs(fwd - rev).max() < 1e-12 * np.abs(fwd).max()
+
+
 def test_bcond_ts_river_values_and_local_bounds(built):
     """BCOND_TS (bcond_ts.F:60-95, 96-160): with the river's T and S given,
     every layer of the river node takes them (both tracers in one call, as
